@@ -1,0 +1,226 @@
+"""Benchmark: diffusion steps/s on the Im-3m 128^3 SCFT propagator sweep.
+
+Workload (one GPU): space group Im-3m (#229), 128^3 grid, cubic cell 4.0, AB
+diblock f = 0.3, chiN = 20, Ns = 100 contour steps (ds = 0.01) for q and q+
+batched together; w_A = +3 S(r) + noise, w_B = -3 S(r) + noise with S the BCC
+{110} first-star cosine sum and seeded N(0, 0.05) noise, symmetrised over the
+96 operations.  One "step" = one contour step of the batch (q and q+ each take
+one Strang step q <- e^{-w ds/2} D e^{-w ds/2} q through the centred KRFFT
+pipeline); `value` counts field steps (2 per contour step) per second.
+
+Prints one JSON line (rank 0).  --gpus above 1 exits with a message: this
+build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "crystallographicfft.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+N = (128, 128, 128)
+SG = 229
+CELL = 4.0
+F_A = 0.3
+CHIN = 20.0
+NS = 100
+HBM_PEAK_GBS = 8000.0
+
+
+def make_fields(ops_s, shift, Lv, seed=2024):
+    from cfft_hip.scft import first_star_bcc, symmetrize_subgrid
+    b = [s * n for s, n in zip(shift, N)]
+    rng = np.random.default_rng(seed)
+    S = first_star_bcc(N, b, CELL)
+    sub = tuple(slice(0, None, int(l)) for l in Lv)
+    S0 = np.ascontiguousarray(S[sub])
+    nA = symmetrize_subgrid(rng.normal(0.0, 0.05, N), ops_s, Lv)
+    nB = symmetrize_subgrid(rng.normal(0.0, 0.05, N), ops_s, Lv)
+    return 3.0 * S0 + nA, -3.0 * S0 + nB
+
+
+def stage_bytes(plan, e):
+    """Algorithmic HBM bytes per launch of each stage (tables excluded):
+    each distinct input array read once, each output written once."""
+    B = plan.batch
+    M3 = int(np.prod(plan.M))
+    chan = plan.n_ch * int(np.prod(plan.H)) * 2 * e
+    spec = plan.n_spec * 2 * e
+    return {
+        "fold": B * (2 * M3 * e + chan),            # f0, E -> channels
+        "fft_x": B * 2 * chan, "fft_y": B * 2 * chan, "fft_z": B * 2 * chan,
+        "recon": B * (chan + spec),                  # channels -> F (+K)
+        "irecon": B * (spec + chan),                 # F -> channels
+        "ifft_x": B * 2 * chan, "ifft_y": B * 2 * chan, "ifft_z": B * 2 * chan,
+        "unfold": B * (chan + 2 * M3 * e),           # channels, E -> f0
+        "qmul": B * 2 * chan,
+    }
+
+
+def step_bytes(plan, e):
+    """SURVEY-style algorithmic bytes of one Strang step per field:
+    4 M^3 e + 8 n_ch H^3 2e + 2 n_spec 2e."""
+    M3 = int(np.prod(plan.M))
+    return 4 * M3 * e + 8 * plan.n_ch * int(np.prod(plan.H)) * 2 * e + 2 * plan.n_spec * 2 * e
+
+
+def time_sweeps(sw, steps, warmup, torch):
+    """Wall time of `steps` contour steps of the batch (a sweep restarts from
+    q = 1 every Ns steps), after `warmup` untimed full sweeps."""
+    for _ in range(warmup):
+        sw.reset()
+        sw.run()
+    sw.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        if done > 0 and done % NS == 0:
+            sw.reset()
+        n = min(NS - done % NS, steps - done)
+        sw.run(n, start=done % NS)
+        done += n
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def profile_stages(sw, steps, torch):
+    import cfft_hip._lib as L
+    import ctypes as C
+    lib = L.lib()
+    h = sw.plan.handle
+    L.check(lib.cfft_profile(h, 1))
+    sw.reset()
+    sw.run(min(steps, NS))
+    torch.cuda.synchronize()
+    ms = np.zeros(len(L.STAGES))
+    cnt = np.zeros(len(L.STAGES), dtype=np.int64)
+    L.check(lib.cfft_profile_read(h, ms.ctypes.data_as(L.PD), cnt.ctypes.data_as(L.PI64)))
+    L.check(lib.cfft_profile(h, 0))
+    return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(L.STAGES) if cnt[i]}
+
+
+def cpu_baseline(ops_s, wA, Lv, budget_s):
+    """The oracle (NumPy FP64, M2 route, one core) on a bounded sample of the
+    same workload: Strang steps of q along the A block."""
+    import oracle as O
+    import cfft_hip as cf
+    R, t = cf.ops_arrays(ops_s)
+    t0 = time.perf_counter()
+    spec = O.calc_spectral_asu(R, t, N)
+    table = O.m2_backward_table(spec, R, t, N)
+    lat = CELL * np.eye(3)
+    ds = 1.0 / NS
+    setup = time.perf_counter() - t0
+    q = np.ones(tuple(n // l for n, l in zip(N, Lv)))
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        q = O.strang_step(q, wA, ds, lambda f: O.m2_scft(f, spec, R, t, N, ds, lat, table))
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    return n / el, n, setup, q
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus > 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import cfft_hip as cf
+
+    torch.cuda.set_device(0)
+    ops = cf.get_ops(SG, 3, N)
+    shift, ops_s = cf.find_optimal_shift(ops, N)
+    Lv = cf.auto_L(ops_s)
+    wA, wB = make_fields(ops_s, shift, Lv)
+    lat = CELL * np.eye(3)
+
+    res = {}
+    for name, prec, e in (("fp64", cf.FP64, 8), ("mixed", cf.MIXED, 4)):
+        sw = cf.scft.PropagatorSweep(N, ops_s, lat, F_A, NS, prec)
+        sw.set_fields(wA, wB)
+        # full sweep once for Q
+        sw.reset()
+        sw.run()
+        torch.cuda.synchronize()
+        Qq, Qd = sw.Q()
+        qfields = sw.fields()
+        el = time_sweeps(sw, args.steps, args.warmup, torch)
+        stages = profile_stages(sw, args.steps, torch)
+        sb = stage_bytes(sw.plan, e)
+        dom = max(stages, key=lambda k: stages[k][0])
+        avg_ms = stages[dom][0] / stages[dom][1]
+        achieved = sb[dom] / (avg_ms * 1e-3) / 1e9
+        res[name] = dict(el=el, Qq=Qq, Qd=Qd, q=qfields, stages=stages, dom=dom, avg_ms=avg_ms,
+                         achieved=achieved, plan=sw.plan, sbytes=sb[dom],
+                         step_bytes=step_bytes(sw.plan, e))
+        del sw
+
+    r64, r32 = res["fp64"], res["mixed"]
+    K = args.steps
+    value = 2 * K / r64["el"]
+    mixed_value = 2 * K / r32["el"]
+    dq = abs(r32["Qq"] - r64["Qq"])
+    qerr = max(float(np.max(np.abs(r32["q"][b] - r64["q"][b]) / np.max(np.abs(r64["q"][b]))))
+               for b in range(2))
+
+    cpu = None
+    if not args.no_cpu:
+        rate, n_cpu, setup, qc = cpu_baseline(ops_s, wA, Lv, args.cpu_seconds)
+        cpu = {"value": rate, "unit": "diffusion steps/s", "cores": 1, "kind": "port",
+               "sample": f"{n_cpu} Strang steps of q on the A block, oracle M2 route "
+                         f"(NumPy FP64 + pocketfft), plan setup {setup:.1f}s excluded"}
+
+    line = {
+        "metric": "diffusion steps/s 128^3 Im-3m",
+        "value": value,
+        "unit": "diffusion steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": r64["el"] / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded BCC first-star w fields + symmetrised noise)",
+        "config": {"workload": "Im-3m #229 128^3 cell 4.0 AB diblock f=0.3 chiN=20 Ns=100, "
+                               "q and q+ batched (centred KRFFT M7 path)",
+                   "grid": list(N), "space_group": SG, "batch_fields": 2, "contour_steps": NS},
+        "roofline": {"bound": "hbm", "kernel": r64["dom"], "achieved": r64["achieved"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_launch": r64["sbytes"],
+                     "avg_launch_ms": r64["avg_ms"]},
+        "mixed": {"value": mixed_value, "dtype": "f32 data/FFT/propagator, f64 tables+reductions",
+                  "speedup_vs_fp64": mixed_value / value, "abs_dQ_vs_fp64": dq,
+                  "max_rel_q_err_vs_fp64": qerr,
+                  "roofline": {"kernel": r32["dom"], "achieved": r32["achieved"],
+                               "frac": r32["achieved"] / HBM_PEAK_GBS,
+                               "avg_launch_ms": r32["avg_ms"]}},
+        "Q": {"fp64_q": r64["Qq"], "fp64_qdagger": r64["Qd"], "mixed_q": r32["Qq"]},
+        "stage_ms_per_launch": {k: v[0] / v[1] for k, v in r64["stages"].items()},
+        "step_alg_bytes_fp64": r64["step_bytes"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,140 @@
+"""ctypes binding of libcfft_hip.so (the C ABI declared in include/cfft_hip.h).
+
+The library is loaded from the package directory (built in-tree by
+``crystallographicfft.jl_amd/build.py``).  There is no fallback: when the
+library is missing every entry point raises ``CFFTLibraryMissing``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libcfft_hip.so")
+
+CFFT_OK = 0
+CFFT_ERR_INVALID = 1
+CFFT_ERR_NOT_COMMENSURATE = 2
+CFFT_ERR_NOT_DIVISIBLE = 3
+CFFT_ERR_UNREACHABLE = 4
+CFFT_ERR_FILL_MAP = 5
+CFFT_ERR_CENTERING = 6
+CFFT_ERR_ODD_M = 7
+CFFT_ERR_SIZE = 8
+CFFT_ERR_UNKNOWN_METHOD = 9
+CFFT_ERR_BOUNDS = 10
+CFFT_ERR_UNSUPPORTED = 11
+CFFT_ERR_HIP = 12
+
+PATH_AUTO, PATH_M2, PATH_M7, PATH_M2Q, PATH_M7Q = 0, 1, 2, 3, 4
+FP64, MIXED = 0, 1
+
+EXPORTED = [
+    "cfft_last_error", "cfft_find_optimal_shift", "cfft_count_special_positions",
+    "cfft_calc_spectral_asu", "cfft_auto_L", "cfft_select_rep_ops", "cfft_detect_centering",
+    "cfft_fill_map", "cfft_plan_create", "cfft_plan_info", "cfft_plan_tables",
+    "cfft_update_kernel", "cfft_plan_destroy", "cfft_forward", "cfft_backward",
+    "cfft_forward_g0", "cfft_diffuse", "cfft_step", "cfft_sub_to_full", "cfft_full_to_sub",
+    "cfft_forward_dev", "cfft_backward_dev", "cfft_diffuse_dev", "cfft_step_dev",
+    "cfft_exp_dev", "cfft_sync", "cfft_step_ptrs_dev", "cfft_profile", "cfft_profile_read",
+]
+
+STAGES = ["fold", "fft_x", "fft_y", "fft_z", "recon", "irecon", "ifft_x", "ifft_y", "ifft_z",
+          "unfold", "qmul"]
+
+
+class CFFTError(RuntimeError):
+    """A non-zero status from the C ABI (the reference's error()/@assert)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class CFFTLibraryMissing(RuntimeError):
+    pass
+
+
+_lib = None
+
+P = C.c_void_p
+I32 = C.c_int32
+I64 = C.c_int64
+D = C.c_double
+PI32 = C.POINTER(C.c_int32)
+PI64 = C.POINTER(C.c_int64)
+PD = C.POINTER(C.c_double)
+
+
+def _declare(lib):
+    sig = {
+        "cfft_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
+        "cfft_find_optimal_shift": (C.c_int, [I32, PI32, PI32, I32, PD, PI32]),
+        "cfft_count_special_positions": (C.c_int, [I32, PI32, PI32, I32, PI64]),
+        "cfft_calc_spectral_asu": (C.c_int, [I32, PI32, PI32, I32, PI32, PI32, I64, PI64]),
+        "cfft_auto_L": (C.c_int, [I32, PI32, I32, PI32]),
+        "cfft_select_rep_ops": (C.c_int, [I32, PI32, PI32, I32, PI32, PI32]),
+        "cfft_detect_centering": (C.c_int, [I32, PI32, PI32, I32, PI32]),
+        "cfft_fill_map": (C.c_int, [I32, PI32, PI32, I32, PI32, PI32]),
+        "cfft_plan_create": (C.c_int, [PI32, PI32, I32, D, PD, I32, I32, I32, C.POINTER(P)]),
+        "cfft_plan_info": (C.c_int, [P, PI32]),
+        "cfft_plan_tables": (C.c_int, [P, PI32, PI32, PD, PI32]),
+        "cfft_update_kernel": (C.c_int, [P, D, PD]),
+        "cfft_plan_destroy": (C.c_int, [P]),
+        "cfft_forward": (C.c_int, [P, PD, PD]),
+        "cfft_backward": (C.c_int, [P, PD, PD]),
+        "cfft_forward_g0": (C.c_int, [P, PD, PD]),
+        "cfft_diffuse": (C.c_int, [P, PD]),
+        "cfft_step": (C.c_int, [P, PD, PD, D, I32]),
+        "cfft_sub_to_full": (C.c_int, [P, PD, PD]),
+        "cfft_full_to_sub": (C.c_int, [P, PD, PD]),
+        "cfft_forward_dev": (C.c_int, [P, P, P, P]),
+        "cfft_backward_dev": (C.c_int, [P, P, P, P]),
+        "cfft_diffuse_dev": (C.c_int, [P, P, P]),
+        "cfft_step_dev": (C.c_int, [P, P, P, I32, P]),
+        "cfft_exp_dev": (C.c_int, [P, P, P, D, I64, P]),
+        "cfft_sync": (C.c_int, [P]),
+        "cfft_step_ptrs_dev": (C.c_int, [P, P, C.POINTER(P), I32, P]),
+        "cfft_profile": (C.c_int, [P, I32]),
+        "cfft_profile_read": (C.c_int, [P, PD, PI64]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def lib():
+    """Load (once) and return the library; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CFFTLibraryMissing(
+                f"{LIB_PATH} not found: build it with `python crystallographicfft.jl_amd/build.py`")
+        l = C.CDLL(LIB_PATH)
+        _declare(l)
+        _lib = l
+    return _lib
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(4096)
+    lib().cfft_last_error(buf, 4096)
+    return buf.value.decode(errors="replace")
+
+
+def check(status: int):
+    if status != CFFT_OK:
+        raise CFFTError(status, last_error())
+
+
+def i32p(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(PI32)
+
+
+def f64p(a: np.ndarray):
+    assert a.dtype == np.float64 and (a.flags.c_contiguous or a.flags.f_contiguous)
+    return a.ctypes.data_as(PD)

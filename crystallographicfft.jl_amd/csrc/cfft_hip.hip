@@ -1,0 +1,1001 @@
+// C ABI of libcfft_hip.so: plan objects with device-resident tables and
+// buffers, and the launch sequences of the KRFFT / diffusion pipelines.
+// The entry points and their reference counterparts are listed in
+// include/cfft_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cfft_hip.h"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+using namespace cfft;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    throw Error{CFFT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return CFFT_OK;
+  } catch (const Error& e) {
+    g_last_error = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return CFFT_ERR_HIP;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return CFFT_ERR_INVALID;
+  }
+}
+
+void require(bool ok, int code, const std::string& msg) {
+  if (!ok) throw Error{code, msg};
+}
+
+// exp(2 pi i j / P), argument reduced in extended precision.
+void cis2pi(long long j, long long P, double& re, double& im) {
+  long long r = ((j % P) + P) % P;
+  long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)r / (long double)P;
+  re = (double)cosl(a);
+  im = (double)sinl(a);
+  // exact values on the axes
+  if (4 * r == 0) { re = 1.0; im = 0.0; }
+  else if (4 * r == P) { re = 0.0; im = 1.0; }
+  else if (4 * r == 2 * P) { re = -1.0; im = 0.0; }
+  else if (4 * r == 3 * P) { re = 0.0; im = -1.0; }
+}
+
+constexpr int kFftLens[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64};
+bool fft_supported(int n) {
+  for (int v : kFftLens)
+    if (v == n) return true;
+  return false;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void alloc(size_t n) {
+    release();
+    if (n == 0) return;
+    hip_check(hipMalloc(&p, n), "hipMalloc");
+    bytes = n;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+}  // namespace
+
+struct cfft_plan {
+  int N[3], M[3], L[3], H[3];
+  int d = 1, n_spec = 0, cent = 0, n_ch = 0, path = 0, prec = 0, batch = 1, P = 1;
+  double ds = 0.0, lattice[9];
+  std::vector<Op> ops;
+  std::vector<int> rep;
+  SpecASU spec;
+  std::vector<double> K;
+  std::vector<std::array<int, 3>> offs;
+  int chmap[8];
+  long H3 = 0, M3 = 0, n_alive = 0, N3 = 0;
+  size_t esz = 8;  // element size of the plan's real type
+  hipStream_t stream = nullptr;
+  // device tables
+  DevBuf d_hs, d_K, d_W, d_twM, d_twF[3], d_btab, d_Q, d_gidx, d_fmap, d_chmap;
+  // device work buffers
+  DevBuf d_f0, d_ch, d_ch2, d_F, d_E, d_u;
+  std::vector<int32_t> fmap_host;
+  // per-stage hipEvent timer (cfft_profile / cfft_profile_read)
+  bool prof = false;
+  struct Mark {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;
+  ~cfft_plan() {
+    for (auto& m : marks) {
+      (void)hipEventDestroy(m.a);
+      (void)hipEventDestroy(m.b);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+bool is_q_path(const cfft_plan* p) { return p->path == CFFT_PATH_M2Q || p->path == CFFT_PATH_M7Q; }
+
+template <typename T>
+void upload_complex(DevBuf& buf, const std::vector<double>& re, const std::vector<double>& im) {
+  std::vector<cx<T>> h(re.size());
+  for (size_t i = 0; i < re.size(); ++i) h[i] = {(T)re[i], (T)im[i]};
+  buf.alloc(h.size() * sizeof(cx<T>));
+  if (!h.empty()) hip_check(hipMemcpy(buf.p, h.data(), buf.bytes, hipMemcpyHostToDevice), "upload");
+}
+
+template <typename T>
+void upload_real(DevBuf& buf, const std::vector<double>& v) {
+  std::vector<T> h(v.begin(), v.end());
+  buf.alloc(h.size() * sizeof(T));
+  if (!h.empty()) hip_check(hipMemcpy(buf.p, h.data(), buf.bytes, hipMemcpyHostToDevice), "upload");
+}
+
+void upload_bytes(DevBuf& buf, const void* src, size_t n) {
+  buf.alloc(n);
+  if (n) hip_check(hipMemcpy(buf.p, src, n, hipMemcpyHostToDevice), "upload");
+}
+
+// Phase table W[j] = exp(2 pi i j/P) and fold / FFT twiddle tables.
+template <typename T>
+void build_tables(cfft_plan* p) {
+  std::vector<double> re, im;
+  re.resize(p->P);
+  im.resize(p->P);
+  for (int j = 0; j < p->P; ++j) cis2pi(j, p->P, re[j], im[j]);
+  upload_complex<T>(p->d_W, re, im);
+  // twM: exp(-2 pi i n / M_d), n < M_d, d = 0..2 concatenated
+  re.clear();
+  im.clear();
+  for (int d = 0; d < 3; ++d)
+    for (int n = 0; n < p->M[d]; ++n) {
+      double a, b;
+      cis2pi(-n, p->M[d], a, b);
+      re.push_back(a);
+      im.push_back(b);
+    }
+  upload_complex<T>(p->d_twM, re, im);
+  for (int d = 0; d < 3; ++d) {
+    std::vector<double> fr(p->H[d]), fi(p->H[d]);
+    for (int m = 0; m < p->H[d]; ++m) cis2pi(-m, p->H[d], fr[m], fi[m]);
+    upload_complex<T>(p->d_twF[d], fr, fi);
+  }
+  upload_real<T>(p->d_K, p->K);
+}
+
+void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double ds,
+                const double* lattice, int path, int prec, int batch) {
+  require(batch >= 1, CFFT_ERR_INVALID, "batch must be >= 1");
+  require(prec == CFFT_FP64 || prec == CFFT_MIXED, CFFT_ERR_INVALID, "unknown precision");
+  require(path >= CFFT_PATH_AUTO && path <= CFFT_PATH_M7Q, CFFT_ERR_UNKNOWN_METHOD,
+          "Unknown method: path " + std::to_string(path));
+  for (int d = 0; d < 3; ++d) require(N[d] >= 1, CFFT_ERR_INVALID, "grid size must be positive");
+  p->ops = ops;
+  p->ds = ds;
+  std::memcpy(p->lattice, lattice, sizeof(double) * 9);
+  p->prec = prec;
+  p->esz = prec == CFFT_FP64 ? 8 : 4;
+  p->batch = batch;
+  for (int d = 0; d < 3; ++d) p->N[d] = N[d];
+  auto_L(3, ops, p->L);
+  for (int d = 0; d < 3; ++d) {
+    p->M[d] = N[d] / p->L[d];
+    require(p->M[d] * p->L[d] == N[d], CFFT_ERR_NOT_DIVISIBLE,
+            "Grid size N=(" + std::to_string(N[0]) + "," + std::to_string(N[1]) + "," +
+                std::to_string(N[2]) + ") not divisible by auto L.");
+  }
+  p->rep = select_rep_ops(3, ops, p->L, N);
+  p->d = (int)p->rep.size();
+  p->cent = detect_centering(3, N, ops);
+  bool allL2 = p->L[0] == 2 && p->L[1] == 2 && p->L[2] == 2;
+  if (path == CFFT_PATH_AUTO) path = (p->cent != CENT_P && allL2) ? CFFT_PATH_M7 : CFFT_PATH_M2;
+  if (path == CFFT_PATH_M7 || path == CFFT_PATH_M7Q) {
+    require(p->cent != CENT_P, CFFT_ERR_CENTERING,
+            "M7 requires a centred lattice (I/F/C/A); use the M2 path for P-centring.");
+    require(allL2, CFFT_ERR_CENTERING, "M7 requires L = (2,2,2)");
+  }
+  p->path = path;
+  for (int d = 0; d < 3; ++d) {
+    require(p->M[d] % 2 == 0, CFFT_ERR_ODD_M,
+            "Subgrid dimensions M must be even for the radix-2 fold (M" + std::to_string(d) +
+                " = " + std::to_string(p->M[d]) + ")");
+    p->H[d] = p->M[d] / 2;
+    require(fft_supported(p->H[d]), CFFT_ERR_UNSUPPORTED,
+            "FFT length " + std::to_string(p->H[d]) + " not supported (need 2^a 3^b <= 64)");
+  }
+  p->offs = (path == CFFT_PATH_M7 || path == CFFT_PATH_M7Q) ? alive_offsets(p->cent)
+                                                             : alive_offsets(CENT_P);
+  p->n_ch = (int)p->offs.size();
+  for (int i = 0; i < 8; ++i) p->chmap[i] = -1;
+  for (int c = 0; c < p->n_ch; ++c) {
+    const auto& o = p->offs[c];
+    p->chmap[o[0] | (o[1] << 1) | (o[2] << 2)] = c;
+  }
+  p->H3 = 1L * p->H[0] * p->H[1] * p->H[2];
+  p->M3 = 1L * p->M[0] * p->M[1] * p->M[2];
+  p->N3 = 1L * N[0] * N[1] * N[2];
+  p->n_alive = p->n_ch * p->H3;
+  long long P = lcm3(N[0], N[1], N[2]);
+  require(P <= 2048, CFFT_ERR_UNSUPPORTED, "lcm(N) > 2048 not supported by the phase table");
+  p->P = (int)P;
+
+  p->spec = calc_spectral_asu(3, N, ops);
+  p->n_spec = (int)p->spec.idx.size();
+  require(p->n_spec < 0xFFFFF, CFFT_ERR_UNSUPPORTED, "spectral ASU too large");
+  p->K = kernel_spec(p->spec, ds, lattice);
+
+  hip_check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
+  if (prec == CFFT_FP64)
+    build_tables<double>(p);
+  else
+    build_tables<float>(p);
+
+  // signed spectral frequencies (get_k_vector)
+  std::vector<int32_t> hs(3 * (size_t)p->n_spec);
+  for (int i = 0; i < p->n_spec; ++i)
+    for (int d = 0; d < 3; ++d) hs[3 * i + d] = k_signed(p->spec.idx[i][d], N[d]);
+  upload_bytes(p->d_hs, hs.data(), hs.size() * 4);
+  upload_bytes(p->d_chmap, p->chmap, sizeof(p->chmap));
+
+  // alive parity index (c, k) -> q = 2k + off_c
+  auto q_of = [&](long e, int* q) {
+    int c = (int)(e / p->H3);
+    long k = e - c * p->H3;
+    int k0 = (int)(k % p->H[0]), k1 = (int)((k / p->H[0]) % p->H[1]), k2 = (int)(k / (1L * p->H[0] * p->H[1]));
+    q[0] = 2 * k0 + p->offs[c][0];
+    q[1] = 2 * k1 + p->offs[c][1];
+    q[2] = 2 * k2 + p->offs[c][2];
+  };
+  // cosets alpha in Iterators.product order (first dimension fastest)
+  std::vector<std::array<int, 3>> alphas;
+  for (int a2 = 0; a2 < p->L[2]; ++a2)
+    for (int a1 = 0; a1 < p->L[1]; ++a1)
+      for (int a0 = 0; a0 < p->L[0]; ++a0) alphas.push_back({a0, a1, a2});
+
+  if (!is_q_path(p)) {
+    ReverseLookup rl = reverse_lookup(p->spec, ops);
+    std::vector<uint32_t> tab((size_t)p->d * p->n_alive);
+    for (long e = 0; e < p->n_alive; ++e) {
+      int q[3];
+      q_of(e, q);
+      for (int a = 0; a < p->d; ++a) {
+        long h[3];
+        for (int dd = 0; dd < 3; ++dd) h[dd] = q[dd] + 1L * p->M[dd] * alphas[a][dd];
+        long li = h[0] + N[0] * (h[1] + 1L * N[1] * h[2]);
+        uint32_t ent = 0xFFFFFu;
+        if (rl.idx[li] >= 0)
+          ent = (uint32_t)rl.idx[li] | ((uint32_t)rl.j[li] << 20) | ((uint32_t)rl.conj[li] << 31);
+        tab[(size_t)a * p->n_alive + e] = ent;
+      }
+    }
+    upload_bytes(p->d_btab, tab.data(), tab.size() * 4);
+  } else {
+    // Q_b(q) = (1/d) sum_a K(h_a) exp(-2 pi i h_a.t_b/N), h_a = q + M alpha_a
+    // (first row of B^-1 diag(K) B, src/q_fused_krfft.jl:182-413), and the
+    // channel-space gather index of R_b^T q mod M.
+    double A[3][3], c[3][3], B[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) A[i][j] = lattice[i + 3 * j];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+        c[i][j] = A[i1][j1] * A[i2][j2] - A[i1][j2] * A[i2][j1];
+      }
+    double det = A[0][0] * c[0][0] + A[0][1] * c[0][1] + A[0][2] * c[0][2];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) B[i][j] = 2.0 * M_PI * (c[i][j] / det);
+    std::vector<double> qr((size_t)p->d * p->n_alive), qi((size_t)p->d * p->n_alive);
+    std::vector<int32_t> gidx((size_t)p->d * p->n_alive);
+    std::vector<double> Kh(p->d);
+    for (long e = 0; e < p->n_alive; ++e) {
+      int q[3];
+      q_of(e, q);
+      for (int a = 0; a < p->d; ++a) {
+        double hc[3];
+        for (int dd = 0; dd < 3; ++dd)
+          hc[dd] = k_centered(q[dd] + p->M[dd] * alphas[a][dd], N[dd]);
+        double k2 = 0;
+        for (int i = 0; i < 3; ++i) {
+          double kv = B[i][0] * hc[0] + B[i][1] * hc[1] + B[i][2] * hc[2];
+          k2 += kv * kv;
+        }
+        Kh[a] = std::exp(-k2 * ds);
+      }
+      for (int b = 0; b < p->d; ++b) {
+        const Op& g = ops[p->rep[b]];
+        double sr = 0, si = 0;
+        for (int a = 0; a < p->d; ++a) {
+          long long j = 0;
+          for (int dd = 0; dd < 3; ++dd)
+            j += 1LL * (q[dd] + p->M[dd] * alphas[a][dd]) * g.t[dd] * (P / N[dd]);
+          double cr, ci;
+          cis2pi(-j, P, cr, ci);
+          sr += Kh[a] * cr;
+          si += Kh[a] * ci;
+        }
+        qr[(size_t)b * p->n_alive + e] = sr / p->d;
+        qi[(size_t)b * p->n_alive + e] = si / p->d;
+        int rq[3];
+        for (int d1 = 0; d1 < 3; ++d1) {
+          long v = 0;
+          for (int d2 = 0; d2 < 3; ++d2) v += 1L * g.R[d2][d1] * q[d2];
+          rq[d1] = (int)(((v % p->M[d1]) + p->M[d1]) % p->M[d1]);
+        }
+        int cc = p->chmap[(rq[0] & 1) | ((rq[1] & 1) << 1) | ((rq[2] & 1) << 2)];
+        gidx[(size_t)b * p->n_alive + e] =
+            cc < 0 ? -1
+                   : (int32_t)(cc * p->H3 + (rq[0] >> 1) + 1L * p->H[0] * ((rq[1] >> 1) + 1L * p->H[1] * (rq[2] >> 1)));
+      }
+    }
+    if (prec == CFFT_FP64)
+      upload_complex<double>(p->d_Q, qr, qi);
+    else
+      upload_complex<float>(p->d_Q, qr, qi);
+    upload_bytes(p->d_gidx, gidx.data(), gidx.size() * 4);
+  }
+
+  const size_t es = p->esz;
+  p->d_f0.alloc(es * p->M3 * batch);
+  p->d_ch.alloc(2 * es * p->n_alive * batch);
+  if (is_q_path(p)) p->d_ch2.alloc(2 * es * p->n_alive * batch);
+  p->d_F.alloc(2 * es * (size_t)std::max(p->n_spec, 1) * batch);
+  p->d_E.alloc(es * p->M3 * batch);
+}
+
+inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Brackets one kernel launch with hipEvents when the plan's stage timer is on.
+struct StageTimer {
+  cfft_plan* p;
+  hipStream_t s;
+  int stage;
+  hipEvent_t a = nullptr, b = nullptr;
+  StageTimer(cfft_plan* p_, hipStream_t s_, int stage_) : p(p_), s(s_), stage(stage_) {
+    if (!p->prof) return;
+    hip_check(hipEventCreate(&a), "hipEventCreate");
+    hip_check(hipEventCreate(&b), "hipEventCreate");
+    hip_check(hipEventRecord(a, s), "hipEventRecord");
+  }
+  ~StageTimer() {
+    if (!p->prof || !a) return;
+    if (hipEventRecord(b, s) == hipSuccess) p->marks.push_back({stage, a, b});
+  }
+};
+
+template <typename T>
+void launch_check(const char* what) {
+  hip_check(hipGetLastError(), what);
+}
+
+template <typename T, int DIR>
+void fft_axis(cfft_plan* p, cx<T>* data, int axis, hipStream_t s) {
+  const long H0 = p->H[0], H1 = p->H[1], H2 = p->H[2];
+  const long nb = (long)p->batch * p->n_ch;
+  long n_tr, inner, outer, stride;
+  int len = p->H[axis];
+  if (axis == 0) {
+    n_tr = nb * H1 * H2; inner = 1; outer = H0; stride = 1;
+  } else if (axis == 1) {
+    n_tr = nb * H0 * H2; inner = H0; outer = H0 * H1; stride = H0;
+  } else {
+    n_tr = nb * H0 * H1; inner = H0 * H1; outer = H0 * H1 * H2; stride = H0 * H1;
+  }
+  const cx<T>* tw = static_cast<const cx<T>*>(p->d_twF[axis].p);
+  const int bs = 128;
+  if (len == 1) return;
+  StageTimer st(p, s, (DIR < 0 ? CFFT_STAGE_FFT_X : CFFT_STAGE_IFFT_X) + axis);
+  const unsigned g = nblk(n_tr, bs);
+#define CFFT_FFT_CASE(LEN)                                                                          \
+  case LEN:                                                                                         \
+    hipLaunchKernelGGL((k_fft<T, LEN, DIR>), dim3(g), dim3(bs), 0, s, data, n_tr, inner, outer,     \
+                       stride, tw);                                                                 \
+    break;
+  switch (len) {
+    case 1: return;
+    CFFT_FFT_CASE(2)
+    CFFT_FFT_CASE(3)
+    CFFT_FFT_CASE(4)
+    CFFT_FFT_CASE(6)
+    CFFT_FFT_CASE(8)
+    CFFT_FFT_CASE(12)
+    CFFT_FFT_CASE(16)
+    CFFT_FFT_CASE(24)
+    CFFT_FFT_CASE(32)
+    CFFT_FFT_CASE(48)
+    CFFT_FFT_CASE(64)
+    default: throw Error{CFFT_ERR_UNSUPPORTED, "FFT length not supported"};
+  }
+#undef CFFT_FFT_CASE
+  launch_check<T>("k_fft");
+}
+
+template <typename T>
+FoldP fold_params(cfft_plan* p, const T* const* E, bool contig) {
+  FoldP fp{};
+  fp.M0 = p->M[0]; fp.M1 = p->M[1]; fp.M2 = p->M[2];
+  fp.H0 = p->H[0]; fp.H1 = p->H[1]; fp.H2 = p->H[2];
+  fp.n_ch = p->n_ch;
+  for (int c = 0; c < p->n_ch; ++c) fp.offs[c] = p->offs[c][0] | (p->offs[c][1] << 1) | (p->offs[c][2] << 2);
+  fp.batch = p->batch;
+  fp.E_contig = contig ? 1 : 0;
+  for (int b = 0; b < kMaxEPtr; ++b) fp.E[b] = nullptr;
+  if (E) {
+    if (contig)
+      fp.E[0] = E[0];
+    else
+      for (int b = 0; b < p->batch && b < kMaxEPtr; ++b) fp.E[b] = E[b];
+  }
+  return fp;
+}
+
+// f0 (device, plan precision) -> channel spectra in d_ch (forward half up to G0).
+template <typename T>
+void run_fold_fft(cfft_plan* p, const T* f0, const FoldP& fp, hipStream_t s) {
+  cx<T>* ch = static_cast<cx<T>*>(p->d_ch.p);
+  {
+  StageTimer st(p, s, CFFT_STAGE_FOLD);
+  hipLaunchKernelGGL(k_fold<T>, dim3(nblk(p->H3 * p->batch, 256)), dim3(256), 0, s, f0, ch,
+                     static_cast<const cx<T>*>(p->d_twM.p), fp);
+  launch_check<T>("k_fold");
+  }
+  for (int axis = 0; axis < 3; ++axis) fft_axis<T, -1>(p, ch, axis, s);
+}
+
+template <typename T>
+void run_recon(cfft_plan* p, cx<T>* F, bool withK, hipStream_t s) {
+  ReconP rp{};
+  rp.d = p->d;
+  for (int a = 0; a < p->d; ++a) {
+    const Op& g = p->ops[p->rep[a]];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) rp.R[a][i * 3 + j] = g.R[i][j];
+    for (int i = 0; i < 3; ++i) rp.t[a][i] = g.t[i];
+  }
+  for (int d = 0; d < 3; ++d) {
+    rp.M[d] = p->M[d];
+    rp.H[d] = p->H[d];
+    rp.N[d] = p->N[d];
+    rp.Pn[d] = p->P / p->N[d];
+  }
+  rp.P = p->P;
+  for (int i = 0; i < 8; ++i) rp.chmap[i] = p->chmap[i];
+  rp.n_ch = p->n_ch;
+  rp.n_spec = p->n_spec;
+  rp.batch = p->batch;
+  if (p->n_spec == 0) return;
+  StageTimer st(p, s, CFFT_STAGE_RECON);
+  hipLaunchKernelGGL(k_recon<T>, dim3(nblk((long)p->n_spec * p->batch, 256)), dim3(256), 0, s,
+                     static_cast<const cx<T>*>(p->d_ch.p), F, static_cast<const int*>(p->d_hs.p),
+                     withK ? static_cast<const T*>(p->d_K.p) : nullptr,
+                     static_cast<const cx<T>*>(p->d_W.p), rp);
+  launch_check<T>("k_recon");
+}
+
+template <typename T>
+void run_irecon(cfft_plan* p, const cx<T>* F, hipStream_t s) {
+  IreconP ip{};
+  ip.d = p->d;
+  ip.n_alive = p->n_alive;
+  ip.n_spec = p->n_spec;
+  ip.batch = p->batch;
+  StageTimer st(p, s, CFFT_STAGE_IRECON);
+  hipLaunchKernelGGL(k_irecon<T>, dim3(nblk(p->n_alive * p->batch, 256)), dim3(256), 0, s, F,
+                     static_cast<cx<T>*>(p->d_ch.p), static_cast<const uint32_t*>(p->d_btab.p),
+                     static_cast<const cx<T>*>(p->d_W.p), ip, (T)(1.0 / p->d));
+  launch_check<T>("k_irecon");
+}
+
+// channel spectra in `chbuf` -> f0 (inverse FFT + unfold).
+template <typename T>
+void run_ifft_unfold(cfft_plan* p, cx<T>* chbuf, T* f0, const FoldP& fp, hipStream_t s) {
+  for (int axis = 0; axis < 3; ++axis) fft_axis<T, +1>(p, chbuf, axis, s);
+  const T scale = (T)(1.0 / (8.0 * (double)p->H3));
+  StageTimer st(p, s, CFFT_STAGE_UNFOLD);
+  hipLaunchKernelGGL(k_unfold<T>, dim3(nblk(p->H3 * p->batch, 256)), dim3(256), 0, s, chbuf, f0,
+                     static_cast<const cx<T>*>(p->d_twM.p), fp, scale);
+  launch_check<T>("k_unfold");
+}
+
+// One diffusion operator application D on f0 (device), optional E factors on
+// both sides: f0 <- E D (E f0).
+template <typename T>
+void run_diffuse(cfft_plan* p, T* f0, const FoldP& fp, hipStream_t s) {
+  run_fold_fft<T>(p, f0, fp, s);
+  if (is_q_path(p)) {
+    cx<T>* out = static_cast<cx<T>*>(p->d_ch2.p);
+    {
+    StageTimer st(p, s, CFFT_STAGE_QMUL);
+    hipLaunchKernelGGL(k_qmul<T>, dim3(nblk(p->n_alive * p->batch, 256)), dim3(256), 0, s,
+                       static_cast<const cx<T>*>(p->d_ch.p), out, static_cast<const cx<T>*>(p->d_Q.p),
+                       static_cast<const int*>(p->d_gidx.p), p->d, p->n_alive, p->batch);
+    launch_check<T>("k_qmul");
+    }
+    run_ifft_unfold<T>(p, out, f0, fp, s);
+  } else {
+    cx<T>* F = static_cast<cx<T>*>(p->d_F.p);
+    run_recon<T>(p, F, true, s);
+    run_irecon<T>(p, F, s);
+    run_ifft_unfold<T>(p, static_cast<cx<T>*>(p->d_ch.p), f0, fp, s);
+  }
+}
+
+hipStream_t pick(cfft_plan* p, void* s) { return s ? static_cast<hipStream_t>(s) : p->stream; }
+
+template <typename T>
+void to_dev(cfft_plan* p, DevBuf& dst, const double* src, size_t n) {
+  std::vector<T> h(src, src + n);
+  hip_check(hipMemcpyAsync(dst.p, h.data(), n * sizeof(T), hipMemcpyHostToDevice, p->stream), "H2D");
+  hip_check(hipStreamSynchronize(p->stream), "sync");
+}
+
+template <typename T>
+void from_dev(cfft_plan* p, const DevBuf& src, double* dst, size_t n) {
+  std::vector<T> h(n);
+  hip_check(hipMemcpyAsync(h.data(), src.p, n * sizeof(T), hipMemcpyDeviceToHost, p->stream), "D2H");
+  hip_check(hipStreamSynchronize(p->stream), "sync");
+  for (size_t i = 0; i < n; ++i) dst[i] = (double)h[i];
+}
+
+void check_plan(const cfft_plan* p) { require(p != nullptr, CFFT_ERR_INVALID, "null plan"); }
+
+template <typename T>
+void forward_host(cfft_plan* p, const double* f0, double* F) {
+  to_dev<T>(p, p->d_f0, f0, p->M3 * p->batch);
+  FoldP fp = fold_params<T>(p, nullptr, false);
+  run_fold_fft<T>(p, static_cast<const T*>(p->d_f0.p), fp, p->stream);
+  run_recon<T>(p, static_cast<cx<T>*>(p->d_F.p), false, p->stream);
+  from_dev<T>(p, p->d_F, F, 2 * (size_t)p->n_spec * p->batch);
+}
+
+template <typename T>
+void backward_host(cfft_plan* p, const double* F, double* f0) {
+  to_dev<T>(p, p->d_F, F, 2 * (size_t)p->n_spec * p->batch);
+  FoldP fp = fold_params<T>(p, nullptr, false);
+  run_irecon<T>(p, static_cast<const cx<T>*>(p->d_F.p), p->stream);
+  run_ifft_unfold<T>(p, static_cast<cx<T>*>(p->d_ch.p), static_cast<T*>(p->d_f0.p), fp, p->stream);
+  from_dev<T>(p, p->d_f0, f0, p->M3 * p->batch);
+}
+
+template <typename T>
+void g0_host(cfft_plan* p, const double* f0, double* G0) {
+  to_dev<T>(p, p->d_f0, f0, p->M3 * p->batch);
+  FoldP fp = fold_params<T>(p, nullptr, false);
+  run_fold_fft<T>(p, static_cast<const T*>(p->d_f0.p), fp, p->stream);
+  DevBuf g;
+  g.alloc(2 * sizeof(T) * p->M3 * p->batch);
+  hipLaunchKernelGGL(k_assemble<T>, dim3(nblk(p->M3 * p->batch, 256)), dim3(256), 0, p->stream,
+                     static_cast<const cx<T>*>(p->d_ch.p), static_cast<cx<T>*>(g.p), fp,
+                     static_cast<const int*>(p->d_chmap.p));
+  launch_check<T>("k_assemble");
+  from_dev<T>(p, g, G0, 2 * (size_t)p->M3 * p->batch);
+}
+
+template <typename T>
+void diffuse_host(cfft_plan* p, double* f0) {
+  to_dev<T>(p, p->d_f0, f0, p->M3 * p->batch);
+  FoldP fp = fold_params<T>(p, nullptr, false);
+  run_diffuse<T>(p, static_cast<T*>(p->d_f0.p), fp, p->stream);
+  from_dev<T>(p, p->d_f0, f0, p->M3 * p->batch);
+}
+
+template <typename T>
+void exp_dev(cfft_plan* p, const T* w, T* E, double factor, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_exp<T>, dim3(nblk(n, 256)), dim3(256), 0, s, w, E, (T)factor, n);
+  launch_check<T>("k_exp");
+}
+
+template <typename T>
+void step_host(cfft_plan* p, double* q, const double* w, double dt, int nsteps) {
+  to_dev<T>(p, p->d_f0, q, p->M3 * p->batch);
+  to_dev<T>(p, p->d_E, w, p->M3 * p->batch);
+  exp_dev<T>(p, static_cast<const T*>(p->d_E.p), static_cast<T*>(p->d_E.p), -dt / 2.0,
+             p->M3 * p->batch, p->stream);
+  const T* E = static_cast<const T*>(p->d_E.p);
+  FoldP fp = fold_params<T>(p, &E, true);
+  for (int s = 0; s < nsteps; ++s) run_diffuse<T>(p, static_cast<T*>(p->d_f0.p), fp, p->stream);
+  from_dev<T>(p, p->d_f0, q, p->M3 * p->batch);
+}
+
+template <typename T>
+void step_ptrs_dev(cfft_plan* p, T* q, const void* const* E, int nsteps, hipStream_t s) {
+  require(p->batch <= kMaxEPtr, CFFT_ERR_INVALID, "per-entry factors need batch <= 64");
+  FoldP fp = fold_params<T>(p, reinterpret_cast<const T* const*>(E), false);
+  for (int i = 0; i < nsteps; ++i) run_diffuse<T>(p, q, fp, s);
+}
+
+template <typename T>
+void step_dev(cfft_plan* p, T* q, const T* E, int nsteps, hipStream_t s) {
+  FoldP fp = fold_params<T>(p, &E, true);
+  for (int i = 0; i < nsteps; ++i) run_diffuse<T>(p, q, fp, s);
+}
+
+}  // namespace
+
+// ============================================================================
+// extern "C" entry points
+// ============================================================================
+
+extern "C" {
+
+int cfft_last_error(char* buf, size_t n) {
+  if (buf && n) {
+    size_t m = std::min(n - 1, g_last_error.size());
+    std::memcpy(buf, g_last_error.data(), m);
+    buf[m] = 0;
+  }
+  return (int)g_last_error.size();
+}
+
+static void check_dim(int32_t dim) { require(dim == 2 || dim == 3, CFFT_ERR_INVALID, "dim must be 2 or 3"); }
+
+int cfft_find_optimal_shift(int32_t dim, const int32_t* N, const int32_t* ops, int32_t nops,
+                            double* shift_out, int32_t* ops_out) {
+  return guarded([&] {
+    check_dim(dim);
+    require(N && ops && shift_out && ops_out && nops > 0, CFFT_ERR_INVALID, "null argument");
+    int n[3] = {1, 1, 1};
+    for (int d = 0; d < dim; ++d) n[d] = N[d];
+    std::vector<Op> sh;
+    double s[3] = {0, 0, 0};
+    find_optimal_shift(dim, n, unpack_ops(ops, nops), s, sh);
+    for (int d = 0; d < dim; ++d) shift_out[d] = s[d];
+    pack_ops(sh, ops_out);
+  });
+}
+
+int cfft_count_special_positions(int32_t dim, const int32_t* N, const int32_t* ops, int32_t nops,
+                                 int64_t* count_out) {
+  return guarded([&] {
+    check_dim(dim);
+    require(N && ops && count_out && nops > 0, CFFT_ERR_INVALID, "null argument");
+    int n[3] = {1, 1, 1};
+    for (int d = 0; d < dim; ++d) n[d] = N[d];
+    *count_out = count_special_positions(dim, n, unpack_ops(ops, nops));
+  });
+}
+
+int cfft_calc_spectral_asu(int32_t dim, const int32_t* N, const int32_t* ops, int32_t nops,
+                           int32_t* idx_out, int32_t* mult_out, int64_t cap, int64_t* n_out) {
+  return guarded([&] {
+    check_dim(dim);
+    require(N && ops && n_out && nops > 0, CFFT_ERR_INVALID, "null argument");
+    int n[3] = {1, 1, 1};
+    for (int d = 0; d < dim; ++d) n[d] = N[d];
+    SpecASU s = calc_spectral_asu(dim, n, unpack_ops(ops, nops));
+    *n_out = (int64_t)s.idx.size();
+    int64_t m = std::min<int64_t>(cap, (int64_t)s.idx.size());
+    for (int64_t i = 0; i < m; ++i) {
+      if (idx_out)
+        for (int d = 0; d < 3; ++d) idx_out[3 * i + d] = s.idx[i][d];
+      if (mult_out) mult_out[i] = s.mult[i];
+    }
+  });
+}
+
+int cfft_auto_L(int32_t dim, const int32_t* ops, int32_t nops, int32_t* L_out) {
+  return guarded([&] {
+    check_dim(dim);
+    require(ops && L_out && nops > 0, CFFT_ERR_INVALID, "null argument");
+    int L[3];
+    auto_L(dim, unpack_ops(ops, nops), L);
+    for (int d = 0; d < dim; ++d) L_out[d] = L[d];
+  });
+}
+
+int cfft_select_rep_ops(int32_t dim, const int32_t* N, const int32_t* ops, int32_t nops,
+                        const int32_t* L, int32_t* rep_out) {
+  return guarded([&] {
+    check_dim(dim);
+    require(N && ops && L && rep_out && nops > 0, CFFT_ERR_INVALID, "null argument");
+    int n[3] = {1, 1, 1}, l[3] = {1, 1, 1};
+    for (int d = 0; d < dim; ++d) {
+      n[d] = N[d];
+      l[d] = L[d];
+    }
+    std::vector<int> r = select_rep_ops(dim, unpack_ops(ops, nops), l, n);
+    for (size_t i = 0; i < r.size(); ++i) rep_out[i] = r[i];
+  });
+}
+
+int cfft_detect_centering(int32_t dim, const int32_t* N, const int32_t* ops, int32_t nops,
+                          int32_t* cent_out) {
+  return guarded([&] {
+    check_dim(dim);
+    require(N && ops && cent_out && nops > 0, CFFT_ERR_INVALID, "null argument");
+    int n[3] = {1, 1, 1};
+    for (int d = 0; d < dim; ++d) n[d] = N[d];
+    *cent_out = detect_centering(dim, n, unpack_ops(ops, nops));
+  });
+}
+
+int cfft_fill_map(int32_t dim, const int32_t* N, const int32_t* ops, int32_t nops,
+                  const int32_t* L, int32_t* map_out) {
+  return guarded([&] {
+    check_dim(dim);
+    require(N && ops && L && map_out && nops > 0, CFFT_ERR_INVALID, "null argument");
+    int n[3] = {1, 1, 1}, l[3] = {1, 1, 1};
+    for (int d = 0; d < dim; ++d) {
+      n[d] = N[d];
+      l[d] = L[d];
+      require(n[d] % l[d] == 0, CFFT_ERR_NOT_DIVISIBLE, "N not divisible by L");
+    }
+    std::vector<int32_t> m = fill_map(dim, n, unpack_ops(ops, nops), l);
+    std::memcpy(map_out, m.data(), m.size() * 4);
+  });
+}
+
+int cfft_plan_create(const int32_t* N, const int32_t* ops, int32_t nops, double ds,
+                     const double* lattice, int32_t path, int32_t precision, int32_t batch,
+                     cfft_plan** out) {
+  return guarded([&] {
+    require(N && ops && lattice && out && nops > 0, CFFT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::unique_ptr<cfft_plan> p(new cfft_plan());
+    int n[3] = {N[0], N[1], N[2]};
+    build_plan(p.get(), n, unpack_ops(ops, nops), ds, lattice, path, precision, batch);
+    *out = p.release();
+  });
+}
+
+int cfft_plan_info(const cfft_plan* p, int32_t* info) {
+  return guarded([&] {
+    check_plan(p);
+    require(info != nullptr, CFFT_ERR_INVALID, "null argument");
+    int32_t v[16] = {p->M[0], p->M[1], p->M[2], p->L[0], p->L[1], p->L[2], p->n_spec, p->n_ch,
+                     p->cent, p->d,    p->path, p->prec, p->batch, p->H[0], p->H[1], p->H[2]};
+    std::memcpy(info, v, sizeof(v));
+  });
+}
+
+int cfft_plan_tables(const cfft_plan* p, int32_t* idx_out, int32_t* mult_out, double* K_out,
+                     int32_t* rep_out) {
+  return guarded([&] {
+    check_plan(p);
+    for (int i = 0; i < p->n_spec; ++i) {
+      if (idx_out)
+        for (int d = 0; d < 3; ++d) idx_out[3 * i + d] = p->spec.idx[i][d];
+      if (mult_out) mult_out[i] = p->spec.mult[i];
+      if (K_out) K_out[i] = p->K[i];
+    }
+    if (rep_out)
+      for (int a = 0; a < p->d; ++a) rep_out[a] = p->rep[a];
+  });
+}
+
+int cfft_update_kernel(cfft_plan* p, double ds, const double* lattice) {
+  return guarded([&] {
+    check_plan(p);
+    require(!(p->path == CFFT_PATH_M2Q || p->path == CFFT_PATH_M7Q), CFFT_ERR_INVALID,
+            "update_kernel applies to the fwd+K+bwd paths; replan the Q-fused paths");
+    const double* lat = lattice ? lattice : p->lattice;
+    p->K = kernel_spec(p->spec, ds, lat);
+    p->ds = ds;
+    if (lattice) std::memcpy(p->lattice, lattice, sizeof(double) * 9);
+    hip_check(hipStreamSynchronize(p->stream), "sync");
+    if (p->prec == CFFT_FP64)
+      upload_real<double>(p->d_K, p->K);
+    else
+      upload_real<float>(p->d_K, p->K);
+  });
+}
+
+int cfft_plan_destroy(cfft_plan* p) {
+  return guarded([&] {
+    if (!p) return;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    delete p;
+  });
+}
+
+#define CFFT_DISPATCH(p, FN, ...) \
+  ((p)->prec == CFFT_FP64 ? FN<double>(__VA_ARGS__) : FN<float>(__VA_ARGS__))
+
+int cfft_forward(cfft_plan* p, const double* f0, double* F) {
+  return guarded([&] {
+    check_plan(p);
+    require(f0 && F, CFFT_ERR_INVALID, "null argument");
+    CFFT_DISPATCH(p, forward_host, p, f0, F);
+  });
+}
+
+int cfft_backward(cfft_plan* p, const double* F, double* f0) {
+  return guarded([&] {
+    check_plan(p);
+    require(f0 && F, CFFT_ERR_INVALID, "null argument");
+    require(!is_q_path(p), CFFT_ERR_INVALID, "backward transform needs a fwd+K+bwd path plan");
+    CFFT_DISPATCH(p, backward_host, p, F, f0);
+  });
+}
+
+int cfft_forward_g0(cfft_plan* p, const double* f0, double* G0) {
+  return guarded([&] {
+    check_plan(p);
+    require(f0 && G0, CFFT_ERR_INVALID, "null argument");
+    CFFT_DISPATCH(p, g0_host, p, f0, G0);
+  });
+}
+
+int cfft_diffuse(cfft_plan* p, double* f0) {
+  return guarded([&] {
+    check_plan(p);
+    require(f0 != nullptr, CFFT_ERR_INVALID, "null argument");
+    CFFT_DISPATCH(p, diffuse_host, p, f0);
+  });
+}
+
+int cfft_step(cfft_plan* p, double* q, const double* w, double dt, int32_t nsteps) {
+  return guarded([&] {
+    check_plan(p);
+    require(q && w && nsteps >= 0, CFFT_ERR_INVALID, "bad argument");
+    CFFT_DISPATCH(p, step_host, p, q, w, dt, nsteps);
+  });
+}
+
+int cfft_sub_to_full(cfft_plan* p, const double* f0, double* u) {
+  return guarded([&] {
+    check_plan(p);
+    require(f0 && u, CFFT_ERR_INVALID, "null argument");
+    if (p->fmap_host.empty()) {
+      p->fmap_host = fill_map(3, p->N, p->ops, p->L);
+      upload_bytes(p->d_fmap, p->fmap_host.data(), p->fmap_host.size() * 4);
+    }
+    if (p->d_u.bytes < p->esz * p->N3) p->d_u.alloc(p->esz * p->N3);
+    auto run = [&](auto tag) {
+      using T = decltype(tag);
+      to_dev<T>(p, p->d_f0, f0, p->M3);
+      hipLaunchKernelGGL(k_sub_to_full<T>, dim3(nblk(p->N3, 256)), dim3(256), 0, p->stream,
+                         static_cast<const T*>(p->d_f0.p), static_cast<T*>(p->d_u.p),
+                         static_cast<const int*>(p->d_fmap.p), p->N3);
+      launch_check<T>("k_sub_to_full");
+      from_dev<T>(p, p->d_u, u, p->N3);
+    };
+    if (p->prec == CFFT_FP64) run(double{}); else run(float{});
+  });
+}
+
+int cfft_full_to_sub(cfft_plan* p, const double* u, double* f0) {
+  return guarded([&] {
+    check_plan(p);
+    require(f0 && u, CFFT_ERR_INVALID, "null argument");
+    if (p->d_u.bytes < p->esz * p->N3) p->d_u.alloc(p->esz * p->N3);
+    auto run = [&](auto tag) {
+      using T = decltype(tag);
+      to_dev<T>(p, p->d_u, u, p->N3);
+      hipLaunchKernelGGL(k_full_to_sub<T>, dim3(nblk(p->M3, 256)), dim3(256), 0, p->stream,
+                         static_cast<const T*>(p->d_u.p), static_cast<T*>(p->d_f0.p), p->M[0],
+                         p->M[1], p->M[2], p->L[0], p->L[1], p->L[2], p->N[0], p->N[1]);
+      launch_check<T>("k_full_to_sub");
+      from_dev<T>(p, p->d_f0, f0, p->M3);
+    };
+    if (p->prec == CFFT_FP64) run(double{}); else run(float{});
+  });
+}
+
+int cfft_forward_dev(cfft_plan* p, const void* f0, void* F, void* stream) {
+  return guarded([&] {
+    check_plan(p);
+    require(f0 && F, CFFT_ERR_INVALID, "null argument");
+    hipStream_t s = pick(p, stream);
+    auto run = [&](auto tag) {
+      using T = decltype(tag);
+      FoldP fp = fold_params<T>(p, nullptr, false);
+      run_fold_fft<T>(p, static_cast<const T*>(f0), fp, s);
+      run_recon<T>(p, static_cast<cx<T>*>(F), false, s);
+    };
+    if (p->prec == CFFT_FP64) run(double{}); else run(float{});
+  });
+}
+
+int cfft_backward_dev(cfft_plan* p, const void* F, void* f0, void* stream) {
+  return guarded([&] {
+    check_plan(p);
+    require(f0 && F, CFFT_ERR_INVALID, "null argument");
+    require(!is_q_path(p), CFFT_ERR_INVALID, "backward transform needs a fwd+K+bwd path plan");
+    hipStream_t s = pick(p, stream);
+    auto run = [&](auto tag) {
+      using T = decltype(tag);
+      FoldP fp = fold_params<T>(p, nullptr, false);
+      run_irecon<T>(p, static_cast<const cx<T>*>(F), s);
+      run_ifft_unfold<T>(p, static_cast<cx<T>*>(p->d_ch.p), static_cast<T*>(f0), fp, s);
+    };
+    if (p->prec == CFFT_FP64) run(double{}); else run(float{});
+  });
+}
+
+int cfft_diffuse_dev(cfft_plan* p, void* f0, void* stream) {
+  return guarded([&] {
+    check_plan(p);
+    require(f0 != nullptr, CFFT_ERR_INVALID, "null argument");
+    hipStream_t s = pick(p, stream);
+    auto run = [&](auto tag) {
+      using T = decltype(tag);
+      FoldP fp = fold_params<T>(p, nullptr, false);
+      run_diffuse<T>(p, static_cast<T*>(f0), fp, s);
+    };
+    if (p->prec == CFFT_FP64) run(double{}); else run(float{});
+  });
+}
+
+int cfft_step_dev(cfft_plan* p, void* q, const void* E, int32_t nsteps, void* stream) {
+  return guarded([&] {
+    check_plan(p);
+    require(q && E && nsteps >= 0, CFFT_ERR_INVALID, "bad argument");
+    hipStream_t s = pick(p, stream);
+    if (p->prec == CFFT_FP64)
+      step_dev<double>(p, static_cast<double*>(q), static_cast<const double*>(E), nsteps, s);
+    else
+      step_dev<float>(p, static_cast<float*>(q), static_cast<const float*>(E), nsteps, s);
+  });
+}
+
+int cfft_step_ptrs_dev(cfft_plan* p, void* q, const void* const* E, int32_t nsteps,
+                       void* stream) {
+  return guarded([&] {
+    check_plan(p);
+    require(q && E && nsteps >= 0, CFFT_ERR_INVALID, "bad argument");
+    hipStream_t s = pick(p, stream);
+    if (p->prec == CFFT_FP64)
+      step_ptrs_dev<double>(p, static_cast<double*>(q), E, nsteps, s);
+    else
+      step_ptrs_dev<float>(p, static_cast<float*>(q), E, nsteps, s);
+  });
+}
+
+int cfft_profile(cfft_plan* p, int32_t enable) {
+  return guarded([&] {
+    check_plan(p);
+    p->prof = enable != 0;
+  });
+}
+
+int cfft_profile_read(cfft_plan* p, double* ms_out, int64_t* count_out) {
+  return guarded([&] {
+    check_plan(p);
+    require(ms_out && count_out, CFFT_ERR_INVALID, "null argument");
+    for (int i = 0; i < CFFT_N_STAGES; ++i) {
+      ms_out[i] = 0.0;
+      count_out[i] = 0;
+    }
+    for (auto& m : p->marks) {
+      hip_check(hipEventSynchronize(m.b), "hipEventSynchronize");
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, m.a, m.b), "hipEventElapsedTime");
+      ms_out[m.stage] += ms;
+      count_out[m.stage] += 1;
+      (void)hipEventDestroy(m.a);
+      (void)hipEventDestroy(m.b);
+    }
+    p->marks.clear();
+  });
+}
+
+int cfft_exp_dev(cfft_plan* p, const void* w, void* E, double factor, int64_t n, void* stream) {
+  return guarded([&] {
+    check_plan(p);
+    require(w && E && n >= 0, CFFT_ERR_INVALID, "bad argument");
+    hipStream_t s = pick(p, stream);
+    if (p->prec == CFFT_FP64)
+      exp_dev<double>(p, static_cast<const double*>(w), static_cast<double*>(E), factor, n, s);
+    else
+      exp_dev<float>(p, static_cast<const float*>(w), static_cast<float*>(E), factor, n, s);
+  });
+}
+
+int cfft_sync(cfft_plan* p) {
+  return guarded([&] {
+    check_plan(p);
+    hip_check(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,200 @@
+/*
+ * cfft_hip.h — C ABI of the MI355X (gfx950) KRFFT / pseudo-spectral diffusion
+ * library (libcfft_hip.so).
+ *
+ * Each entry point replaces one operation of the reference Julia package
+ * liuyxpp/CrystallographicFFT.jl; the reference interface it stands in for is
+ * cited beside it (path:line in that repository).  Plain pointers and sizes
+ * only.  Every function returns a status code (CFFT_OK = 0); the message of
+ * the last failure on the calling thread is available from cfft_last_error.
+ *
+ * Conventions
+ *   - Grids are column-major (x fastest) with 0-based coordinates, as the
+ *     reference's Julia arrays.  A subgrid field f0 has M0*M1*M2 entries,
+ *     M = N / L (L from auto_L); batched fields are stored one after another.
+ *   - Operations: int32 array of nops x 12 entries, R row-major (9) then t (3)
+ *     in grid units: x -> R x + t (mod N).  2-D operations use the top-left
+ *     2x2 block of R (R[2][2] = 1) and t[2] = 0.
+ *   - Spectral coefficients: interleaved complex (re, im), one per spectral-ASU
+ *     point, ordered lexicographically by (h1, h2, h3) as the reference's
+ *     SpectralIndexing.points.
+ *   - Host-pointer entry points are blocking and use FP64 at the boundary.
+ *     *_dev entry points take device pointers of the plan's precision (double
+ *     for CFFT_FP64, float for CFFT_MIXED) and an optional hipStream_t
+ *     (NULL = the plan's own stream); they return without synchronising.
+ *   - A plan is not thread-safe; use one plan per stream.
+ */
+#ifndef CFFT_HIP_H
+#define CFFT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference's error() / @assert sites) ------------- */
+#define CFFT_OK 0
+#define CFFT_ERR_INVALID 1          /* null pointer / bad argument              */
+#define CFFT_ERR_NOT_COMMENSURATE 2 /* src/symmetry_ops.jl:41-43                */
+#define CFFT_ERR_NOT_DIVISIBLE 3    /* src/krfft.jl:309-311, 1109-1111          */
+#define CFFT_ERR_UNREACHABLE 4      /* src/krfft.jl:983, q_fused_krfft.jl:127    */
+#define CFFT_ERR_FILL_MAP 5         /* src/q_fused_krfft.jl:777-778             */
+#define CFFT_ERR_CENTERING 6        /* src/q_fused_krfft.jl:833-836, prefold:56 */
+#define CFFT_ERR_ODD_M 7            /* src/centering_fold.jl:61                 */
+#define CFFT_ERR_SIZE 8             /* src/q_fused_krfft.jl:998-1001            */
+#define CFFT_ERR_UNKNOWN_METHOD 9   /* src/diffusion_solver.jl:250              */
+#define CFFT_ERR_BOUNDS 10          /* src/spectral_indexing.jl:196-198         */
+#define CFFT_ERR_UNSUPPORTED 11     /* FFT length / table size outside support  */
+#define CFFT_ERR_HIP 12             /* HIP allocation / launch failure          */
+
+/* ---- pipeline paths ---------------------------------------------------- */
+#define CFFT_PATH_AUTO 0 /* M7 when centred and foldable, else M2             */
+#define CFFT_PATH_M2 1   /* fwd + spectral K + bwd, plan_m2_scft               */
+#define CFFT_PATH_M7 2   /* centred fwd + K + bwd, plan_centered_scft          */
+#define CFFT_PATH_M2Q 3  /* Q-fused multiply on the M-grid, plan_m2_q          */
+#define CFFT_PATH_M7Q 4  /* centring fold + Q multiply, plan_m7_scft           */
+
+#define CFFT_FP64 0  /* all arithmetic FP64                                   */
+#define CFFT_MIXED 1 /* FP32 data/FFT/propagator, FP64 tables and reductions  */
+
+typedef struct cfft_plan cfft_plan;
+
+/* Message of the last failure on this thread; returns its full length. */
+int cfft_last_error(char* buf, size_t n);
+
+/* ---- plan-time maps (host only; no GPU needed) ------------------------- */
+
+/* find_optimal_shift, src/asu.jl:57-123.  shift_out[dim] = b/N;
+ * ops_out (nops x 12) = shifted ops t' = t + (R b - b) N. */
+int cfft_find_optimal_shift(int32_t dim, const int32_t* N, const int32_t* ops, int32_t nops,
+                            double* shift_out, int32_t* ops_out);
+
+/* _count_special_positions_fast, src/asu.jl:148-214. */
+int cfft_count_special_positions(int32_t dim, const int32_t* N, const int32_t* ops,
+                                 int32_t nops, int64_t* count_out);
+
+/* calc_spectral_asu, src/spectral_indexing.jl:22-176.  Writes at most cap
+ * points (idx_out: cap x 3, 0-based; mult_out: cap); *n_out = total count.
+ * Call with cap = 0 to query the size. */
+int cfft_calc_spectral_asu(int32_t dim, const int32_t* N, const int32_t* ops, int32_t nops,
+                           int32_t* idx_out, int32_t* mult_out, int64_t cap, int64_t* n_out);
+
+/* auto_L, src/krfft.jl:225-279. */
+int cfft_auto_L(int32_t dim, const int32_t* ops, int32_t nops, int32_t* L_out);
+
+/* _select_rep_ops, src/krfft.jl:958-987: rep_out[prod(L)] op indices (0-based). */
+int cfft_select_rep_ops(int32_t dim, const int32_t* N, const int32_t* ops, int32_t nops,
+                        const int32_t* L, int32_t* rep_out);
+
+/* detect_centering_type, src/symmetry_ops.jl:133-179 (0 P, 1 C, 2 A, 3 I, 4 F). */
+int cfft_detect_centering(int32_t dim, const int32_t* N, const int32_t* ops, int32_t nops,
+                          int32_t* cent_out);
+
+/* _build_fill_map, src/q_fused_krfft.jl:733-783: map_out[N0*N1*N2] = 0-based
+ * linear M-grid index of the subgrid point each full-grid point copies. */
+int cfft_fill_map(int32_t dim, const int32_t* N, const int32_t* ops, int32_t nops,
+                  const int32_t* L, int32_t* map_out);
+
+/* ---- plans ------------------------------------------------------------- */
+
+/* plan_m2_scft (src/q_fused_krfft.jl:948-986), plan_centered_scft
+ * (src/centering_fold.jl:868-894), plan_m2_q (src/q_fused_krfft.jl:83-169),
+ * plan_m7_scft (:826-855), plan_krfft(spec, ops) (src/krfft.jl:301-389) and
+ * plan_m2_backward (:1101-1170) in one object.  ops are the SHIFTED ops
+ * (output of cfft_find_optimal_shift); lattice is 3x3 column-major (columns
+ * a, b, c); batch = number of fields processed per call (>= 1). */
+int cfft_plan_create(const int32_t* N, const int32_t* ops, int32_t nops, double ds,
+                     const double* lattice, int32_t path, int32_t precision, int32_t batch,
+                     cfft_plan** out);
+
+/* Plan metadata: info[16] = {M0, M1, M2, L0, L1, L2, n_spec, n_ch, centering,
+ * d, path, precision, batch, H0, H1, H2}. */
+int cfft_plan_info(const cfft_plan* plan, int32_t* info);
+
+/* Spectral-ASU points of the plan: idx_out (n_spec x 3), mult_out (n_spec);
+ * K_out (n_spec) = diffusion kernel; rep_out (d) = representative op indices.
+ * Any pointer may be NULL. */
+int cfft_plan_tables(const cfft_plan* plan, int32_t* idx_out, int32_t* mult_out, double* K_out,
+                     int32_t* rep_out);
+
+/* update_m2_kernel! / update_kernel! (src/q_fused_krfft.jl:1036-1052,
+ * src/centering_fold.jl:939-953): recompute K for a new ds (and lattice). */
+int cfft_update_kernel(cfft_plan* plan, double ds, const double* lattice);
+
+int cfft_plan_destroy(cfft_plan* plan);
+
+/* ---- execute (host pointers, FP64, blocking) --------------------------- */
+
+/* fft_reconstruct! / fft_reconstruct_centered! (src/krfft.jl:554-562,
+ * src/centering_fold.jl:421-431): f0 (batch x M^3 real) -> F_spec
+ * (batch x n_spec complex). */
+int cfft_forward(cfft_plan* plan, const double* f0, double* F_spec);
+
+/* execute_m2_backward! / execute_centered_ikrfft! (src/krfft.jl:1269-1277,
+ * src/centering_fold.jl:745-764): F_spec -> f0 (real part). */
+int cfft_backward(cfft_plan* plan, const double* F_spec, double* f0);
+
+/* Subgrid spectrum G0 = FFT_M(f0) as assembled by assemble_G0!
+ * (src/centering_fold.jl:277-296): f0 -> G0 (batch x M^3 complex). */
+int cfft_forward_g0(cfft_plan* plan, const double* f0, double* G0);
+
+/* execute_m2_scft! / execute_centered_scft! / execute_m2_q! / execute_m7_scft!
+ * (src/q_fused_krfft.jl:989-1026, src/centering_fold.jl:908-931,
+ * src/q_fused_krfft.jl:553-576, :870-895): f0 <- D f0 in place. */
+int cfft_diffuse(cfft_plan* plan, double* f0);
+
+/* step_diffusion! (src/diffusion_solver.jl:254-267) repeated nsteps times:
+ * q <- e^{-w dt/2} D e^{-w dt/2} q.  D uses the plan's ds. */
+int cfft_step(cfft_plan* plan, double* q, const double* w, double dt, int32_t nsteps);
+
+/* subgrid_to_fullgrid! / fullgrid_to_subgrid! (src/q_fused_krfft.jl:681-719),
+ * one field (batch entry 0 layout). */
+int cfft_sub_to_full(cfft_plan* plan, const double* f0, double* u);
+int cfft_full_to_sub(cfft_plan* plan, const double* u, double* f0);
+
+/* ---- execute (device pointers) ----------------------------------------- */
+int cfft_forward_dev(cfft_plan* plan, const void* f0, void* F_spec, void* stream);
+int cfft_backward_dev(cfft_plan* plan, const void* F_spec, void* f0, void* stream);
+int cfft_diffuse_dev(cfft_plan* plan, void* f0, void* stream);
+/* E = exp(-w dt/2) per batch entry (batch x M^3, plan precision), applied on
+ * both sides of D: q <- E D (E q). */
+int cfft_step_dev(cfft_plan* plan, void* q, const void* E, int32_t nsteps, void* stream);
+/* As cfft_step_dev with one factor array per batch entry: E is a host array
+ * of `batch` device pointers (batch <= 64); the SCFT sweep uses it to switch
+ * between w_A and w_B along the contour independently for q and q+. */
+int cfft_step_ptrs_dev(cfft_plan* plan, void* q, const void* const* E, int32_t nsteps,
+                       void* stream);
+/* E <- exp(-w * factor), n entries of the plan precision. */
+int cfft_exp_dev(cfft_plan* plan, const void* w, void* E, double factor, int64_t n, void* stream);
+
+/* ---- per-stage timer (hipEvents around each kernel launch) ------------- */
+#define CFFT_STAGE_FOLD 0
+#define CFFT_STAGE_FFT_X 1
+#define CFFT_STAGE_FFT_Y 2
+#define CFFT_STAGE_FFT_Z 3
+#define CFFT_STAGE_RECON 4
+#define CFFT_STAGE_IRECON 5
+#define CFFT_STAGE_IFFT_X 6
+#define CFFT_STAGE_IFFT_Y 7
+#define CFFT_STAGE_IFFT_Z 8
+#define CFFT_STAGE_UNFOLD 9
+#define CFFT_STAGE_QMUL 10
+#define CFFT_N_STAGES 11
+/* No counterpart in the reference (its benches time stages with @elapsed,
+ * test/profile_centering_fold.jl:48-107).  enable != 0 brackets every
+ * subsequent kernel launch of the plan with a pair of hipEvents. */
+int cfft_profile(cfft_plan* plan, int32_t enable);
+/* Sum of the recorded kernel durations (ms_out[CFFT_N_STAGES]) and launch
+ * counts per stage since the last read; clears the record. */
+int cfft_profile_read(cfft_plan* plan, double* ms_out, int64_t* count_out);
+
+/* Synchronise the plan's stream. */
+int cfft_sync(cfft_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFFT_HIP_H */

@@ -1,0 +1,242 @@
+"""GPU parity: HIP pipelines through the C ABI vs the oracle.
+
+FP64 bar: 1e-12 relative (to the field's max magnitude) against the oracle and
+the full-grid FFT reference.  Mixed precision (FP32 data / FFT / propagator):
+2e-5 relative.
+"""
+import numpy as np
+import pytest
+
+import cfft_hip as cf
+import oracle as O
+from conftest import group_setup
+
+pytestmark = pytest.mark.gpu
+
+LAT4 = 4.0 * np.eye(3)
+DS = 0.01
+TOL64 = 1e-12
+TOL32 = 2e-5
+
+CENTRED = [229, 230, 70, 72, 225, 227, 63, 139, 74, 69]
+PRIMITIVE = [47, 221, 223, 224, 123, 136, 200, 10, 2, 25]
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+def field(sg, N, seed=1):
+    ops, ops_s, shift, R, t = group_setup(sg, N)
+    u = O.make_symmetric(R, t, N, seed=seed)
+    Lv = O.auto_L(t)
+    return ops_s, R, t, u, Lv
+
+
+@pytest.mark.parametrize("sg", PRIMITIVE + CENTRED)
+@pytest.mark.parametrize("path", [cf.PATH_M2, cf.PATH_M7])
+def test_forward_backward_diffuse_fp64(sg, path):
+    N = (16, 16, 16)
+    ops_s, R, t, u, Lv = field(sg, N)
+    cent = O.detect_centering_type(R, t, N)
+    if path == cf.PATH_M7 and (cent == O.CENT_P or not np.all(Lv == 2)):
+        pytest.skip("M7 needs a centred lattice with L = (2,2,2)")
+    print(f"sg={sg} N={N} path={path}", flush=True)
+    plan = cf.DevicePlan(N, ops_s, DS, LAT4, path)
+    spec = O.calc_spectral_asu(R, t, N)
+    idx, mult, K, rep = plan.tables()
+    assert np.array_equal(idx, spec.idx)
+    f0 = O.extract_subgrid(u, Lv)
+    # forward vs full-grid FFT at the spectral-ASU points
+    F = plan.forward(f0)
+    ref = np.fft.fftn(u)[tuple(spec.idx[:, d] for d in range(3))]
+    assert rel(F, ref) < TOL64
+    # backward round trip
+    assert rel(plan.backward(F), f0) < TOL64
+    # diffusion vs full-grid FFT.K.IFFT and vs the oracle
+    d = plan.diffuse(f0)
+    assert rel(d, O.fullgrid_reference(u, N, DS, LAT4, Lv)) < TOL64
+    assert rel(d, O.m2_scft(f0, spec, R, t, N, DS, LAT4)) < TOL64
+    # G0 = FFT_M(f0) (fold + channel FFTs + assemble)
+    G0 = plan.forward_g0(f0)
+    if path == cf.PATH_M2:
+        assert rel(G0, np.fft.fftn(f0)) < TOL64
+    else:
+        assert rel(G0, O.assemble_G0(O.fft_channels(O.centering_fold(f0, cent)), cent, f0.shape)) < TOL64
+
+
+@pytest.mark.parametrize("sg", [47, 221, 229, 230, 70, 10])
+@pytest.mark.parametrize("path", [cf.PATH_M2Q, cf.PATH_M7Q])
+def test_q_fused_fp64(sg, path):
+    N = (16, 16, 16)
+    ops_s, R, t, u, Lv = field(sg, N)
+    cent = O.detect_centering_type(R, t, N)
+    if path == cf.PATH_M7Q and cent == O.CENT_P:
+        pytest.skip("M7Q needs a centred lattice")
+    plan = cf.DevicePlan(N, ops_s, DS, LAT4, path)
+    f0 = O.extract_subgrid(u, Lv)
+    qp = O.m2q_plan(R, t, N, DS, LAT4)
+    ref = O.m2q_execute(f0, qp) if path == cf.PATH_M2Q else O.m7_scft(f0, qp, cent)
+    assert rel(plan.diffuse(f0), ref) < 1e-11
+    assert rel(plan.diffuse(f0), O.fullgrid_reference(u, N, DS, LAT4, Lv)) < 1e-11
+
+
+@pytest.mark.parametrize("sg,N", [(229, (32, 32, 32)), (230, (32, 32, 32)), (70, (32, 32, 32)),
+                                  (47, (32, 32, 32)), (223, (48, 48, 48)), (229, (48, 48, 48)),
+                                  (47, (24, 24, 24)), (230, (96, 96, 96))])
+def test_sizes_fp64(sg, N):
+    """Mixed radices (H = 6, 8, 12, 16, 24) and both paths."""
+    ops_s, R, t, u, Lv = field(sg, N, seed=3)
+    f0 = O.extract_subgrid(u, Lv)
+    spec = O.calc_spectral_asu(R, t, N)
+    ref = O.fullgrid_reference(u, N, DS, LAT4, Lv)
+    for path in (cf.PATH_AUTO, cf.PATH_M2):
+        print(f"sg={sg} N={N} path={path}", flush=True)
+        plan = cf.DevicePlan(N, ops_s, DS, LAT4, path)
+        F = plan.forward(f0)
+        Fref = np.fft.fftn(u)[tuple(spec.idx[:, d] for d in range(3))]
+        assert rel(F, Fref) < TOL64
+        assert rel(plan.diffuse(f0), ref) < TOL64
+
+
+@pytest.mark.parametrize("sg", [229, 47, 230])
+def test_step_and_batch(sg):
+    """step_diffusion (Strang) vs the oracle, batch of 3 fields = 3 singles."""
+    N = (16, 16, 16)
+    ops_s, R, t, u, Lv = field(sg, N)
+    spec = O.calc_spectral_asu(R, t, N)
+    w = O.extract_subgrid(O.make_symmetric(R, t, N, seed=7), Lv) * 3.0
+    q = O.extract_subgrid(u, Lv) + 1.0
+    plan = cf.DevicePlan(N, ops_s, DS, LAT4, cf.PATH_AUTO)
+    out = plan.step(q, w, DS, nsteps=5)
+    ref = q.copy()
+    for _ in range(5):
+        ref = O.strang_step(ref, w, DS, lambda f: O.m2_scft(f, spec, R, t, N, DS, LAT4))
+    assert rel(out, ref) < TOL64
+    pb = cf.DevicePlan(N, ops_s, DS, LAT4, cf.PATH_AUTO, batch=3)
+    qs = np.stack([q, 2 * q, q - 0.5])
+    ws = np.stack([w, w, 0.5 * w])
+    outb = pb.step(qs, ws, DS, nsteps=2)
+    for b in range(3):
+        single = plan.step(qs[b], ws[b], DS, nsteps=2)
+        assert rel(outb[b], single) < 1e-15
+
+
+@pytest.mark.parametrize("sg", [229, 230, 47, 70])
+def test_mixed_precision(sg):
+    N = (32, 32, 32)
+    ops_s, R, t, u, Lv = field(sg, N)
+    f0 = O.extract_subgrid(u, Lv)
+    p64 = cf.DevicePlan(N, ops_s, DS, LAT4, cf.PATH_AUTO, cf.FP64)
+    p32 = cf.DevicePlan(N, ops_s, DS, LAT4, cf.PATH_AUTO, cf.MIXED)
+    assert rel(p32.forward(f0), p64.forward(f0)) < TOL32
+    assert rel(p32.diffuse(f0), p64.diffuse(f0)) < TOL32
+
+
+def test_update_kernel_equals_replan():
+    """test/runtests.jl:344-362: update == replan (< 1e-14), and it changes."""
+    N = (16, 16, 16)
+    ops_s, R, t, u, Lv = field(221, N)
+    f0 = O.extract_subgrid(u, Lv)
+    p = cf.plan_m2_scft(N, 221, 3, 0.05, LAT4)
+    a = f0.copy()
+    cf.execute_m2_scft(p, a)
+    cf.update_m2_kernel(p, N, 221, 3, 0.10, LAT4)
+    b = f0.copy()
+    cf.execute_m2_scft(p, b)
+    p2 = cf.plan_m2_scft(N, 221, 3, 0.10, LAT4)
+    c = f0.copy()
+    cf.execute_m2_scft(p2, c)
+    assert np.max(np.abs(b - c)) < 1e-14
+    assert np.max(np.abs(a - b)) > 1e-4
+
+
+def test_ds_zero_identity():
+    """test/test_m2_scft.jl:99-116, test/test_centered_scft.jl:126-153."""
+    N = (32, 32, 32)
+    for sg in (47, 229, 230):
+        ops_s, R, t, u, Lv = field(sg, N)
+        f0 = O.extract_subgrid(u, Lv)
+        p = cf.DevicePlan(N, ops_s, 0.0, LAT4, cf.PATH_AUTO)
+        assert np.max(np.abs(p.diffuse(f0) - f0)) < 1e-12
+
+
+def test_reference_shaped_api():
+    """plan_*/execute_* names of the reference (test/runtests.jl:196-341)."""
+    N = (16, 16, 16)
+    ops_s, R, t, u, Lv = field(229, N)
+    spec = cf.calc_spectral_asu(ops_s, 3, N)
+    f0 = O.extract_subgrid(u, Lv)
+    ref = O.fullgrid_reference(u, N, 0.05, np.eye(3), Lv)
+    scft = cf.plan_centered_scft(spec, ops_s, N, 0.05, np.eye(3))
+    a = f0.copy()
+    cf.execute_centered_scft(scft, a)
+    assert np.max(np.abs(a - ref)) < 1e-12
+    fwd = cf.plan_krfft_centered(spec, ops_s)
+    assert isinstance(fwd, cf.CenteredKRFFTPlan)
+    Fs = cf.execute_centered_krfft(fwd, u).copy()
+    bwd = cf.plan_centered_ikrfft(spec, ops_s, fwd)
+    out = np.zeros_like(f0)
+    cf.execute_centered_ikrfft(bwd, Fs, out)
+    assert np.max(np.abs(out - f0)) < 1e-12
+    pk = cf.plan_krfft(spec, ops_s)
+    Fk = cf.execute_krfft(pk, u)
+    assert np.max(np.abs(Fk - Fs)) < 1e-11
+    bp = cf.plan_m2_backward(spec, ops_s)
+    assert np.max(np.abs(cf.execute_m2_backward(bp, Fk).real - f0)) < 1e-12
+    q = cf.plan_m2_q(N, 229, 3, 0.05, np.eye(3))
+    b = f0.copy()
+    cf.execute_m2_q(q, b)
+    assert np.max(np.abs(b - ref)) < 1e-10
+    m7 = cf.plan_m7_scft(N, 229, 3, 0.05, np.eye(3))
+    c = f0.copy()
+    cf.execute_m7_scft(m7, c)
+    assert np.max(np.abs(c - b)) < 1e-12
+    full = np.zeros(N)
+    cf.subgrid_to_fullgrid(full, f0, q)
+    assert np.max(np.abs(full - u)) < 1e-14
+    sub = np.zeros_like(f0)
+    cf.fullgrid_to_subgrid(sub, u, q)
+    assert np.array_equal(sub, f0)
+    solver = cf.plan_diffusion(N, np.eye(3), 229, 3, 0.05, method="qfused")
+    d = f0.copy()
+    cf.apply_diffusion_operator(solver, d)
+    assert np.max(np.abs(d - ref)) < 1e-10
+    w = 0.3 * f0
+    s = f0.copy()
+    cf.step_diffusion(solver, s, w, 0.05)
+    e = np.exp(-w * 0.025)
+    assert np.max(np.abs(s - e * O.fullgrid_reference(
+        O.subgrid_to_fullgrid(e * f0, O.fill_map(R, t, Lv, N), N), N, 0.05, np.eye(3), Lv))) < 1e-10
+
+
+def test_full_size_im3m_128_properties():
+    """Config grid (Im-3m 128^3, cell 4.0): forward vs the oracle forward
+    (the spectral-ASU points are checked bit-exact on the CPU), backward round
+    trip, ds = 0 identity and linearity."""
+    from cfft_hip.scft import symmetrize_subgrid
+    N = (128, 128, 128)
+    print("Im-3m 128^3 properties", flush=True)
+    ops, ops_s, shift, R, t = group_setup(229, N)
+    Lv = O.auto_L(t)
+    rng = np.random.default_rng(11)
+    f0 = symmetrize_subgrid(rng.standard_normal(N), ops_s, Lv)
+    g0 = symmetrize_subgrid(rng.standard_normal(N), ops_s, Lv)
+    p = cf.DevicePlan(N, ops_s, 0.0, LAT4, cf.PATH_M7)
+    idx, mult, _, _ = p.tables()
+    spec = O.SpectralASU(idx, mult, N)
+    rep = O.select_rep_ops(R, t, Lv, N)
+    Fref = O.krfft_forward(np.fft.fftn(f0), spec, R, t, rep, N)
+    for prec, tol in ((cf.FP64, TOL64), (cf.MIXED, TOL32)):
+        p = cf.DevicePlan(N, ops_s, 0.0, LAT4, cf.PATH_M7, prec)
+        F = p.forward(f0)
+        assert rel(F, Fref) < tol
+        assert rel(p.backward(F), f0) < tol
+        assert rel(p.diffuse(f0), f0) < tol
+    p = cf.DevicePlan(N, ops_s, DS, LAT4, cf.PATH_M7)
+    lhs = p.diffuse(2.0 * f0 - 3.0 * g0)
+    rhs = 2.0 * p.diffuse(f0) - 3.0 * p.diffuse(g0)
+    assert rel(lhs, rhs) < TOL64
+    # the M7 and M2 routes agree on the config grid
+    p2 = cf.DevicePlan(N, ops_s, DS, LAT4, cf.PATH_M2)
+    assert rel(p.diffuse(f0), p2.diffuse(f0)) < TOL64
