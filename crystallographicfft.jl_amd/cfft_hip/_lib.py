@@ -39,10 +39,13 @@ EXPORTED = [
     "cfft_forward_g0", "cfft_diffuse", "cfft_step", "cfft_sub_to_full", "cfft_full_to_sub",
     "cfft_forward_dev", "cfft_backward_dev", "cfft_diffuse_dev", "cfft_step_dev",
     "cfft_exp_dev", "cfft_sync", "cfft_step_ptrs_dev", "cfft_profile", "cfft_profile_read",
+    "cfft_sweep_dev", "cfft_plan_set_option",
 ]
 
-STAGES = ["fold", "fft_x", "fft_y", "fft_z", "recon", "irecon", "ifft_x", "ifft_y", "ifft_z",
-          "unfold", "qmul"]
+OPT_PERSISTENT = 1
+
+STAGES = ["p1_fold_fftx", "fft_x", "p2_fft_yz", "fft_z", "p3_recon", "p4_irecon_ifft_zy", "ifft_x",
+          "ifft_y", "ifft_z", "p5_ifftx_unfold", "qmul", "sweep"]
 
 
 class CFFTError(RuntimeError):
@@ -99,6 +102,8 @@ def _declare(lib):
         "cfft_step_ptrs_dev": (C.c_int, [P, P, C.POINTER(P), I32, P]),
         "cfft_profile": (C.c_int, [P, I32]),
         "cfft_profile_read": (C.c_int, [P, PD, PI64]),
+        "cfft_sweep_dev": (C.c_int, [P, P, C.POINTER(P), C.POINTER(P), PI32, I32, I32, P]),
+        "cfft_plan_set_option": (C.c_int, [P, I32, I64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -110,6 +115,15 @@ def lib():
     """Load (once) and return the library; raise if it is not built."""
     global _lib
     if _lib is None:
+        # PyTorch wheels bundle their own HIP/HSA runtime (same SONAME
+        # libamdhip64.so.7).  When both are used in one process, loading torch
+        # first makes the library bind to that runtime; loading this library
+        # first would load /opt/rocm's runtime beside torch's HSA runtime and
+        # torch would then find no GPU.
+        try:
+            import torch  # noqa: F401
+        except Exception:
+            pass
         if not os.path.exists(LIB_PATH):
             raise CFFTLibraryMissing(
                 f"{LIB_PATH} not found: build it with `python crystallographicfft.jl_amd/build.py`")

@@ -357,6 +357,9 @@ class DevicePlan:
     def sync(self):
         L.check(L.lib().cfft_sync(self._h))
 
+    def set_option(self, key, value):
+        L.check(L.lib().cfft_plan_set_option(self._h, int(key), int(value)))
+
 
 # ---------------------------------------------------------------------------
 # reference-shaped plans

@@ -86,24 +86,23 @@ class PropagatorSweep:
         s = self.stream()
         self.plan.exp_dev(self.w[0].data_ptr(), self.EA.data_ptr(), -self.ds / 2.0, self.M3, s)
         self.plan.exp_dev(self.w[1].data_ptr(), self.EB.data_ptr(), -self.ds / 2.0, self.M3, s)
-        # per-step factor pointers: q uses A for j < nA; q+ uses B for j < ns - nA
+        # q starts on the A block (E_A for steps < nA), q+ on the B block
+        # (E_B for steps < ns - nA); cfft_sweep_dev switches per entry.
         pa, pb = self.EA.data_ptr(), self.EB.data_ptr()
-        self._ptrs = []
-        for j in range(self.ns):
-            arr = (C.c_void_p * 2)(pa if j < self.nA else pb, pb if j < self.ns - self.nA else pa)
-            self._ptrs.append(arr)
+        self._E1 = (C.c_void_p * 2)(pa, pb)
+        self._E2 = (C.c_void_p * 2)(pb, pa)
+        self._sw = (C.c_int32 * 2)(self.nA, self.ns - self.nA)
 
     def reset(self):
         self.q.fill_(1.0)
 
     def run(self, steps=None, start=0):
-        """Advance q and q+ through `steps` contour steps (default Ns)."""
-        s = self.stream()
-        lib = L.lib()
-        h = self.plan.handle
-        qp = C.c_void_p(self.q.data_ptr())
-        for j in range(start, self.ns if steps is None else start + steps):
-            L.check(lib.cfft_step_ptrs_dev(h, qp, self._ptrs[j], 1, C.c_void_p(s)))
+        """Advance q and q+ through `steps` contour steps (default Ns) starting
+        at contour step `start`, in one library call."""
+        n = self.ns - start if steps is None else steps
+        L.check(L.lib().cfft_sweep_dev(self.plan.handle, C.c_void_p(self.q.data_ptr()), self._E1,
+                                       self._E2, self._sw, int(start), int(n),
+                                       C.c_void_p(self.stream())))
 
     def Q(self):
         """(Q from q, Q from q+), reduced in FP64."""

@@ -15,6 +15,7 @@
 
 #include "../../include/cfft_hip.h"
 #include "kernels.hpp"
+#include "pipeline.hpp"
 #include "plan.hpp"
 
 using namespace cfft;
@@ -106,6 +107,8 @@ struct cfft_plan {
   // device work buffers
   DevBuf d_f0, d_ch, d_ch2, d_F, d_E, d_u;
   std::vector<int32_t> fmap_host;
+  // options (cfft_plan_set_option)
+  int persistent = 0;  // contour sweeps as one cooperative launch
   // per-stage hipEvent timer (cfft_profile / cfft_profile_read)
   bool prof = false;
   struct Mark {
@@ -235,10 +238,7 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
   p->K = kernel_spec(p->spec, ds, lattice);
 
   hip_check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
-  if (prec == CFFT_FP64)
-    build_tables<double>(p);
-  else
-    build_tables<float>(p);
+  build_tables<double>(p);  // twiddles, phases and K stay FP64 in both precisions
 
   // signed spectral frequencies (get_k_vector)
   std::vector<int32_t> hs(3 * (size_t)p->n_spec);
@@ -248,10 +248,11 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
   upload_bytes(p->d_chmap, p->chmap, sizeof(p->chmap));
 
   // alive parity index (c, k) -> q = 2k + off_c
+  // channel-space index e = c H^3 + (x H2 + z) H1 + y  (layout [c][x][z][y])
   auto q_of = [&](long e, int* q) {
     int c = (int)(e / p->H3);
     long k = e - c * p->H3;
-    int k0 = (int)(k % p->H[0]), k1 = (int)((k / p->H[0]) % p->H[1]), k2 = (int)(k / (1L * p->H[0] * p->H[1]));
+    int k1 = (int)(k % p->H[1]), k2 = (int)((k / p->H[1]) % p->H[2]), k0 = (int)(k / (1L * p->H[1] * p->H[2]));
     q[0] = 2 * k0 + p->offs[c][0];
     q[1] = 2 * k1 + p->offs[c][1];
     q[2] = 2 * k2 + p->offs[c][2];
@@ -264,7 +265,7 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
 
   if (!is_q_path(p)) {
     ReverseLookup rl = reverse_lookup(p->spec, ops);
-    std::vector<uint32_t> tab((size_t)p->d * p->n_alive);
+    std::vector<uint32_t> tab((size_t)8 * p->n_alive, 0xFFFFFu);
     for (long e = 0; e < p->n_alive; ++e) {
       int q[3];
       q_of(e, q);
@@ -275,7 +276,7 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
         uint32_t ent = 0xFFFFFu;
         if (rl.idx[li] >= 0)
           ent = (uint32_t)rl.idx[li] | ((uint32_t)rl.j[li] << 20) | ((uint32_t)rl.conj[li] << 31);
-        tab[(size_t)a * p->n_alive + e] = ent;
+        tab[(size_t)e * 8 + a] = ent;
       }
     }
     upload_bytes(p->d_btab, tab.data(), tab.size() * 4);
@@ -334,7 +335,7 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
         int cc = p->chmap[(rq[0] & 1) | ((rq[1] & 1) << 1) | ((rq[2] & 1) << 2)];
         gidx[(size_t)b * p->n_alive + e] =
             cc < 0 ? -1
-                   : (int32_t)(cc * p->H3 + (rq[0] >> 1) + 1L * p->H[0] * ((rq[1] >> 1) + 1L * p->H[1] * (rq[2] >> 1)));
+                   : (int32_t)(cc * p->H3 + ((rq[0] >> 1) * 1L * p->H[2] + (rq[2] >> 1)) * p->H[1] + (rq[1] >> 1));
       }
     }
     if (prec == CFFT_FP64)
@@ -377,46 +378,38 @@ void launch_check(const char* what) {
   hip_check(hipGetLastError(), what);
 }
 
+#define CFFT_LEN_SWITCH(len, MACRO)                                                 \
+  switch (len) {                                                                    \
+    MACRO(1) MACRO(2) MACRO(3) MACRO(4) MACRO(6) MACRO(8) MACRO(12) MACRO(16)       \
+    MACRO(24) MACRO(32) MACRO(48) MACRO(64)                                         \
+    default: throw Error{CFFT_ERR_UNSUPPORTED, "FFT length not supported"};        \
+  }
+
+// Multi-lane FFT along y (axis 1) or z (axis 2) of a channel array in the
+// [b][c][x][z][y] layout (generic path for non-cubic H-grids).
 template <typename T, int DIR>
 void fft_axis(cfft_plan* p, cx<T>* data, int axis, hipStream_t s) {
   const long H0 = p->H[0], H1 = p->H[1], H2 = p->H[2];
   const long nb = (long)p->batch * p->n_ch;
-  long n_tr, inner, outer, stride;
-  int len = p->H[axis];
-  if (axis == 0) {
-    n_tr = nb * H1 * H2; inner = 1; outer = H0; stride = 1;
-  } else if (axis == 1) {
-    n_tr = nb * H0 * H2; inner = H0; outer = H0 * H1; stride = H0;
-  } else {
-    n_tr = nb * H0 * H1; inner = H0 * H1; outer = H0 * H1 * H2; stride = H0 * H1;
-  }
-  const cx<T>* tw = static_cast<const cx<T>*>(p->d_twF[axis].p);
-  const int bs = 128;
+  const int len = p->H[axis];
   if (len == 1) return;
-  StageTimer st(p, s, (DIR < 0 ? CFFT_STAGE_FFT_X : CFFT_STAGE_IFFT_X) + axis);
-  const unsigned g = nblk(n_tr, bs);
-#define CFFT_FFT_CASE(LEN)                                                                          \
-  case LEN:                                                                                         \
-    hipLaunchKernelGGL((k_fft<T, LEN, DIR>), dim3(g), dim3(bs), 0, s, data, n_tr, inner, outer,     \
-                       stride, tw);                                                                 \
-    break;
-  switch (len) {
-    case 1: return;
-    CFFT_FFT_CASE(2)
-    CFFT_FFT_CASE(3)
-    CFFT_FFT_CASE(4)
-    CFFT_FFT_CASE(6)
-    CFFT_FFT_CASE(8)
-    CFFT_FFT_CASE(12)
-    CFFT_FFT_CASE(16)
-    CFFT_FFT_CASE(24)
-    CFFT_FFT_CASE(32)
-    CFFT_FFT_CASE(48)
-    CFFT_FFT_CASE(64)
-    default: throw Error{CFFT_ERR_UNSUPPORTED, "FFT length not supported"};
+  long n_tr, inner, outer, stride;
+  if (axis == 1) {
+    n_tr = nb * H0 * H2; inner = 1; outer = H1; stride = 1;
+  } else {
+    n_tr = nb * H0 * H1; inner = H1; outer = H1 * H2; stride = H1;
   }
+  const cx<double>* tw = static_cast<const cx<double>*>(p->d_twF[axis].p);
+  StageTimer st(p, s, (DIR < 0 ? CFFT_STAGE_FFT_X : CFFT_STAGE_IFFT_X) + axis);
+#define CFFT_FFT_CASE(LEN)                                                                      \
+  case LEN:                                                                                     \
+    hipLaunchKernelGGL((k_fft_ml<T, double, LEN, DIR>),                                          \
+                       dim3(nblk(n_tr, MLGeom<LEN>::G)), dim3(MLGeom<LEN>::WG), 0, s, data, n_tr, \
+                       inner, outer, stride, tw);                                               \
+    break;
+  CFFT_LEN_SWITCH(len, CFFT_FFT_CASE)
 #undef CFFT_FFT_CASE
-  launch_check<T>("k_fft");
+  launch_check<T>("k_fft_ml");
 }
 
 template <typename T>
@@ -438,95 +431,247 @@ FoldP fold_params(cfft_plan* p, const T* const* E, bool contig) {
   return fp;
 }
 
-// f0 (device, plan precision) -> channel spectra in d_ch (forward half up to G0).
-template <typename T>
-void run_fold_fft(cfft_plan* p, const T* f0, const FoldP& fp, hipStream_t s) {
-  cx<T>* ch = static_cast<cx<T>*>(p->d_ch.p);
-  {
-  StageTimer st(p, s, CFFT_STAGE_FOLD);
-  hipLaunchKernelGGL(k_fold<T>, dim3(nblk(p->H3 * p->batch, 256)), dim3(256), 0, s, f0, ch,
-                     static_cast<const cx<T>*>(p->d_twM.p), fp);
-  launch_check<T>("k_fold");
-  }
-  for (int axis = 0; axis < 3; ++axis) fft_axis<T, -1>(p, ch, axis, s);
-}
+// Interaction factors of one launch: none, one contiguous batch x M^3 array,
+// or per-entry pointers switching from E1 to E2 at step sw[b].
+struct ESel {
+  int mode = 0;  // 0 none, 1 contiguous, 2 per entry
+  const void* contig = nullptr;
+  const void* const* E1 = nullptr;
+  const void* const* E2 = nullptr;
+  const int32_t* sw = nullptr;
+};
 
-template <typename T>
-void run_recon(cfft_plan* p, cx<T>* F, bool withK, hipStream_t s) {
-  ReconP rp{};
-  rp.d = p->d;
+PipeP pipe_params(cfft_plan* p, const ESel& es) {
+  PipeP pp{};
+  for (int d = 0; d < 3; ++d) {
+    pp.M[d] = p->M[d];
+    pp.H[d] = p->H[d];
+    pp.Pn[d] = p->P / p->N[d];
+  }
+  pp.n_ch = p->n_ch;
+  for (int c = 0; c < 8; ++c) pp.offs[c] = 0;
+  for (int c = 0; c < p->n_ch; ++c) pp.offs[c] = p->offs[c][0] | (p->offs[c][1] << 1) | (p->offs[c][2] << 2);
+  for (int i = 0; i < 8; ++i) pp.chmap[i] = p->chmap[i];
+  pp.batch = p->batch;
+  int yb = std::min(8, p->H[1]);
+  while (p->H[1] % yb) --yb;
+  pp.YB = yb;
+  pp.n_spec = p->n_spec;
+  pp.d = p->d;
+  pp.P = p->P;
   for (int a = 0; a < p->d; ++a) {
     const Op& g = p->ops[p->rep[a]];
     for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) rp.R[a][i * 3 + j] = g.R[i][j];
-    for (int i = 0; i < 3; ++i) rp.t[a][i] = g.t[i];
+      for (int j = 0; j < 3; ++j) pp.R[a][i * 3 + j] = g.R[i][j];
+    for (int i = 0; i < 3; ++i) pp.t[a][i] = g.t[i];
   }
-  for (int d = 0; d < 3; ++d) {
-    rp.M[d] = p->M[d];
-    rp.H[d] = p->H[d];
-    rp.N[d] = p->N[d];
-    rp.Pn[d] = p->P / p->N[d];
+  pp.E_contig = es.mode == 2 ? 0 : 1;
+  for (int b = 0; b < kMaxEPtr; ++b) {
+    pp.E1[b] = nullptr;
+    pp.E2[b] = nullptr;
+    pp.sw[b] = 0;
   }
-  rp.P = p->P;
-  for (int i = 0; i < 8; ++i) rp.chmap[i] = p->chmap[i];
-  rp.n_ch = p->n_ch;
-  rp.n_spec = p->n_spec;
-  rp.batch = p->batch;
+  if (es.mode == 1) pp.E1[0] = es.contig;
+  if (es.mode == 2) {
+    require(p->batch <= kMaxEPtr, CFFT_ERR_INVALID, "per-entry factors need batch <= 64");
+    for (int b = 0; b < p->batch; ++b) {
+      pp.E1[b] = es.E1[b];
+      pp.E2[b] = es.E2 ? es.E2[b] : es.E1[b];
+      pp.sw[b] = es.sw ? es.sw[b] : INT32_MAX;
+    }
+  }
+  pp.scale_unfold = 1.0 / (8.0 * (double)p->H3);
+  pp.scale_irecon = 1.0 / p->d;
+  return pp;
+}
+
+PipeBufs pipe_bufs(cfft_plan* p, const void* f0_in, void* f0_out, void* F, bool withK) {
+  PipeBufs B{};
+  B.f0_in = f0_in;
+  B.f0_out = f0_out;
+  B.ch = p->d_ch.p;
+  B.F = F;
+  B.hs = static_cast<const int*>(p->d_hs.p);
+  B.K = withK ? static_cast<const double*>(p->d_K.p) : nullptr;
+  B.W = p->d_W.p;
+  B.twM = p->d_twM.p;
+  B.tw0 = p->d_twF[0].p;
+  B.tw1 = p->d_twF[1].p;
+  B.tw2 = p->d_twF[2].p;
+  B.btab = static_cast<const uint32_t*>(p->d_btab.p);
+  return B;
+}
+
+bool cubic(const cfft_plan* p) { return p->H[0] == p->H[1] && p->H[1] == p->H[2]; }
+long n_tiles(const cfft_plan* p, const PipeP& pp) { return (long)p->batch * p->H[2] * (p->H[1] / pp.YB); }
+long n_planes(const cfft_plan* p) { return (long)p->batch * p->n_ch * p->H[0]; }
+
+template <typename T>
+void launch_p1(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipStream_t s) {
+  StageTimer st(p, s, CFFT_STAGE_FOLD);
+  const unsigned g = (unsigned)n_tiles(p, pp);
+#define CFFT_P1(L) \
+  case L: hipLaunchKernelGGL((k_p1<T, double, L>), dim3(g), dim3(kNT), 0, s, pp, B, step); break;
+  CFFT_LEN_SWITCH(p->H[0], CFFT_P1)
+#undef CFFT_P1
+  launch_check<T>("k_p1");
+}
+
+template <typename T>
+void launch_p2(cfft_plan* p, const PipeP& pp, const PipeBufs& B, hipStream_t s) {
+  if (!cubic(p)) {
+    fft_axis<T, -1>(p, static_cast<cx<T>*>(B.ch), 1, s);
+    fft_axis<T, -1>(p, static_cast<cx<T>*>(B.ch), 2, s);
+    return;
+  }
+  StageTimer st(p, s, CFFT_STAGE_FFT_Y);
+  const unsigned g = (unsigned)n_planes(p);
+#define CFFT_P2(L) \
+  case L: hipLaunchKernelGGL((k_p2<T, double, L, L>), dim3(g), dim3(kNT), 0, s, pp, B); break;
+  CFFT_LEN_SWITCH(p->H[1], CFFT_P2)
+#undef CFFT_P2
+  launch_check<T>("k_p2");
+}
+
+template <typename T>
+void launch_p3(cfft_plan* p, const PipeP& pp, const PipeBufs& B, hipStream_t s) {
   if (p->n_spec == 0) return;
   StageTimer st(p, s, CFFT_STAGE_RECON);
-  hipLaunchKernelGGL(k_recon<T>, dim3(nblk((long)p->n_spec * p->batch, 256)), dim3(256), 0, s,
-                     static_cast<const cx<T>*>(p->d_ch.p), F, static_cast<const int*>(p->d_hs.p),
-                     withK ? static_cast<const T*>(p->d_K.p) : nullptr,
-                     static_cast<const cx<T>*>(p->d_W.p), rp);
-  launch_check<T>("k_recon");
+  hipLaunchKernelGGL((k_p3<T, double>), dim3(nblk(8L * p->n_spec * p->batch, kNT)), dim3(kNT), 0, s, pp, B);
+  launch_check<T>("k_p3");
 }
 
 template <typename T>
-void run_irecon(cfft_plan* p, const cx<T>* F, hipStream_t s) {
-  IreconP ip{};
-  ip.d = p->d;
-  ip.n_alive = p->n_alive;
-  ip.n_spec = p->n_spec;
-  ip.batch = p->batch;
+void launch_p4(cfft_plan* p, const PipeP& pp, const PipeBufs& B, hipStream_t s) {
+  if (!cubic(p)) {
+    IreconP ip{};
+    ip.d = p->d;
+    ip.n_alive = p->n_alive;
+    ip.n_spec = p->n_spec;
+    ip.batch = p->batch;
+    {
+      StageTimer st(p, s, CFFT_STAGE_IRECON);
+      hipLaunchKernelGGL((k_irecon<T, double>), dim3(nblk(8L * p->n_alive * p->batch, 256)), dim3(256), 0, s,
+                         static_cast<const cx<T>*>(B.F), static_cast<cx<T>*>(B.ch), B.btab,
+                         static_cast<const cx<double>*>(B.W), ip, 1.0 / p->d);
+      launch_check<T>("k_irecon");
+    }
+    fft_axis<T, +1>(p, static_cast<cx<T>*>(B.ch), 2, s);
+    fft_axis<T, +1>(p, static_cast<cx<T>*>(B.ch), 1, s);
+    return;
+  }
   StageTimer st(p, s, CFFT_STAGE_IRECON);
-  hipLaunchKernelGGL(k_irecon<T>, dim3(nblk(p->n_alive * p->batch, 256)), dim3(256), 0, s, F,
-                     static_cast<cx<T>*>(p->d_ch.p), static_cast<const uint32_t*>(p->d_btab.p),
-                     static_cast<const cx<T>*>(p->d_W.p), ip, (T)(1.0 / p->d));
-  launch_check<T>("k_irecon");
+  const unsigned g = (unsigned)n_planes(p);
+#define CFFT_P4(L) \
+  case L: hipLaunchKernelGGL((k_p4<T, double, L, L>), dim3(g), dim3(kNT), 0, s, pp, B); break;
+  CFFT_LEN_SWITCH(p->H[1], CFFT_P4)
+#undef CFFT_P4
+  launch_check<T>("k_p4");
 }
 
-// channel spectra in `chbuf` -> f0 (inverse FFT + unfold).
 template <typename T>
-void run_ifft_unfold(cfft_plan* p, cx<T>* chbuf, T* f0, const FoldP& fp, hipStream_t s) {
-  for (int axis = 0; axis < 3; ++axis) fft_axis<T, +1>(p, chbuf, axis, s);
-  const T scale = (T)(1.0 / (8.0 * (double)p->H3));
+void launch_p5(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipStream_t s) {
   StageTimer st(p, s, CFFT_STAGE_UNFOLD);
-  hipLaunchKernelGGL(k_unfold<T>, dim3(nblk(p->H3 * p->batch, 256)), dim3(256), 0, s, chbuf, f0,
-                     static_cast<const cx<T>*>(p->d_twM.p), fp, scale);
-  launch_check<T>("k_unfold");
+  const unsigned g = (unsigned)n_tiles(p, pp);
+#define CFFT_P5(L) \
+  case L: hipLaunchKernelGGL((k_p5<T, double, L>), dim3(g), dim3(kNT), 0, s, pp, B, step); break;
+  CFFT_LEN_SWITCH(p->H[0], CFFT_P5)
+#undef CFFT_P5
+  launch_check<T>("k_p5");
 }
 
-// One diffusion operator application D on f0 (device), optional E factors on
-// both sides: f0 <- E D (E f0).
+// f0 -> channel spectra (P1, P2).
 template <typename T>
-void run_diffuse(cfft_plan* p, T* f0, const FoldP& fp, hipStream_t s) {
-  run_fold_fft<T>(p, f0, fp, s);
+void run_forward_channels(cfft_plan* p, const T* f0, const PipeP& pp, int step, hipStream_t s) {
+  PipeBufs B = pipe_bufs(p, f0, nullptr, nullptr, false);
+  launch_p1<T>(p, pp, B, step, s);
+  launch_p2<T>(p, pp, B, s);
+}
+
+template <typename T>
+void run_recon(cfft_plan* p, const PipeP& pp, cx<T>* F, bool withK, hipStream_t s) {
+  PipeBufs B = pipe_bufs(p, nullptr, nullptr, F, withK);
+  launch_p3<T>(p, pp, B, s);
+}
+
+// F -> f0 (P4, P5).
+template <typename T>
+void run_backward(cfft_plan* p, const PipeP& pp, const cx<T>* F, T* f0, int step, hipStream_t s) {
+  PipeBufs B = pipe_bufs(p, nullptr, f0, const_cast<cx<T>*>(F), false);
+  launch_p4<T>(p, pp, B, s);
+  launch_p5<T>(p, pp, B, step, s);
+}
+
+// One diffusion operator application per step on f0 (device), with the
+// interaction factors of `pp` on both sides: f0 <- E D (E f0).
+template <typename T>
+void run_diffuse(cfft_plan* p, T* f0, const PipeP& pp, int step, hipStream_t s) {
   if (is_q_path(p)) {
+    run_forward_channels<T>(p, f0, pp, step, s);
     cx<T>* out = static_cast<cx<T>*>(p->d_ch2.p);
     {
-    StageTimer st(p, s, CFFT_STAGE_QMUL);
-    hipLaunchKernelGGL(k_qmul<T>, dim3(nblk(p->n_alive * p->batch, 256)), dim3(256), 0, s,
-                       static_cast<const cx<T>*>(p->d_ch.p), out, static_cast<const cx<T>*>(p->d_Q.p),
-                       static_cast<const int*>(p->d_gidx.p), p->d, p->n_alive, p->batch);
-    launch_check<T>("k_qmul");
+      StageTimer st(p, s, CFFT_STAGE_QMUL);
+      hipLaunchKernelGGL(k_qmul<T>, dim3(nblk(p->n_alive * p->batch, 256)), dim3(256), 0, s,
+                         static_cast<const cx<T>*>(p->d_ch.p), out, static_cast<const cx<T>*>(p->d_Q.p),
+                         static_cast<const int*>(p->d_gidx.p), p->d, p->n_alive, p->batch);
+      launch_check<T>("k_qmul");
     }
-    run_ifft_unfold<T>(p, out, f0, fp, s);
-  } else {
-    cx<T>* F = static_cast<cx<T>*>(p->d_F.p);
-    run_recon<T>(p, F, true, s);
-    run_irecon<T>(p, F, s);
-    run_ifft_unfold<T>(p, static_cast<cx<T>*>(p->d_ch.p), f0, fp, s);
+    PipeBufs B = pipe_bufs(p, nullptr, f0, nullptr, false);
+    if (cubic(p)) {
+      StageTimer st(p, s, CFFT_STAGE_IFFT_Z);
+      const unsigned g = (unsigned)n_planes(p);
+#define CFFT_P4Q(L) \
+  case L: hipLaunchKernelGGL((k_p4q<T, double, L, L>), dim3(g), dim3(kNT), 0, s, pp, B, out); break;
+      CFFT_LEN_SWITCH(p->H[1], CFFT_P4Q)
+#undef CFFT_P4Q
+      launch_check<T>("k_p4q");
+    } else {
+      fft_axis<T, +1>(p, out, 2, s);
+      fft_axis<T, +1>(p, out, 1, s);
+      B.ch = out;
+    }
+    launch_p5<T>(p, pp, B, step, s);
+    return;
   }
+  cx<T>* F = static_cast<cx<T>*>(p->d_F.p);
+  run_forward_channels<T>(p, f0, pp, step, s);
+  run_recon<T>(p, pp, F, true, s);
+  run_backward<T>(p, pp, F, f0, step, s);
+}
+
+// Persistent cooperative sweep of nsteps Strang steps (cubic H-grids, fwd+K+bwd
+// paths); returns false when it cannot be used so the caller falls back to
+// one launch per phase.
+template <typename T>
+bool run_sweep_persistent(cfft_plan* p, T* q, const PipeP& pp, int step0, int nsteps, hipStream_t s) {
+  if (!p->persistent || !cubic(p) || is_q_path(p)) return false;
+  const void* kern = nullptr;
+#define CFFT_SW(L) \
+  case L: kern = reinterpret_cast<const void*>(&k_sweep<T, double, L>); break;
+  switch (p->H[0]) {
+    CFFT_SW(2) CFFT_SW(3) CFFT_SW(4) CFFT_SW(6) CFFT_SW(8) CFFT_SW(12) CFFT_SW(16) CFFT_SW(24)
+    CFFT_SW(32) CFFT_SW(48) CFFT_SW(64)
+    default: return false;
+  }
+#undef CFFT_SW
+  int dev = 0, ncu = 0, per_cu = 0, coop = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  hip_check(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev), "attr");
+  if (!coop) return false;
+  hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "attr");
+  hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kNT, 0), "occupancy");
+  if (per_cu < 1) return false;
+  long want = std::max(n_tiles(p, pp), n_planes(p));
+  want = std::max(want, (8L * p->n_spec * p->batch + kNT - 1) / kNT);
+  const long grid = std::min<long>((long)per_cu * ncu, want);
+  PipeBufs B = pipe_bufs(p, q, q, p->d_F.p, true);
+  PipeP ppc = pp;
+  int s0 = step0, ns = nsteps;
+  void* args[] = {&ppc, &B, &s0, &ns};
+  StageTimer st(p, s, CFFT_STAGE_SWEEP);
+  hip_check(hipLaunchCooperativeKernel(kern, dim3((unsigned)grid), dim3(kNT), args, 0, s),
+            "hipLaunchCooperativeKernel");
+  return true;
 }
 
 hipStream_t pick(cfft_plan* p, void* s) { return s ? static_cast<hipStream_t>(s) : p->stream; }
@@ -551,26 +696,26 @@ void check_plan(const cfft_plan* p) { require(p != nullptr, CFFT_ERR_INVALID, "n
 template <typename T>
 void forward_host(cfft_plan* p, const double* f0, double* F) {
   to_dev<T>(p, p->d_f0, f0, p->M3 * p->batch);
-  FoldP fp = fold_params<T>(p, nullptr, false);
-  run_fold_fft<T>(p, static_cast<const T*>(p->d_f0.p), fp, p->stream);
-  run_recon<T>(p, static_cast<cx<T>*>(p->d_F.p), false, p->stream);
+  PipeP pp = pipe_params(p, ESel{});
+  run_forward_channels<T>(p, static_cast<const T*>(p->d_f0.p), pp, 0, p->stream);
+  run_recon<T>(p, pp, static_cast<cx<T>*>(p->d_F.p), false, p->stream);
   from_dev<T>(p, p->d_F, F, 2 * (size_t)p->n_spec * p->batch);
 }
 
 template <typename T>
 void backward_host(cfft_plan* p, const double* F, double* f0) {
   to_dev<T>(p, p->d_F, F, 2 * (size_t)p->n_spec * p->batch);
-  FoldP fp = fold_params<T>(p, nullptr, false);
-  run_irecon<T>(p, static_cast<const cx<T>*>(p->d_F.p), p->stream);
-  run_ifft_unfold<T>(p, static_cast<cx<T>*>(p->d_ch.p), static_cast<T*>(p->d_f0.p), fp, p->stream);
+  PipeP pp = pipe_params(p, ESel{});
+  run_backward<T>(p, pp, static_cast<const cx<T>*>(p->d_F.p), static_cast<T*>(p->d_f0.p), 0, p->stream);
   from_dev<T>(p, p->d_f0, f0, p->M3 * p->batch);
 }
 
 template <typename T>
 void g0_host(cfft_plan* p, const double* f0, double* G0) {
   to_dev<T>(p, p->d_f0, f0, p->M3 * p->batch);
+  PipeP pp = pipe_params(p, ESel{});
+  run_forward_channels<T>(p, static_cast<const T*>(p->d_f0.p), pp, 0, p->stream);
   FoldP fp = fold_params<T>(p, nullptr, false);
-  run_fold_fft<T>(p, static_cast<const T*>(p->d_f0.p), fp, p->stream);
   DevBuf g;
   g.alloc(2 * sizeof(T) * p->M3 * p->batch);
   hipLaunchKernelGGL(k_assemble<T>, dim3(nblk(p->M3 * p->batch, 256)), dim3(256), 0, p->stream,
@@ -583,8 +728,8 @@ void g0_host(cfft_plan* p, const double* f0, double* G0) {
 template <typename T>
 void diffuse_host(cfft_plan* p, double* f0) {
   to_dev<T>(p, p->d_f0, f0, p->M3 * p->batch);
-  FoldP fp = fold_params<T>(p, nullptr, false);
-  run_diffuse<T>(p, static_cast<T*>(p->d_f0.p), fp, p->stream);
+  PipeP pp = pipe_params(p, ESel{});
+  run_diffuse<T>(p, static_cast<T*>(p->d_f0.p), pp, 0, p->stream);
   from_dev<T>(p, p->d_f0, f0, p->M3 * p->batch);
 }
 
@@ -594,29 +739,25 @@ void exp_dev(cfft_plan* p, const T* w, T* E, double factor, long n, hipStream_t 
   launch_check<T>("k_exp");
 }
 
+// nsteps Strang steps with the interaction factors of `es`.
+template <typename T>
+void run_steps(cfft_plan* p, T* q, const ESel& es, int step0, int nsteps, hipStream_t s) {
+  PipeP pp = pipe_params(p, es);
+  if (nsteps > 0 && run_sweep_persistent<T>(p, q, pp, step0, nsteps, s)) return;
+  for (int i = 0; i < nsteps; ++i) run_diffuse<T>(p, q, pp, step0 + i, s);
+}
+
 template <typename T>
 void step_host(cfft_plan* p, double* q, const double* w, double dt, int nsteps) {
   to_dev<T>(p, p->d_f0, q, p->M3 * p->batch);
   to_dev<T>(p, p->d_E, w, p->M3 * p->batch);
   exp_dev<T>(p, static_cast<const T*>(p->d_E.p), static_cast<T*>(p->d_E.p), -dt / 2.0,
              p->M3 * p->batch, p->stream);
-  const T* E = static_cast<const T*>(p->d_E.p);
-  FoldP fp = fold_params<T>(p, &E, true);
-  for (int s = 0; s < nsteps; ++s) run_diffuse<T>(p, static_cast<T*>(p->d_f0.p), fp, p->stream);
+  ESel es;
+  es.mode = 1;
+  es.contig = p->d_E.p;
+  run_steps<T>(p, static_cast<T*>(p->d_f0.p), es, 0, nsteps, p->stream);
   from_dev<T>(p, p->d_f0, q, p->M3 * p->batch);
-}
-
-template <typename T>
-void step_ptrs_dev(cfft_plan* p, T* q, const void* const* E, int nsteps, hipStream_t s) {
-  require(p->batch <= kMaxEPtr, CFFT_ERR_INVALID, "per-entry factors need batch <= 64");
-  FoldP fp = fold_params<T>(p, reinterpret_cast<const T* const*>(E), false);
-  for (int i = 0; i < nsteps; ++i) run_diffuse<T>(p, q, fp, s);
-}
-
-template <typename T>
-void step_dev(cfft_plan* p, T* q, const T* E, int nsteps, hipStream_t s) {
-  FoldP fp = fold_params<T>(p, &E, true);
-  for (int i = 0; i < nsteps; ++i) run_diffuse<T>(p, q, fp, s);
 }
 
 }  // namespace
@@ -782,10 +923,7 @@ int cfft_update_kernel(cfft_plan* p, double ds, const double* lattice) {
     p->ds = ds;
     if (lattice) std::memcpy(p->lattice, lattice, sizeof(double) * 9);
     hip_check(hipStreamSynchronize(p->stream), "sync");
-    if (p->prec == CFFT_FP64)
-      upload_real<double>(p->d_K, p->K);
-    else
-      upload_real<float>(p->d_K, p->K);
+    upload_real<double>(p->d_K, p->K);
   });
 }
 
@@ -888,9 +1026,9 @@ int cfft_forward_dev(cfft_plan* p, const void* f0, void* F, void* stream) {
     hipStream_t s = pick(p, stream);
     auto run = [&](auto tag) {
       using T = decltype(tag);
-      FoldP fp = fold_params<T>(p, nullptr, false);
-      run_fold_fft<T>(p, static_cast<const T*>(f0), fp, s);
-      run_recon<T>(p, static_cast<cx<T>*>(F), false, s);
+      PipeP pp = pipe_params(p, ESel{});
+      run_forward_channels<T>(p, static_cast<const T*>(f0), pp, 0, s);
+      run_recon<T>(p, pp, static_cast<cx<T>*>(F), false, s);
     };
     if (p->prec == CFFT_FP64) run(double{}); else run(float{});
   });
@@ -904,9 +1042,8 @@ int cfft_backward_dev(cfft_plan* p, const void* F, void* f0, void* stream) {
     hipStream_t s = pick(p, stream);
     auto run = [&](auto tag) {
       using T = decltype(tag);
-      FoldP fp = fold_params<T>(p, nullptr, false);
-      run_irecon<T>(p, static_cast<const cx<T>*>(F), s);
-      run_ifft_unfold<T>(p, static_cast<cx<T>*>(p->d_ch.p), static_cast<T*>(f0), fp, s);
+      PipeP pp = pipe_params(p, ESel{});
+      run_backward<T>(p, pp, static_cast<const cx<T>*>(F), static_cast<T*>(f0), 0, s);
     };
     if (p->prec == CFFT_FP64) run(double{}); else run(float{});
   });
@@ -919,8 +1056,8 @@ int cfft_diffuse_dev(cfft_plan* p, void* f0, void* stream) {
     hipStream_t s = pick(p, stream);
     auto run = [&](auto tag) {
       using T = decltype(tag);
-      FoldP fp = fold_params<T>(p, nullptr, false);
-      run_diffuse<T>(p, static_cast<T*>(f0), fp, s);
+      PipeP pp = pipe_params(p, ESel{});
+      run_diffuse<T>(p, static_cast<T*>(f0), pp, 0, s);
     };
     if (p->prec == CFFT_FP64) run(double{}); else run(float{});
   });
@@ -931,10 +1068,13 @@ int cfft_step_dev(cfft_plan* p, void* q, const void* E, int32_t nsteps, void* st
     check_plan(p);
     require(q && E && nsteps >= 0, CFFT_ERR_INVALID, "bad argument");
     hipStream_t s = pick(p, stream);
+    ESel es;
+    es.mode = 1;
+    es.contig = E;
     if (p->prec == CFFT_FP64)
-      step_dev<double>(p, static_cast<double*>(q), static_cast<const double*>(E), nsteps, s);
+      run_steps<double>(p, static_cast<double*>(q), es, 0, nsteps, s);
     else
-      step_dev<float>(p, static_cast<float*>(q), static_cast<const float*>(E), nsteps, s);
+      run_steps<float>(p, static_cast<float*>(q), es, 0, nsteps, s);
   });
 }
 
@@ -944,10 +1084,41 @@ int cfft_step_ptrs_dev(cfft_plan* p, void* q, const void* const* E, int32_t nste
     check_plan(p);
     require(q && E && nsteps >= 0, CFFT_ERR_INVALID, "bad argument");
     hipStream_t s = pick(p, stream);
+    ESel es;
+    es.mode = 2;
+    es.E1 = E;
     if (p->prec == CFFT_FP64)
-      step_ptrs_dev<double>(p, static_cast<double*>(q), E, nsteps, s);
+      run_steps<double>(p, static_cast<double*>(q), es, 0, nsteps, s);
     else
-      step_ptrs_dev<float>(p, static_cast<float*>(q), E, nsteps, s);
+      run_steps<float>(p, static_cast<float*>(q), es, 0, nsteps, s);
+  });
+}
+
+int cfft_sweep_dev(cfft_plan* p, void* q, const void* const* E1, const void* const* E2,
+                   const int32_t* sw, int32_t step0, int32_t nsteps, void* stream) {
+  return guarded([&] {
+    check_plan(p);
+    require(q && E1 && E2 && sw && nsteps >= 0, CFFT_ERR_INVALID, "bad argument");
+    hipStream_t s = pick(p, stream);
+    ESel es;
+    es.mode = 2;
+    es.E1 = E1;
+    es.E2 = E2;
+    es.sw = sw;
+    if (p->prec == CFFT_FP64)
+      run_steps<double>(p, static_cast<double*>(q), es, step0, nsteps, s);
+    else
+      run_steps<float>(p, static_cast<float*>(q), es, step0, nsteps, s);
+  });
+}
+
+int cfft_plan_set_option(cfft_plan* p, int32_t key, int64_t value) {
+  return guarded([&] {
+    check_plan(p);
+    switch (key) {
+      case CFFT_OPT_PERSISTENT: p->persistent = value != 0; break;
+      default: throw Error{CFFT_ERR_INVALID, "unknown option " + std::to_string(key)};
+    }
   });
 }
 

@@ -46,10 +46,23 @@ __host__ __device__ constexpr bool fft_len_ok(int len) {
   return n == 1;
 }
 
-// R-point DFT in place; DIR = -1 forward, +1 inverse.
-template <int R, int DIR, typename T>
+// x * w (DIR < 0) or x * conj(w) (DIR > 0), the product formed in the
+// table precision TW and rounded once to the data precision T.
+template <typename T, typename TW, int DIR>
+__device__ __forceinline__ cx<T> twmul(cx<T> x, cx<TW> w) {
+  const TW xr = (TW)x.x, xi = (TW)x.y;
+  if constexpr (DIR < 0)
+    return {(T)(xr * w.x - xi * w.y), (T)(xr * w.y + xi * w.x)};
+  else
+    return {(T)(xr * w.x + xi * w.y), (T)(xi * w.x - xr * w.y)};
+}
+
+// R-point DFT in place; DIR = -1 forward, +1 inverse.  The radix-3 constant
+// is applied in the table precision TW.
+template <int R, int DIR, typename T, typename TW = T>
 __device__ __forceinline__ void dft_small(cx<T>* v) {
-  if constexpr (R == 2) {
+  if constexpr (R == 1) {
+  } else if constexpr (R == 2) {
     cx<T> a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
@@ -68,12 +81,12 @@ __device__ __forceinline__ void dft_small(cx<T>* v) {
       v[3] = cadd(t1, mi_t3);
     }
   } else {  // R == 3
-    const T h = T(0.86602540378443864676372317075294);  // sqrt(3)/2
+    const TW h = TW(0.86602540378443864676372317075294);  // sqrt(3)/2
     cx<T> a = v[0];
     cx<T> s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
     cx<T> m = {a.x - T(0.5) * s.x, a.y - T(0.5) * s.y};
     // forward: y1 = m - i h d, y2 = m + i h d
-    cx<T> ihd = {-h * d.y, h * d.x};  // i * h * d
+    cx<T> ihd = {(T)(-h * (TW)d.y), (T)(h * (TW)d.x)};  // i * h * d
     v[0] = cadd(a, s);
     if constexpr (DIR < 0) {
       v[1] = csub(m, ihd);
@@ -85,9 +98,11 @@ __device__ __forceinline__ void dft_small(cx<T>* v) {
   }
 }
 
-// Stockham pass(es) starting at sub-transform length NS; tw[m] = exp(-2 pi i m/LEN).
-template <typename T, int LEN, int DIR, int NS>
-__device__ __forceinline__ void stockham(cx<T> (&a)[LEN], const cx<T>* __restrict__ tw) {
+// Stockham pass(es) starting at sub-transform length NS on LEN values held in
+// registers; twiddle exp(-2 pi i m/LEN) is tw[m * TWS] (a table of a longer
+// transform can be shared by passing its stride).
+template <typename T, typename TW, int LEN, int DIR, int NS, int TWS>
+__device__ __forceinline__ void stockham(cx<T> (&a)[LEN], const cx<TW>* __restrict__ tw) {
   if constexpr (NS < LEN) {
     constexpr int R = fft_radix(LEN, NS);
     constexpr int J = LEN / R;
@@ -101,23 +116,69 @@ __device__ __forceinline__ void stockham(cx<T> (&a)[LEN], const cx<T>* __restric
         v[r] = a[j + r * J];
         if (r > 0 && k > 0) {
           const int m = (r * k * (LEN / (NS * R))) % LEN;
-          v[r] = (DIR < 0) ? cmul(v[r], tw[m]) : cmulc(v[r], tw[m]);
+          v[r] = twmul<T, TW, DIR>(v[r], tw[m * TWS]);
         }
       }
-      dft_small<R, DIR>(v);
+      dft_small<R, DIR, T, TW>(v);
 #pragma unroll
       for (int r = 0; r < R; ++r) b[(j / NS) * NS * R + k + r * NS] = v[r];
     }
 #pragma unroll
     for (int i = 0; i < LEN; ++i) a[i] = b[i];
-    stockham<T, LEN, DIR, NS * R>(a, tw);
+    stockham<T, TW, LEN, DIR, NS * R, TWS>(a, tw);
   }
 }
 
 template <typename T, int LEN, int DIR>
 __device__ __forceinline__ void fft_reg(cx<T> (&a)[LEN], const cx<T>* __restrict__ tw) {
   static_assert(fft_len_ok(LEN), "FFT length must be 2^a 3^b");
-  stockham<T, LEN, DIR, 1>(a, tw);
+  stockham<T, T, LEN, DIR, 1, 1>(a, tw);
+}
+
+// Split of a length-LEN transform over R1 cooperating lanes with R2 = LEN/R1
+// values each (R2 divisible by R1).
+template <int LEN>
+struct MLS {
+  static constexpr int R1 = (LEN % 16 == 0) ? 4 : ((LEN % 4 == 0) ? 2 : 1);
+  static constexpr int R2 = LEN / R1;
+};
+
+// Multi-lane FFT.  A group of R1 lanes (lane index r) owns one transform; on
+// entry v[m] = x[r + R1 m] (m < R2).  Each lane runs an R2-point FFT in
+// registers, twiddles by w^(r k'), the group exchanges through LDS, and each
+// lane runs R2/R1 R1-point DFTs.  On exit v[j R1 + k2] = X[(r + R1 j) + R2 k2].
+// xch is the workgroup's exchange area of R2 * R1 * GP entries (GP = groups per
+// workgroup + 1); every lane of the workgroup must call this function.
+template <typename T, typename TW, int LEN, int DIR, int GP>
+__device__ __forceinline__ void fft_ml(cx<T> (&v)[MLS<LEN>::R2], int r, int g, cx<T>* xch,
+                                       const cx<TW>* __restrict__ tw) {
+  constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2;
+  stockham<T, TW, R2, DIR, 1, R1>(v, tw);
+  if constexpr (R1 > 1) {
+#pragma unroll
+    for (int k = 1; k < R2; ++k) v[k] = twmul<T, TW, DIR>(v[k], tw[(r * k) % LEN]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < R2; ++k) xch[(k * R1 + r) * GP + g] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R2 / R1; ++j) {
+      const int kp = r + R1 * j;
+      cx<T> u[R1];
+#pragma unroll
+      for (int s = 0; s < R1; ++s) u[s] = xch[(kp * R1 + s) * GP + g];
+      dft_small<R1, DIR, T, TW>(u);
+#pragma unroll
+      for (int k2 = 0; k2 < R1; ++k2) v[j * R1 + k2] = u[k2];
+    }
+  }
+}
+
+// Output position of register p after fft_ml.
+template <int LEN>
+__device__ __forceinline__ int ml_out_pos(int r, int p) {
+  constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2;
+  return (r + R1 * (p / R1)) + R2 * (p % R1);
 }
 
 }  // namespace cfft

@@ -60,107 +60,15 @@ __device__ __forceinline__ void wht8(T* v) {
   }
 }
 
-template <typename T>
-__device__ __forceinline__ cx<T> fold_twiddle(int o, const cx<T>* __restrict__ twM, int x, int y,
-                                              int z, int M0, int M1) {
-  cx<T> t = {T(1), T(0)};
+// exp(-2 pi i off.(x,y,z)/M) of the fold, in the table precision.
+template <typename TW>
+__device__ __forceinline__ cx<TW> fold_twiddle(int o, const cx<TW>* __restrict__ twM, int x, int y,
+                                               int z, int M0, int M1) {
+  cx<TW> t = {TW(1), TW(0)};
   if (o & 1) t = twM[x];
   if (o & 2) t = cmul(t, twM[M0 + y]);
   if (o & 4) t = cmul(t, twM[M0 + M1 + z]);
   return t;
-}
-
-// Centring fold, src/centering_fold.jl:140-257 (optionally fused with the
-// interaction factor exp(-w dt/2) of step_diffusion!).
-template <typename T>
-__global__ void __launch_bounds__(256) k_fold(const T* __restrict__ f0, cx<T>* __restrict__ ch,
-                                              const cx<T>* __restrict__ twM, FoldP p) {
-  const long H3 = (long)p.H0 * p.H1 * p.H2;
-  const long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (tid >= H3 * p.batch) return;
-  const int b = (int)(tid / H3);
-  const long n = tid - b * H3;
-  const int x = (int)(n % p.H0);
-  const int y = (int)((n / p.H0) % p.H1);
-  const int z = (int)(n / ((long)p.H0 * p.H1));
-  const long M3 = (long)p.M0 * p.M1 * p.M2;
-  const T* fb = f0 + b * M3;
-  const T* Eb = p.E_contig ? (p.E[0] ? static_cast<const T*>(p.E[0]) + b * M3 : nullptr)
-                           : static_cast<const T*>(p.E[b]);
-  T v[8];
-#pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    const long idx = (x + (e & 1) * p.H0) +
-                     (long)p.M0 * ((y + ((e >> 1) & 1) * p.H1) + (long)p.M1 * (z + ((e >> 2) & 1) * p.H2));
-    v[e] = Eb ? fb[idx] * Eb[idx] : fb[idx];
-  }
-  wht8(v);
-  cx<T>* cb = ch + (long)b * p.n_ch * H3 + n;
-  for (int c = 0; c < p.n_ch; ++c) {
-    const int o = p.offs[c];
-    const cx<T> tw = fold_twiddle(o, twM, x, y, z, p.M0, p.M1);
-    T val = T(0);
-#pragma unroll
-    for (int e = 0; e < 8; ++e)
-      if (e == o) val = v[e];
-    cb[c * H3] = {val * tw.x, val * tw.y};
-  }
-}
-
-// Centring unfold, src/centering_fold.jl:481-580 (+ interaction factor); the
-// IFFT's 1/H^3 and the inverse WHT's 1/8 are applied here as one scale.
-template <typename T>
-__global__ void __launch_bounds__(256) k_unfold(const cx<T>* __restrict__ ch, T* __restrict__ f0,
-                                                const cx<T>* __restrict__ twM, FoldP p, T scale) {
-  const long H3 = (long)p.H0 * p.H1 * p.H2;
-  const long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (tid >= H3 * p.batch) return;
-  const int b = (int)(tid / H3);
-  const long n = tid - b * H3;
-  const int x = (int)(n % p.H0);
-  const int y = (int)((n / p.H0) % p.H1);
-  const int z = (int)(n / ((long)p.H0 * p.H1));
-  const cx<T>* cb = ch + (long)b * p.n_ch * H3 + n;
-  T v[8];
-#pragma unroll
-  for (int e = 0; e < 8; ++e) v[e] = T(0);
-  for (int c = 0; c < p.n_ch; ++c) {
-    const int o = p.offs[c];
-    const cx<T> tw = fold_twiddle(o, twM, x, y, z, p.M0, p.M1);
-    const cx<T> g = cb[c * H3];
-    const T val = g.x * tw.x + g.y * tw.y;  // Re(g * conj(tw))
-#pragma unroll
-    for (int e = 0; e < 8; ++e)
-      if (e == o) v[e] = val;
-  }
-  wht8(v);
-  const long M3 = (long)p.M0 * p.M1 * p.M2;
-  T* fb = f0 + b * M3;
-  const T* Eb = p.E_contig ? (p.E[0] ? static_cast<const T*>(p.E[0]) + b * M3 : nullptr)
-                           : static_cast<const T*>(p.E[b]);
-#pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    const long idx = (x + (e & 1) * p.H0) +
-                     (long)p.M0 * ((y + ((e >> 1) & 1) * p.H1) + (long)p.M1 * (z + ((e >> 2) & 1) * p.H2));
-    T r = v[e] * scale;
-    fb[idx] = Eb ? r * Eb[idx] : r;
-  }
-}
-
-// Batched 1-D FFT along one axis: transform i reads elements
-// base + j*stride (j < LEN), base = (i % inner) + (i / inner) * outer.
-template <typename T, int LEN, int DIR>
-__global__ void __launch_bounds__(128) k_fft(cx<T>* __restrict__ data, long n_tr, long inner,
-                                             long outer, long stride, const cx<T>* __restrict__ tw) {
-  const long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (tid >= n_tr) return;
-  const long base = (tid % inner) + (tid / inner) * outer;
-  cx<T> a[LEN];
-#pragma unroll
-  for (int j = 0; j < LEN; ++j) a[j] = data[base + j * stride];
-  fft_reg<T, LEN, DIR>(a, tw);
-#pragma unroll
-  for (int j = 0; j < LEN; ++j) data[base + j * stride] = a[j];
 }
 
 __device__ __forceinline__ int pmodi(int a, int n) {
@@ -168,68 +76,81 @@ __device__ __forceinline__ int pmodi(int a, int n) {
   return r < 0 ? r + n : r;
 }
 
-// Forward butterfly reconstruction + spectral K multiply,
-// src/krfft.jl:529-545 and src/q_fused_krfft.jl:1013-1016:
-// F(h) = K(h) sum_a exp(-2 pi i h.t_a/N) G0(R_a^T h mod M).
 template <typename T>
-__global__ void __launch_bounds__(256) k_recon(const cx<T>* __restrict__ ch, cx<T>* __restrict__ F,
-                                               const int* __restrict__ hs, const T* __restrict__ K,
-                                               const cx<T>* __restrict__ W, ReconP p) {
-  const long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (tid >= (long)p.n_spec * p.batch) return;
-  const int b = (int)(tid / p.n_spec);
-  const int i = (int)(tid - (long)b * p.n_spec);
-  const int h0 = hs[3 * i], h1 = hs[3 * i + 1], h2 = hs[3 * i + 2];
-  const long H3 = (long)p.H[0] * p.H[1] * p.H[2];
-  const cx<T>* cb = ch + (long)b * p.n_ch * H3;
-  cx<T> acc = {T(0), T(0)};
+__device__ __forceinline__ const T* fold_E(const FoldP& p, int b, long M3) {
+  return p.E_contig ? (p.E[0] ? static_cast<const T*>(p.E[0]) + b * M3 : nullptr)
+                    : static_cast<const T*>(p.E[b]);
+}
+
+// Workgroup geometry of the multi-lane kernels: R1 lanes per transform.
+template <int LEN>
+struct MLGeom {
+  static constexpr int WG = (MLS<LEN>::R2 >= 16) ? 128 : 256;
+  static constexpr int G = WG / MLS<LEN>::R1;  // transforms per workgroup
+  static constexpr int GP = G + 1;              // padded LDS row
+  static constexpr int XCH = MLS<LEN>::R1 > 1 ? MLS<LEN>::R2 * MLS<LEN>::R1 * GP : 1;
+};
+
+// Batched multi-lane FFT along y or z: transform i reads elements
+// base + j*stride (j < LEN), base = (i % inner) + (i / inner) * outer.
+template <typename T, typename TW, int LEN, int DIR>
+__global__ void __launch_bounds__(MLGeom<LEN>::WG)
+    k_fft_ml(cx<T>* __restrict__ data, long n_tr, long inner, long outer, long stride,
+             const cx<TW>* __restrict__ tw) {
+  constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2, G = MLGeom<LEN>::G, GP = MLGeom<LEN>::GP;
+  __shared__ cx<T> xch[MLGeom<LEN>::XCH];
+  const int r = threadIdx.x % R1, g = threadIdx.x / R1;
+  const long gid = (long)blockIdx.x * G + g;
+  const bool valid = gid < n_tr;
+  const long base = valid ? (gid % inner) + (gid / inner) * outer : 0;
+  cx<T> a[R2];
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    if (a < p.d) {
-      const int q0 = pmodi(p.R[a][0] * h0 + p.R[a][3] * h1 + p.R[a][6] * h2, p.M[0]);
-      const int q1 = pmodi(p.R[a][1] * h0 + p.R[a][4] * h1 + p.R[a][7] * h2, p.M[1]);
-      const int q2 = pmodi(p.R[a][2] * h0 + p.R[a][5] * h1 + p.R[a][8] * h2, p.M[2]);
-      const int c = p.chmap[(q0 & 1) | ((q1 & 1) << 1) | ((q2 & 1) << 2)];
-      if (c >= 0) {
-        const int j = h0 * p.t[a][0] * p.Pn[0] + h1 * p.t[a][1] * p.Pn[1] + h2 * p.t[a][2] * p.Pn[2];
-        const cx<T> w = W[pmodi(-j, p.P)];
-        const long k = (q0 >> 1) + (long)p.H[0] * ((q1 >> 1) + (long)p.H[1] * (q2 >> 1));
-        acc = cadd(acc, cmul(w, cb[c * H3 + k]));
-      }
-    }
+  for (int m = 0; m < R2; ++m) a[m] = valid ? data[base + (long)(r + R1 * m) * stride] : cx<T>{T(0), T(0)};
+  fft_ml<T, TW, LEN, DIR, GP>(a, r, g, xch, tw);
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < R2; ++q) data[base + (long)ml_out_pos<LEN>(r, q) * stride] = a[q];
   }
-  const T kk = K ? K[i] : T(1);
-  F[tid] = {acc.x * kk, acc.y * kk};
+}
+
+// Sum over the 8 lanes of an aligned lane octet.
+template <typename T>
+__device__ __forceinline__ cx<T> octet_sum(cx<T> v) {
+#pragma unroll
+  for (int off = 4; off >= 1; off >>= 1) {
+    v.x += __shfl_xor(v.x, off);
+    v.y += __shfl_xor(v.y, off);
+  }
+  return v;
 }
 
 // Inverse reconstruction on the alive parities, src/krfft.jl:1285-1311 and
 // src/centering_fold.jl:782-813: G0(q) = (1/d) sum_alpha F(q + M alpha), the
-// full-grid coefficients read through the spectral reverse lookup.
-// Table entry: bits 0-19 spectral index (0xFFFFF = extinct), 20-30 phase
-// exponent j (phase exp(2 pi i j/P)), bit 31 conjugate flag.
-template <typename T>
+// full-grid coefficients read through the spectral reverse lookup.  Table
+// layout [alive q][8 aliases]; entry bits 0-19 spectral index (0xFFFFF =
+// extinct), 20-30 phase exponent j (phase exp(2 pi i j/P)), bit 31 conjugate.
+template <typename T, typename TW>
 __global__ void __launch_bounds__(256) k_irecon(const cx<T>* __restrict__ F, cx<T>* __restrict__ ch,
                                                 const uint32_t* __restrict__ tab,
-                                                const cx<T>* __restrict__ W, IreconP p, T scale) {
+                                                const cx<TW>* __restrict__ W, IreconP p, TW scale) {
   const long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (tid >= p.n_alive * p.batch) return;
-  const int b = (int)(tid / p.n_alive);
-  const long e = tid - (long)b * p.n_alive;
-  const cx<T>* Fb = F + (long)b * p.n_spec;
-  cx<T> acc = {T(0), T(0)};
-#pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    if (a < p.d) {
-      const uint32_t ent = tab[a * p.n_alive + e];
-      const uint32_t idx = ent & 0xFFFFFu;
-      if (idx != 0xFFFFFu) {
-        cx<T> v = Fb[idx];
-        if (ent >> 31) v.y = -v.y;
-        acc = cadd(acc, cmul(v, W[(ent >> 20) & 0x7FFu]));
-      }
+  const int a = (int)(tid & 7);
+  const long q = tid >> 3;
+  const bool valid = q < p.n_alive * p.batch;
+  const int b = valid ? (int)(q / p.n_alive) : 0;
+  const long e = valid ? q - (long)b * p.n_alive : 0;
+  cx<T> v = {T(0), T(0)};
+  if (valid) {
+    const uint32_t ent = tab[e * 8 + a];
+    const uint32_t idx = ent & 0xFFFFFu;
+    if (idx != 0xFFFFFu) {
+      cx<T> f = F[(long)b * p.n_spec + idx];
+      if (ent >> 31) f.y = -f.y;
+      v = twmul<T, TW, -1>(f, W[(ent >> 20) & 0x7FFu]);
     }
   }
-  ch[tid] = {acc.x * scale, acc.y * scale};
+  v = octet_sum(v);
+  if (valid && a == 0) ch[q] = {(T)((TW)v.x * scale), (T)((TW)v.y * scale)};
 }
 
 // Q-fused multiply on the alive parities, src/q_fused_krfft.jl:581-592:
@@ -276,7 +197,8 @@ __global__ void k_sub_to_full(const T* __restrict__ f0, T* __restrict__ u, const
 }
 
 // G0 assembly (src/centering_fold.jl:277-296) for inspection: M-grid
-// spectrum from the channels, zeros at extinct parities.
+// spectrum from the channels (layout [b][c][x][z][y]), zeros at extinct
+// parities.
 template <typename T>
 __global__ void k_assemble(const cx<T>* __restrict__ ch, cx<T>* __restrict__ G0, FoldP p,
                            const int* __restrict__ chmap) {
@@ -289,7 +211,8 @@ __global__ void k_assemble(const cx<T>* __restrict__ ch, cx<T>* __restrict__ G0,
   const int c = chmap[(q0 & 1) | ((q1 & 1) << 1) | ((q2 & 1) << 2)];
   const long H3 = (long)p.H0 * p.H1 * p.H2;
   cx<T> v = {T(0), T(0)};
-  if (c >= 0) v = ch[((long)b * p.n_ch + c) * H3 + (q0 >> 1) + (long)p.H0 * ((q1 >> 1) + (long)p.H1 * (q2 >> 1))];
+  if (c >= 0)
+    v = ch[((long)b * p.n_ch + c) * H3 + ((long)(q0 >> 1) * p.H2 + (q2 >> 1)) * p.H1 + (q1 >> 1)];
   G0[tid] = v;
 }
 

@@ -1,0 +1,379 @@
+// Five-phase KRFFT / diffusion pipeline on LDS tiles and planes (gfx950).
+//
+// Channel array layout ("XZY"): ch[b][c][x][z][y], y fastest, over the H-grid
+// (H = M/2).  A (c, x) slice is one contiguous H2 x H1 plane, so the y and z
+// transforms of a plane run in one workgroup out of LDS.
+//
+//   P1  fold + E + FFT x         tile (b, z, y-block)      f0 -> ch
+//   P2  FFT y, FFT z             plane (b, c, x)           ch -> ch
+//   P3  reconstruction + K       8 lanes per point         ch -> F
+//   P4  inverse recon + IFFT z, IFFT y   plane (b, c, x)   F -> ch
+//   P5  IFFT x + unfold + E      tile (b, z, y-block)      ch -> f0
+//
+// Each phase is a device function over one work item, used both by one
+// kernel per phase and by the persistent sweep kernel (grid-wide barriers
+// between phases).
+#pragma once
+
+#include <hip/hip_cooperative_groups.h>
+#include <hip/hip_runtime.h>
+
+#include "fft.hpp"
+#include "kernels.hpp"
+
+namespace cfft {
+
+constexpr int kNT = 256;  // threads per workgroup of the phase kernels
+
+struct PipeP {
+  int M[3], H[3];
+  int n_ch, offs[8], chmap[8];
+  int batch, YB;  // y rows per P1/P5 tile (divides H1)
+  int n_spec, d, P, Pn[3];
+  int R[8][9], t[8][3];
+  // interaction factors: entry b uses E1[b] for steps < sw[b], else E2[b]
+  int E_contig;
+  const void* E1[kMaxEPtr];
+  const void* E2[kMaxEPtr];
+  int sw[kMaxEPtr];
+  double scale_unfold;  // 1 / (8 H0 H1 H2)
+  double scale_irecon;  // 1 / d
+};
+
+struct PipeBufs {
+  const void* f0_in;  // P1 input
+  void* f0_out;       // P5 output
+  void* ch;           // channels
+  void* F;            // spectral coefficients
+  const int* hs;      // signed spectral frequencies (3 per point)
+  const double* K;    // kernel (nullptr: 1)
+  const void* W;      // exp(2 pi i j / P)
+  const void* twM;    // fold twiddles (M0 + M1 + M2)
+  const void* tw0;    // FFT twiddles per axis
+  const void* tw1;
+  const void* tw2;
+  const uint32_t* btab;  // inverse reconstruction table [e][8]
+};
+
+template <int LEN>
+struct PipeG {
+  static constexpr int R1 = MLS<LEN>::R1;
+  static constexpr int G = kNT / R1;
+  static constexpr int GP = G + 1;
+  static constexpr int XCH = R1 > 1 ? MLS<LEN>::R2 * R1 * GP : 1;
+};
+
+// FFT of n_tr length-LEN transforms held in LDS: element j of transform t is
+// buf[t * ts + j * es].  Every thread of the workgroup must call.
+template <typename T, typename TW, int LEN, int DIR>
+__device__ __forceinline__ void lds_fft(cx<T>* buf, int n_tr, int ts, int es, cx<T>* xch,
+                                        const cx<TW>* __restrict__ tw) {
+  if constexpr (LEN > 1) {
+    constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2, G = PipeG<LEN>::G, GP = PipeG<LEN>::GP;
+    const int r = threadIdx.x % R1, g = threadIdx.x / R1;
+    for (int t0 = 0; t0 < n_tr; t0 += G) {
+      const int t = t0 + g;
+      const bool ok = t < n_tr;
+      cx<T> v[R2];
+#pragma unroll
+      for (int m = 0; m < R2; ++m) v[m] = ok ? buf[t * ts + (r + R1 * m) * es] : cx<T>{T(0), T(0)};
+      fft_ml<T, TW, LEN, DIR, GP>(v, r, g, xch, tw);
+      if (ok) {
+#pragma unroll
+        for (int q = 0; q < R2; ++q) buf[t * ts + ml_out_pos<LEN>(r, q) * es] = v[q];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ const T* pipe_E(const PipeP& p, int b, int step, long M3) {
+  if (p.E_contig) return p.E1[0] ? static_cast<const T*>(p.E1[0]) + b * M3 : nullptr;
+  return static_cast<const T*>(step < p.sw[b] ? p.E1[b] : p.E2[b]);
+}
+
+__device__ __forceinline__ long corner_index(const PipeP& p, int x, int y, int z, int e) {
+  return (x + (e & 1) * p.H[0]) +
+         (long)p.M[0] * ((y + ((e >> 1) & 1) * p.H[1]) + (long)p.M[1] * (z + ((e >> 2) & 1) * p.H[2]));
+}
+
+// ---- P1: fold + E + FFT x --------------------------------------------------
+// tile item = (b, z, y-block); LDS A[c][yy][x] with row stride H0 + 1.
+template <typename T, typename TW, int L0>
+__device__ void ph_fold_fftx(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds,
+                             cx<T>* xch) {
+  const int H0 = p.H[0], H1 = p.H[1], H2 = p.H[2], YB = p.YB;
+  const int nyb = H1 / YB;
+  const int yb = (int)(item % nyb);
+  const int z = (int)((item / nyb) % H2);
+  const int b = (int)(item / ((long)nyb * H2));
+  const int y0 = yb * YB;
+  const long M3 = (long)p.M[0] * p.M[1] * p.M[2];
+  const T* fb = static_cast<const T*>(B.f0_in) + b * M3;
+  const T* Eb = pipe_E<T>(p, b, step, M3);
+  const cx<TW>* twM = static_cast<const cx<TW>*>(B.twM);
+  const int rs = H0 + 1;
+  for (int pt = threadIdx.x; pt < H0 * YB; pt += kNT) {
+    const int x = pt % H0, yy = pt / H0, y = y0 + yy;
+    T v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const long idx = corner_index(p, x, y, z, e);
+      v[e] = Eb ? fb[idx] * Eb[idx] : fb[idx];
+    }
+    wht8(v);
+    for (int c = 0; c < p.n_ch; ++c) {
+      const int o = p.offs[c];
+      T val = T(0);
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        if (e == o) val = v[e];
+      const cx<TW> tw = fold_twiddle(o, twM, x, y, z, p.M[0], p.M[1]);
+      lds[(c * YB + yy) * rs + x] = {(T)((TW)val * tw.x), (T)((TW)val * tw.y)};
+    }
+  }
+  __syncthreads();
+  lds_fft<T, TW, L0, -1>(lds, p.n_ch * YB, rs, 1, xch, static_cast<const cx<TW>*>(B.tw0));
+  // store to ch[b][c][x][z][y] (y fastest within a tile row)
+  cx<T>* ch = static_cast<cx<T>*>(B.ch);
+  const int n = p.n_ch * H0 * YB;
+  for (int i = threadIdx.x; i < n; i += kNT) {
+    const int yy = i % YB, x = (i / YB) % H0, c = i / (YB * H0);
+    ch[(((long)b * p.n_ch + c) * H0 + x) * ((long)H2 * H1) + (long)z * H1 + y0 + yy] =
+        lds[(c * YB + yy) * rs + x];
+  }
+  __syncthreads();
+}
+
+// ---- P2: FFT y then FFT z of one (b, c, x) plane ----------------------------
+template <typename T, typename TW, int L1, int L2>
+__device__ void ph_fft_yz(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds, cx<T>* xch) {
+  const int H1 = p.H[1], H2 = p.H[2];
+  const int rs = H1 + 1;
+  cx<T>* plane = static_cast<cx<T>*>(B.ch) + item * ((long)H1 * H2);
+  for (int i = threadIdx.x; i < H1 * H2; i += kNT) lds[(i / H1) * rs + (i % H1)] = plane[i];
+  __syncthreads();
+  lds_fft<T, TW, L1, -1>(lds, H2, rs, 1, xch, static_cast<const cx<TW>*>(B.tw1));
+  lds_fft<T, TW, L2, -1>(lds, H1, 1, rs, xch, static_cast<const cx<TW>*>(B.tw2));
+  for (int i = threadIdx.x; i < H1 * H2; i += kNT) plane[i] = lds[(i / H1) * rs + (i % H1)];
+  __syncthreads();
+}
+
+// ---- P3: forward reconstruction + K (one lane octet per spectral point) -----
+// F(h) = K(h) sum_a exp(-2 pi i h.t_a/N) G0(R_a^T h mod M)   (src/krfft.jl:529-545)
+template <typename T, typename TW>
+__device__ __forceinline__ void ph_recon_point(const PipeP& p, const PipeBufs& B, long pt, bool valid) {
+  const int a = threadIdx.x & 7;
+  const int b = valid ? (int)(pt / p.n_spec) : 0;
+  const int i = valid ? (int)(pt - (long)b * p.n_spec) : 0;
+  cx<T> v = {T(0), T(0)};
+  if (valid && a < p.d) {
+    const int* hs = B.hs;
+    const int h0 = hs[3 * i], h1 = hs[3 * i + 1], h2 = hs[3 * i + 2];
+    const int q0 = pmodi(p.R[a][0] * h0 + p.R[a][3] * h1 + p.R[a][6] * h2, p.M[0]);
+    const int q1 = pmodi(p.R[a][1] * h0 + p.R[a][4] * h1 + p.R[a][7] * h2, p.M[1]);
+    const int q2 = pmodi(p.R[a][2] * h0 + p.R[a][5] * h1 + p.R[a][8] * h2, p.M[2]);
+    const int c = p.chmap[(q0 & 1) | ((q1 & 1) << 1) | ((q2 & 1) << 2)];
+    if (c >= 0) {
+      const int j = h0 * p.t[a][0] * p.Pn[0] + h1 * p.t[a][1] * p.Pn[1] + h2 * p.t[a][2] * p.Pn[2];
+      const long H3 = (long)p.H[0] * p.H[1] * p.H[2];
+      const long k = ((long)(q0 >> 1) * p.H[2] + (q2 >> 1)) * p.H[1] + (q1 >> 1);
+      const cx<T>* ch = static_cast<const cx<T>*>(B.ch);
+      v = twmul<T, TW, +1>(ch[((long)b * p.n_ch + c) * H3 + k],
+                           static_cast<const cx<TW>*>(B.W)[pmodi(j, p.P)]);
+    }
+  }
+  v = octet_sum(v);
+  if (valid && a == 0) {
+    const TW kk = B.K ? (TW)B.K[i] : TW(1);
+    static_cast<cx<T>*>(B.F)[pt] = {(T)((TW)v.x * kk), (T)((TW)v.y * kk)};
+  }
+}
+
+// ---- P4: inverse reconstruction, IFFT z, IFFT y of one (b, c, x) plane ------
+// G0(q) = (1/d) sum_alpha F(q + M alpha) through the reverse-lookup table
+// (src/krfft.jl:1285-1311, src/centering_fold.jl:782-813).
+template <typename T, typename TW, int L1, int L2>
+__device__ void ph_irecon_ifft_zy(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds,
+                                  cx<T>* xch) {
+  const int H1 = p.H[1], H2 = p.H[2];
+  const int rs = H1 + 1;
+  const long H3 = (long)p.H[0] * H1 * H2;
+  const long n_alive = (long)p.n_ch * H3;
+  const int b = (int)(item / ((long)p.n_ch * p.H[0]));
+  const long e0 = item * ((long)H1 * H2) - (long)b * n_alive;  // channel-space start
+  const cx<T>* Fb = static_cast<const cx<T>*>(B.F) + (long)b * p.n_spec;
+  const cx<TW>* W = static_cast<const cx<TW>*>(B.W);
+  const uint4* tab4 = reinterpret_cast<const uint4*>(B.btab);
+  for (int i = threadIdx.x; i < H1 * H2; i += kNT) {
+    const long e = e0 + i;
+    const uint4 ta = tab4[2 * e], tb = tab4[2 * e + 1];
+    const uint32_t ent[8] = {ta.x, ta.y, ta.z, ta.w, tb.x, tb.y, tb.z, tb.w};
+    cx<T> acc = {T(0), T(0)};
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const uint32_t idx = ent[a] & 0xFFFFFu;
+      const bool ok = idx != 0xFFFFFu;
+      cx<T> f = Fb[ok ? idx : 0];
+      if (ent[a] >> 31) f.y = -f.y;
+      const cx<T> prod = twmul<T, TW, -1>(f, W[(ent[a] >> 20) & 0x7FFu]);
+      if (ok) acc = cadd(acc, prod);
+    }
+    lds[(i / H1) * rs + (i % H1)] = {(T)((TW)acc.x * p.scale_irecon), (T)((TW)acc.y * p.scale_irecon)};
+  }
+  __syncthreads();
+  lds_fft<T, TW, L2, +1>(lds, H1, 1, rs, xch, static_cast<const cx<TW>*>(B.tw2));
+  lds_fft<T, TW, L1, +1>(lds, H2, rs, 1, xch, static_cast<const cx<TW>*>(B.tw1));
+  cx<T>* plane = static_cast<cx<T>*>(B.ch) + item * ((long)H1 * H2);
+  for (int i = threadIdx.x; i < H1 * H2; i += kNT) plane[i] = lds[(i / H1) * rs + (i % H1)];
+  __syncthreads();
+}
+
+// IFFT z, IFFT y of one plane of `src` written to `dst` (Q-fused path).
+template <typename T, typename TW, int L1, int L2>
+__device__ void ph_ifft_zy(const PipeP& p, const PipeBufs& B, const cx<T>* src, long item,
+                           cx<T>* lds, cx<T>* xch) {
+  const int H1 = p.H[1], H2 = p.H[2];
+  const int rs = H1 + 1;
+  const cx<T>* in = src + item * ((long)H1 * H2);
+  for (int i = threadIdx.x; i < H1 * H2; i += kNT) lds[(i / H1) * rs + (i % H1)] = in[i];
+  __syncthreads();
+  lds_fft<T, TW, L2, +1>(lds, H1, 1, rs, xch, static_cast<const cx<TW>*>(B.tw2));
+  lds_fft<T, TW, L1, +1>(lds, H2, rs, 1, xch, static_cast<const cx<TW>*>(B.tw1));
+  cx<T>* plane = static_cast<cx<T>*>(B.ch) + item * ((long)H1 * H2);
+  for (int i = threadIdx.x; i < H1 * H2; i += kNT) plane[i] = lds[(i / H1) * rs + (i % H1)];
+  __syncthreads();
+}
+
+// ---- P5: IFFT x + unfold + E (tile (b, z, y-block)) ------------------------
+// f0(x + H eps) = scale * sum_c (-1)^(off_c.eps) Re(g_c(x) conj(tw_c))
+// (src/centering_fold.jl:481-580); scale = 1/(8 H^3) (IFFT and inverse WHT).
+template <typename T, typename TW, int L0>
+__device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds,
+                                cx<T>* xch) {
+  const int H0 = p.H[0], H1 = p.H[1], H2 = p.H[2], YB = p.YB;
+  const int nyb = H1 / YB;
+  const int yb = (int)(item % nyb);
+  const int z = (int)((item / nyb) % H2);
+  const int b = (int)(item / ((long)nyb * H2));
+  const int y0 = yb * YB;
+  const int rs = H0 + 1;
+  const cx<T>* ch = static_cast<const cx<T>*>(B.ch);
+  const int n = p.n_ch * H0 * YB;
+  for (int i = threadIdx.x; i < n; i += kNT) {
+    const int yy = i % YB, x = (i / YB) % H0, c = i / (YB * H0);
+    lds[(c * YB + yy) * rs + x] =
+        ch[(((long)b * p.n_ch + c) * H0 + x) * ((long)H2 * H1) + (long)z * H1 + y0 + yy];
+  }
+  __syncthreads();
+  lds_fft<T, TW, L0, +1>(lds, p.n_ch * YB, rs, 1, xch, static_cast<const cx<TW>*>(B.tw0));
+  const long M3 = (long)p.M[0] * p.M[1] * p.M[2];
+  T* fb = static_cast<T*>(B.f0_out) + b * M3;
+  const T* Eb = pipe_E<T>(p, b, step, M3);
+  const cx<TW>* twM = static_cast<const cx<TW>*>(B.twM);
+  for (int pt = threadIdx.x; pt < H0 * YB; pt += kNT) {
+    const int x = pt % H0, yy = pt / H0, y = y0 + yy;
+    T v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = T(0);
+    for (int c = 0; c < p.n_ch; ++c) {
+      const int o = p.offs[c];
+      const cx<TW> tw = fold_twiddle(o, twM, x, y, z, p.M[0], p.M[1]);
+      const cx<T> g = lds[(c * YB + yy) * rs + x];
+      const T val = (T)((TW)g.x * tw.x + (TW)g.y * tw.y);  // Re(g conj(tw))
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        if (e == o) v[e] = val;
+    }
+    wht8(v);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const long idx = corner_index(p, x, y, z, e);
+      const T r = (T)((TW)v[e] * p.scale_unfold);
+      fb[idx] = Eb ? r * Eb[idx] : r;
+    }
+  }
+  __syncthreads();
+}
+
+// ---- kernels: one per phase -------------------------------------------------
+template <typename T, typename TW, int L0>
+__global__ void __launch_bounds__(kNT) k_p1(PipeP p, PipeBufs B, int step) {
+  __shared__ cx<T> lds[64 * (L0 + 1)];
+  __shared__ cx<T> xch[PipeG<L0>::XCH];
+  ph_fold_fftx<T, TW, L0>(p, B, blockIdx.x, step, lds, xch);
+}
+
+template <typename T, typename TW, int L1, int L2>
+__global__ void __launch_bounds__(kNT) k_p2(PipeP p, PipeBufs B) {
+  constexpr int X = PipeG<L1>::XCH > PipeG<L2>::XCH ? PipeG<L1>::XCH : PipeG<L2>::XCH;
+  __shared__ cx<T> lds[L2 * (L1 + 1)];
+  __shared__ cx<T> xch[X];
+  ph_fft_yz<T, TW, L1, L2>(p, B, blockIdx.x, lds, xch);
+}
+
+template <typename T, typename TW>
+__global__ void __launch_bounds__(kNT) k_p3(PipeP p, PipeBufs B) {
+  const long pt = ((long)blockIdx.x * kNT + threadIdx.x) >> 3;
+  ph_recon_point<T, TW>(p, B, pt, pt < (long)p.n_spec * p.batch);
+}
+
+template <typename T, typename TW, int L1, int L2>
+__global__ void __launch_bounds__(kNT) k_p4(PipeP p, PipeBufs B) {
+  constexpr int X = PipeG<L1>::XCH > PipeG<L2>::XCH ? PipeG<L1>::XCH : PipeG<L2>::XCH;
+  __shared__ cx<T> lds[L2 * (L1 + 1)];
+  __shared__ cx<T> xch[X];
+  ph_irecon_ifft_zy<T, TW, L1, L2>(p, B, blockIdx.x, lds, xch);
+}
+
+template <typename T, typename TW, int L1, int L2>
+__global__ void __launch_bounds__(kNT) k_p4q(PipeP p, PipeBufs B, const cx<T>* src) {
+  constexpr int X = PipeG<L1>::XCH > PipeG<L2>::XCH ? PipeG<L1>::XCH : PipeG<L2>::XCH;
+  __shared__ cx<T> lds[L2 * (L1 + 1)];
+  __shared__ cx<T> xch[X];
+  ph_ifft_zy<T, TW, L1, L2>(p, B, src, blockIdx.x, lds, xch);
+}
+
+template <typename T, typename TW, int L0>
+__global__ void __launch_bounds__(kNT) k_p5(PipeP p, PipeBufs B, int step) {
+  __shared__ cx<T> lds[64 * (L0 + 1)];
+  __shared__ cx<T> xch[PipeG<L0>::XCH];
+  ph_ifftx_unfold<T, TW, L0>(p, B, blockIdx.x, step, lds, xch);
+}
+
+// ---- persistent sweep: nsteps Strang steps in one cooperative launch ---------
+// f0 is advanced in place (P1 reads it, P5 overwrites it after the barrier
+// that follows P4).  Work items are distributed grid-stride over the resident
+// workgroups; every workgroup reaches every barrier.
+template <typename T, typename TW, int L>
+__global__ void __launch_bounds__(kNT) k_sweep(PipeP p, PipeBufs B, int step0, int nsteps) {
+  namespace cg = cooperative_groups;
+  cg::grid_group grid = cg::this_grid();
+  constexpr int TILE = 64 * (L + 1);
+  constexpr int PLANE = L * (L + 1);
+  __shared__ cx<T> lds[TILE > PLANE ? TILE : PLANE];
+  __shared__ cx<T> xch[PipeG<L>::XCH];
+  const long n_tiles = (long)p.batch * p.H[2] * (p.H[1] / p.YB);
+  const long n_planes = (long)p.batch * p.n_ch * p.H[0];
+  const long n_pts = (long)p.n_spec * p.batch;
+  const long n_oct = (n_pts * 8 + kNT - 1) / kNT;  // workgroup-chunks of P3
+  for (int s = 0; s < nsteps; ++s) {
+    const int step = step0 + s;
+    for (long it = blockIdx.x; it < n_tiles; it += gridDim.x) ph_fold_fftx<T, TW, L>(p, B, it, step, lds, xch);
+    grid.sync();
+    for (long it = blockIdx.x; it < n_planes; it += gridDim.x) ph_fft_yz<T, TW, L, L>(p, B, it, lds, xch);
+    grid.sync();
+    for (long it = blockIdx.x; it < n_oct; it += gridDim.x) {
+      const long pt = (it * kNT + threadIdx.x) >> 3;
+      ph_recon_point<T, TW>(p, B, pt, pt < n_pts);
+    }
+    grid.sync();
+    for (long it = blockIdx.x; it < n_planes; it += gridDim.x) ph_irecon_ifft_zy<T, TW, L, L>(p, B, it, lds, xch);
+    grid.sync();
+    for (long it = blockIdx.x; it < n_tiles; it += gridDim.x) ph_ifftx_unfold<T, TW, L>(p, B, it, step, lds, xch);
+    grid.sync();
+  }
+}
+
+}  // namespace cfft

@@ -166,22 +166,31 @@ int cfft_step_dev(cfft_plan* plan, void* q, const void* E, int32_t nsteps, void*
  * between w_A and w_B along the contour independently for q and q+. */
 int cfft_step_ptrs_dev(cfft_plan* plan, void* q, const void* const* E, int32_t nsteps,
                        void* stream);
+/* Contour sweep: nsteps Strang steps q <- E D (E q), batch entry b using
+ * E1[b] at absolute step indices < sw[b] and E2[b] after (host arrays of
+ * `batch` device pointers / ints, batch <= 64; steps are numbered from step0).
+ * No counterpart in the reference (it has no SCFT driver); on cubic H-grids
+ * the whole sweep is one cooperative launch with grid barriers between the
+ * pipeline phases. */
+int cfft_sweep_dev(cfft_plan* plan, void* q, const void* const* E1, const void* const* E2,
+                   const int32_t* sw, int32_t step0, int32_t nsteps, void* stream);
 /* E <- exp(-w * factor), n entries of the plan precision. */
 int cfft_exp_dev(cfft_plan* plan, const void* w, void* E, double factor, int64_t n, void* stream);
 
 /* ---- per-stage timer (hipEvents around each kernel launch) ------------- */
-#define CFFT_STAGE_FOLD 0
-#define CFFT_STAGE_FFT_X 1
-#define CFFT_STAGE_FFT_Y 2
-#define CFFT_STAGE_FFT_Z 3
-#define CFFT_STAGE_RECON 4
-#define CFFT_STAGE_IRECON 5
-#define CFFT_STAGE_IFFT_X 6
-#define CFFT_STAGE_IFFT_Y 7
-#define CFFT_STAGE_IFFT_Z 8
-#define CFFT_STAGE_UNFOLD 9
+#define CFFT_STAGE_FOLD 0   /* P1: fold + E + FFT x                 */
+#define CFFT_STAGE_FFT_X 1  /* (unused: FFT x is fused into P1)      */
+#define CFFT_STAGE_FFT_Y 2  /* P2: FFT y + FFT z of a plane          */
+#define CFFT_STAGE_FFT_Z 3  /* z pass of the non-cubic path          */
+#define CFFT_STAGE_RECON 4  /* P3: reconstruction + K                */
+#define CFFT_STAGE_IRECON 5 /* P4: inverse recon + IFFT z + IFFT y   */
+#define CFFT_STAGE_IFFT_X 6 /* (unused)                              */
+#define CFFT_STAGE_IFFT_Y 7 /* y pass of the non-cubic path          */
+#define CFFT_STAGE_IFFT_Z 8 /* z pass (non-cubic) / Q-path IFFT z,y  */
+#define CFFT_STAGE_UNFOLD 9 /* P5: IFFT x + unfold + E               */
 #define CFFT_STAGE_QMUL 10
-#define CFFT_N_STAGES 11
+#define CFFT_STAGE_SWEEP 11 /* persistent sweep (all phases, all steps) */
+#define CFFT_N_STAGES 12
 /* No counterpart in the reference (its benches time stages with @elapsed,
  * test/profile_centering_fold.jl:48-107).  enable != 0 brackets every
  * subsequent kernel launch of the plan with a pair of hipEvents. */
@@ -189,6 +198,14 @@ int cfft_profile(cfft_plan* plan, int32_t enable);
 /* Sum of the recorded kernel durations (ms_out[CFFT_N_STAGES]) and launch
  * counts per stage since the last read; clears the record. */
 int cfft_profile_read(cfft_plan* plan, double* ms_out, int64_t* count_out);
+
+/* ---- plan options -------------------------------------------------------- */
+/* value != 0: run cfft_sweep_dev / cfft_step_dev on cubic H-grids as one
+ * cooperative launch with grid barriers between phases (default 0: one
+ * launch per phase and step). */
+#define CFFT_OPT_PERSISTENT 1
+/* No counterpart in the reference. */
+int cfft_plan_set_option(cfft_plan* plan, int32_t key, int64_t value);
 
 /* Synchronise the plan's stream. */
 int cfft_sync(cfft_plan* plan);
