@@ -22,18 +22,39 @@ FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = [os.path.join(CSRC, "cfft_hip.hip"), os.path.join(CSRC, "plan.cpp")]
+SOURCES = [os.path.join(CSRC, f) for f in ("cfft_hip.hip", "blocks.hip", "plan.cpp")]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    return subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
 
 
 def build(verbose: bool = True) -> str:
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *SOURCES]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
+    """Compile every source to an object in parallel, then link."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs, procs = [], []
+    cflags = [f for f in FLAGS if f != "-shared"]
+    for src in SOURCES:
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        objs.append(obj)
+        procs.append((src, _run([HIPCC, *cflags, "-c", "-o", obj, src], verbose)))
+    failed = False
+    for src, pr in procs:
+        out, _ = pr.communicate()
+        if pr.returncode != 0:
+            sys.stderr.write(out)
+            failed = True
+    if failed:
         raise RuntimeError("hipcc failed building libcfft_hip.so")
+    tmp = LIB + ".tmp"
+    pr = _run([HIPCC, *FLAGS, "-o", tmp, *objs], verbose)
+    out, _ = pr.communicate()
+    if pr.returncode != 0:
+        sys.stderr.write(out)
+        raise RuntimeError("hipcc failed linking libcfft_hip.so")
     os.replace(tmp, LIB)
     return LIB
 

@@ -10,6 +10,7 @@ from .symmetry_ops import (CentA, CentC, CentF, CentI, CentP, CenteringType, Sym
                            check_shift_invariance, convert_op, detect_centering_type, dual_ops,
                            get_ops, ops_arrays, pack_ops, strip_centering)
 from .plans import (ASUPoint, CenteredKRFFTBackwardPlan, CenteredKRFFTPlan, CenteredSCFTPlan,
+                    CenteringPreFoldKRFFTPlan,
                     DevicePlan, GeneralCFFTPlan, M2BackwardPlan, M2QPlan, M2SCFTPlan,
                     M7SCFTPlan, SpectralIndexing, auto_L, build_fill_map, calc_spectral_asu,
                     count_special_positions, detect_centering, execute_centered_ikrfft,
@@ -21,6 +22,8 @@ from .plans import (ASUPoint, CenteredKRFFTBackwardPlan, CenteredKRFFTPlan, Cent
                     plan_centered_scft, plan_krfft, plan_krfft_centered, plan_m2_backward,
                     plan_m2_q, plan_m2_scft, plan_m7_scft, select_rep_ops, subgrid_to_fullgrid,
                     update_kernel, update_m2_kernel)
+from .asu import (ASUBlock, CFFTPlan, CrystallographicASU, calc_asu, ldiv, mul, pack_asu,
+                  plan_cfft)
 from . import scft  # noqa: F401
 from .diffusion_solver import (DiffusionSolver, QFusedDiffusionSolver, apply_diffusion_operator,
                                interaction_step, plan_diffusion, step_diffusion)

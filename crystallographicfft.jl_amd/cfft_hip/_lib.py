@@ -40,6 +40,10 @@ EXPORTED = [
     "cfft_forward_dev", "cfft_backward_dev", "cfft_diffuse_dev", "cfft_step_dev",
     "cfft_exp_dev", "cfft_sync", "cfft_step_ptrs_dev", "cfft_profile", "cfft_profile_read",
     "cfft_sweep_dev", "cfft_plan_set_option",
+    "cfft_calc_asu", "cfft_pack_asu", "cfft_block_plan_create", "cfft_block_plan_info",
+    "cfft_block_forward", "cfft_block_inverse", "cfft_block_plan_destroy",
+    "cfft_prefold_plan_create", "cfft_prefold_plan_info", "cfft_prefold_forward",
+    "cfft_prefold_plan_destroy",
 ]
 
 OPT_PERSISTENT = 1
@@ -104,6 +108,17 @@ def _declare(lib):
         "cfft_profile_read": (C.c_int, [P, PD, PI64]),
         "cfft_sweep_dev": (C.c_int, [P, P, C.POINTER(P), C.POINTER(P), PI32, I32, I32, P]),
         "cfft_plan_set_option": (C.c_int, [P, I32, I64]),
+        "cfft_calc_asu": (C.c_int, [I32, PI32, PI32, I32, PI32, PI32, PI32, I64, PI64]),
+        "cfft_pack_asu": (C.c_int, [I32, PI32, PI32, I64, PI32, I64, PI64]),
+        "cfft_block_plan_create": (C.c_int, [I32, PI32, I64, C.POINTER(P)]),
+        "cfft_block_plan_info": (C.c_int, [P, PI64, PI64, PI32]),
+        "cfft_block_forward": (C.c_int, [P, PD, PD]),
+        "cfft_block_inverse": (C.c_int, [P, PD, PD]),
+        "cfft_block_plan_destroy": (C.c_int, [P]),
+        "cfft_prefold_plan_create": (C.c_int, [PI32, PI32, I32, C.POINTER(P)]),
+        "cfft_prefold_plan_info": (C.c_int, [P, PI32, PI32]),
+        "cfft_prefold_forward": (C.c_int, [P, PD, PD]),
+        "cfft_prefold_plan_destroy": (C.c_int, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

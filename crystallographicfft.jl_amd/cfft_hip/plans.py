@@ -16,7 +16,8 @@ Reference counterparts:
   plan_m2_scft / execute_m2_scft / update_m2_kernel
                               src/q_fused_krfft.jl:948-1052
   plan_krfft_centered / execute_centered_krfft / fft_reconstruct_centered
-                              src/centering_fold.jl:328-431
+                              src/centering_fold.jl:328-431,
+                              src/centering_prefold.jl:162-257 (pre-fold)
   plan_centered_ikrfft / execute_centered_ikrfft / ifft_unrecon_centered
                               src/centering_fold.jl:630-775
   plan_centered_scft / execute_centered_scft / update_kernel
@@ -490,9 +491,61 @@ class CenteredKRFFTPlan:
         self.subgrid_dims = self.dev.M
 
 
+class CenteringPreFoldKRFFTPlan:
+    """src/centering_prefold.jl:162-205: full-grid centring pre-fold into
+    4 (I) / 2 (F, C, A) channels on the halved grid, one FFT per channel and
+    a merge onto the spectral ASU of ``ops``.  (The reference names this
+    struct CenteredKRFFTPlan as well, in a second file; here it has a name of
+    its own so that both centred forward plans can coexist.)"""
+
+    def __init__(self, ops, N):
+        N3, dim = _N3(N)
+        if dim != 3:
+            raise L.CFFTError(L.CFFT_ERR_INVALID, "the centring pre-fold is 3-D")
+        packed = _pack_any(ops, 3)
+        h = C.c_void_p()
+        L.check(L.lib().cfft_prefold_plan_create(L.i32p(np.array(N3, dtype=np.int32)),
+                                                 L.i32p(packed), len(ops), C.byref(h)))
+        self._h = h
+        info = np.zeros(6, dtype=np.int32)
+        L.check(L.lib().cfft_prefold_plan_info(h, L.i32p(info), None))
+        n = int(info[0])
+        idx = np.zeros((max(n, 1), 3), dtype=np.int32)
+        L.check(L.lib().cfft_prefold_plan_info(h, L.i32p(info), L.i32p(idx)))
+        self.N_full = N3
+        self.N_sub = tuple(int(v) for v in info[3:6])
+        self.n_channels = int(info[1])
+        self.centering = CenteringType(int(info[2]))
+        self.full_spec_asu = calc_spectral_asu(ops, 3, N3)
+        if not np.array_equal(self.full_spec_asu.idx, idx[:n].astype(self.full_spec_asu.idx.dtype)):
+            raise L.CFFTError(L.CFFT_ERR_INVALID, "spectral ASU mismatch")
+        self.output_buffer = np.zeros(n, dtype=np.complex128)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            L.lib().cfft_prefold_plan_destroy(h)
+            self._h = None
+
+    def execute(self, u):
+        u = np.asfortranarray(np.asarray(u, dtype=np.float64))
+        if u.shape != self.N_full:
+            raise L.CFFTError(L.CFFT_ERR_INVALID, f"grid shape {u.shape} != {self.N_full}")
+        out = self.output_buffer.view(np.float64)
+        L.check(L.lib().cfft_prefold_forward(self._h, L.f64p(u), L.f64p(out)))
+        return self.output_buffer
+
+
 def plan_krfft_centered(spec, ops_shifted, centering="auto"):
     """src/centering_fold.jl:328-367: falls back to plan_krfft when the lattice
-    is primitive, L != (2,2,2) or M is odd."""
+    is primitive, L != (2,2,2) or M is odd.
+
+    ``plan_krfft_centered(ops, N)`` and ``plan_krfft_centered(sg_num, N)``
+    (src/centering_prefold.jl:175-184) build the full-grid pre-fold plan."""
+    if not isinstance(spec, SpectralIndexing):
+        N = tuple(int(n) for n in ops_shifted)
+        ops = get_ops(int(spec), 3, N) if isinstance(spec, (int, np.integer)) else list(spec)
+        return CenteringPreFoldKRFFTPlan(ops, N)
     N = tuple(spec.N)
     cent = detect_centering(ops_shifted, N) if centering == "auto" else CenteringType(centering)
     Lv = auto_L(ops_shifted)
@@ -515,7 +568,10 @@ def fft_reconstruct_centered(plan: CenteredKRFFTPlan):
 
 
 def execute_centered_krfft(plan, u):
-    """src/centering_fold.jl:392-413."""
+    """src/centering_fold.jl:392-413 (and src/centering_prefold.jl:236-244
+    for a pre-fold plan)."""
+    if isinstance(plan, CenteringPreFoldKRFFTPlan):
+        return plan.execute(u)
     if isinstance(plan, GeneralCFFTPlan):
         return execute_krfft(plan, u)
     pack_stride_real(plan.f0_buffer, u)

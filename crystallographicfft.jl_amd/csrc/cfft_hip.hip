@@ -17,77 +17,11 @@
 #include "kernels.hpp"
 #include "pipeline.hpp"
 #include "plan.hpp"
+#include "common.hpp"
 
 using namespace cfft;
 
-namespace {
-
-thread_local std::string g_last_error;
-
-void hip_check(hipError_t e, const char* what) {
-  if (e != hipSuccess)
-    throw Error{CFFT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)};
-}
-
-template <typename F>
-int guarded(F&& f) {
-  try {
-    f();
-    return CFFT_OK;
-  } catch (const Error& e) {
-    g_last_error = e.msg;
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_last_error = "host allocation failed";
-    return CFFT_ERR_HIP;
-  } catch (const std::exception& e) {
-    g_last_error = e.what();
-    return CFFT_ERR_INVALID;
-  }
-}
-
-void require(bool ok, int code, const std::string& msg) {
-  if (!ok) throw Error{code, msg};
-}
-
-// exp(2 pi i j / P), argument reduced in extended precision.
-void cis2pi(long long j, long long P, double& re, double& im) {
-  long long r = ((j % P) + P) % P;
-  long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)r / (long double)P;
-  re = (double)cosl(a);
-  im = (double)sinl(a);
-  // exact values on the axes
-  if (4 * r == 0) { re = 1.0; im = 0.0; }
-  else if (4 * r == P) { re = 0.0; im = 1.0; }
-  else if (4 * r == 2 * P) { re = -1.0; im = 0.0; }
-  else if (4 * r == 3 * P) { re = 0.0; im = -1.0; }
-}
-
-constexpr int kFftLens[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64};
-bool fft_supported(int n) {
-  for (int v : kFftLens)
-    if (v == n) return true;
-  return false;
-}
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  void alloc(size_t n) {
-    release();
-    if (n == 0) return;
-    hip_check(hipMalloc(&p, n), "hipMalloc");
-    bytes = n;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  ~DevBuf() { release(); }
-};
-
-}  // namespace
+thread_local std::string cfft::g_last_error;
 
 struct cfft_plan {
   int N[3], M[3], L[3], H[3];
@@ -353,7 +287,6 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
   p->d_E.alloc(es * p->M3 * batch);
 }
 
-inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // Brackets one kernel launch with hipEvents when the plan's stage timer is on.
 struct StageTimer {

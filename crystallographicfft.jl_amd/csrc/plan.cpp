@@ -510,6 +510,183 @@ std::vector<int32_t> fill_map(int dim, const int* N, const std::vector<Op>& ops,
   return out;
 }
 
+// calc_asu, src/asu.jl:216-325.  The work list is processed last-in first-out
+// as the reference's pop!; the final sort makes the result independent of it.
+std::vector<ASUPoint> calc_asu(int dim, const int* N0, const std::vector<Op>& ops0) {
+  struct State {
+    std::array<int, 3> N, scale, offset, depth;
+    std::array<bool, 3> gp;
+    std::vector<Op> ops;
+  };
+  std::vector<ASUPoint> pts;
+  std::vector<State> queue;
+  State s0;
+  for (int d = 0; d < 3; ++d) {
+    s0.N[d] = d < dim ? N0[d] : 1;
+    s0.scale[d] = 1;
+    s0.offset[d] = 0;
+    s0.depth[d] = 0;
+    s0.gp[d] = d >= dim;  // padding dimensions behave as general positions
+  }
+  s0.ops = ops0;
+  queue.push_back(s0);
+  auto apply = [&](const Op& op, const int* x, const std::array<int, 3>& n, int* out) {
+    for (int i = 0; i < dim; ++i) {
+      long long s = op.t[i];
+      for (int j = 0; j < dim; ++j) s += 1LL * op.R[i][j] * x[j];
+      out[i] = pmod(s, n[i]);
+    }
+  };
+  while (!queue.empty()) {
+    State st = std::move(queue.back());
+    queue.pop_back();
+    bool zero = false, all_gp = true, all_le1 = true;
+    for (int d = 0; d < dim; ++d) {
+      if (st.N[d] == 0) zero = true;
+      if (!st.gp[d]) all_gp = false;
+      if (st.N[d] > 1) all_le1 = false;
+    }
+    if (zero) continue;
+    if (all_gp || all_le1) {
+      // leaf: one representative (lexicographic minimum) per local orbit
+      const long long n_loc = 1LL * st.N[0] * st.N[1] * st.N[2];
+      std::vector<uint8_t> visited(n_loc, 0);
+      auto lin = [&](const int* x) { return x[0] + 1LL * st.N[0] * (x[1] + 1LL * st.N[1] * x[2]); };
+      std::vector<std::array<int, 3>> orbit, stack;
+      for (long long li = 0; li < n_loc; ++li) {
+        if (visited[li]) continue;
+        std::array<int, 3> p = {int(li % st.N[0]), int((li / st.N[0]) % st.N[1]),
+                                int(li / (1LL * st.N[0] * st.N[1]))};
+        orbit.assign(1, p);
+        stack.assign(1, p);
+        visited[li] = 1;
+        while (!stack.empty()) {
+          std::array<int, 3> c = stack.back();
+          stack.pop_back();
+          for (const Op& op : st.ops) {
+            std::array<int, 3> y = {0, 0, 0};
+            apply(op, c.data(), st.N, y.data());
+            long long ly = lin(y.data());
+            if (!visited[ly]) {
+              visited[ly] = 1;
+              orbit.push_back(y);
+              stack.push_back(y);
+            }
+          }
+        }
+        std::array<int, 3> rep = *std::min_element(orbit.begin(), orbit.end());
+        ASUPoint ap;
+        for (int d = 0; d < 3; ++d) {
+          ap.idx[d] = d < dim ? st.scale[d] * rep[d] + st.offset[d] : 0;
+          ap.depth[d] = d < dim ? st.depth[d] : 0;
+        }
+        ap.mult = (int)orbit.size();
+        pts.push_back(ap);
+      }
+      continue;
+    }
+    std::array<bool, 3> eff = st.gp;
+    for (int d = 0; d < dim; ++d) {
+      if (st.gp[d]) continue;
+      int pe[3] = {0, 0, 0}, po[3] = {0, 0, 0}, out[3] = {0, 0, 0};
+      po[d] = 1;
+      bool ok = true;
+      for (const Op& op : st.ops) {
+        apply(op, pe, st.N, out);
+        if (out[d] % 2 != 0) { ok = false; break; }
+        apply(op, po, st.N, out);
+        if (out[d] % 2 != 1) { ok = false; break; }
+      }
+      if (!ok) eff[d] = true;
+    }
+    int rng[3];
+    for (int d = 0; d < 3; ++d) rng[d] = (d < dim && !eff[d] && st.N[d] > 1) ? 2 : 1;
+    std::vector<State> children;
+    for (int p2 = 0; p2 < rng[2]; ++p2)
+      for (int p1 = 0; p1 < rng[1]; ++p1)
+        for (int p0 = 0; p0 < rng[0]; ++p0) {
+          const int par[3] = {p0, p1, p2};
+          int S[3];
+          for (int d = 0; d < 3; ++d) S[d] = (d >= dim || eff[d]) ? 1 : 2;
+          State ch;
+          bool valid = true;
+          for (const Op& op : st.ops) {
+            Op no = op;
+            for (int i = 0; i < dim && valid; ++i) {
+              long long tmp = op.t[i] - par[i];
+              for (int j = 0; j < dim; ++j) tmp += 1LL * op.R[i][j] * par[j];
+              for (int j = 0; j < dim; ++j) {
+                long long rv = 1LL * op.R[i][j] * S[j];
+                if (rv % S[i] != 0) valid = false;
+                no.R[i][j] = (int)(rv / S[i]);
+              }
+              if (tmp % S[i] != 0) valid = false;
+              no.t[i] = (int)(tmp / S[i]);
+            }
+            if (!valid) break;
+            ch.ops.push_back(no);
+          }
+          if (!valid) continue;
+          for (int d = 0; d < 3; ++d) {
+            const bool e = d >= dim || eff[d];
+            ch.N[d] = e ? st.N[d] : st.N[d] / 2;
+            ch.gp[d] = e || par[d] == 1;
+            ch.depth[d] = e ? st.depth[d] : st.depth[d] + (par[d] == 0 ? 1 : 0);
+            ch.scale[d] = st.scale[d] * S[d];
+            ch.offset[d] = st.scale[d] * par[d] + st.offset[d];
+          }
+          children.push_back(std::move(ch));
+        }
+    for (auto& ch : children) queue.push_back(std::move(ch));
+  }
+  std::sort(pts.begin(), pts.end(), [](const ASUPoint& a, const ASUPoint& b) { return a.idx < b.idx; });
+  return pts;
+}
+
+// pack_asu, src/pack_asu.jl:6-102.
+std::vector<Block> pack_asu(int dim, const std::vector<ASUPoint>& pts) {
+  std::map<std::array<int, 3>, std::vector<std::array<int, 3>>> groups;
+  for (const auto& p : pts) groups[p.depth].push_back(p.idx);
+  std::vector<Block> blocks;
+  for (auto& kv : groups) {
+    auto& pool = kv.second;
+    std::sort(pool.begin(), pool.end());
+    std::set<std::array<int, 3>> in_pool(pool.begin(), pool.end());
+    size_t head = 0;
+    while (true) {
+      while (head < pool.size() && !in_pool.count(pool[head])) ++head;
+      if (head >= pool.size()) break;
+      const std::array<int, 3> start = pool[head];
+      std::array<int, 3> lo = start, hi = start;
+      for (int d = 0; d < dim; ++d) {
+        int steps = 0;
+        while (true) {
+          const int nxt = start[d] + steps + 1;
+          std::array<int, 3> a = lo, b = hi;
+          a[d] = b[d] = nxt;
+          bool full = true;
+          for (int z = a[2]; z <= b[2] && full; ++z)
+            for (int y = a[1]; y <= b[1] && full; ++y)
+              for (int x = a[0]; x <= b[0] && full; ++x)
+                if (!in_pool.count({x, y, z})) full = false;
+          if (!full) break;
+          ++steps;
+        }
+        hi[d] = start[d] + steps;
+      }
+      for (int z = lo[2]; z <= hi[2]; ++z)
+        for (int y = lo[1]; y <= hi[1]; ++y)
+          for (int x = lo[0]; x <= hi[0]; ++x) in_pool.erase({x, y, z});
+      Block b;
+      b.start = lo;
+      for (int d = 0; d < 3; ++d) b.shape[d] = hi[d] - lo[d] + 1;
+      b.depth = kv.first;
+      blocks.push_back(b);
+    }
+  }
+  return blocks;
+}
+
 // K_spec of plan_m2_scft / plan_centered_scft (src/q_fused_krfft.jl:966-973).
 std::vector<double> kernel_spec(const SpecASU& spec, double ds, const double* lattice) {
   // A[i][j] = lattice[i + 3 j] (columns are lattice vectors); B = 2 pi inv(A)^T

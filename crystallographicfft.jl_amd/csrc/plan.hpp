@@ -70,6 +70,23 @@ ReverseLookup reverse_lookup(const SpecASU& spec, const std::vector<Op>& ops);
 
 std::vector<int32_t> fill_map(int dim, const int* N, const std::vector<Op>& ops, const int* L);
 
+// Real-space ASU point (src/asu.jl:23-27).
+struct ASUPoint {
+  std::array<int, 3> idx;
+  std::array<int, 3> depth;
+  int mult;
+};
+// calc_asu, src/asu.jl:216-325 (recursive parity filter), sorted by idx.
+std::vector<ASUPoint> calc_asu(int dim, const int* N, const std::vector<Op>& ops);
+
+// A packed block (src/pack_asu.jl): start, shape (unit stride), depth.
+struct Block {
+  std::array<int, 3> start, shape, depth;
+};
+// pack_asu, src/pack_asu.jl:6-102: greedy boxes per depth group (groups in
+// sorted depth order).
+std::vector<Block> pack_asu(int dim, const std::vector<ASUPoint>& pts);
+
 // K(h) = exp(-|2 pi A^-T h_c|^2 ds), lattice column-major (columns a, b, c).
 std::vector<double> kernel_spec(const SpecASU& spec, double ds, const double* lattice);
 

@@ -210,6 +210,57 @@ int cfft_plan_set_option(cfft_plan* plan, int32_t key, int64_t value);
 /* Synchronise the plan's stream. */
 int cfft_sync(cfft_plan* plan);
 
+/* ---- real-space ASU and block CFFT ---------------------------------------- */
+/* calc_asu(N, ops), src/asu.jl:216-325 (recursive parity filter; ops already
+ * magic-shifted).  Writes at most cap points sorted by idx: idx_out and
+ * depth_out cap x 3 (unused dimensions 0), mult_out cap; *n_out = total. */
+int cfft_calc_asu(int32_t dim, const int32_t* N, const int32_t* ops, int32_t nops,
+                  int32_t* idx_out, int32_t* depth_out, int32_t* mult_out, int64_t cap,
+                  int64_t* n_out);
+
+/* pack_asu, src/pack_asu.jl:6-102: greedy unit-stride boxes per depth group
+ * (groups in sorted depth order) over n_points ASU points (idx and depth:
+ * n_points x 3).  Writes at most cap blocks as 9 ints (start[3], shape[3],
+ * depth[3]); *n_out = block count. */
+int cfft_pack_asu(int32_t dim, const int32_t* idx, const int32_t* depth, int64_t n_points,
+                  int32_t* blocks_out, int64_t cap, int64_t* n_out);
+
+typedef struct cfft_bplan cfft_bplan;
+
+/* plan_cfft, src/cfft_plan.jl:15-40: one DFT per packed block (blocks: 9 ints
+ * each, as cfft_pack_asu writes them); every block's data stored one after
+ * another in this order, column-major inside a block. */
+int cfft_block_plan_create(int32_t dim, const int32_t* blocks, int64_t n_blocks,
+                           cfft_bplan** out);
+/* Block count, total element count and the block list (9 ints per block,
+ * as cfft_pack_asu; blocks_out may be NULL). */
+int cfft_block_plan_info(const cfft_bplan* plan, int64_t* n_blocks, int64_t* total,
+                         int32_t* blocks_out);
+/* mul!(y, p, x), src/cfft_plan.jl:43-53 / src/execution.jl:10-31: unnormalised
+ * forward DFT of every block; in/out interleaved complex, total entries. */
+int cfft_block_forward(cfft_bplan* plan, const double* in, double* out);
+/* ldiv!(y, p, x), src/cfft_plan.jl:56-66 / src/execution.jl:37-64: inverse
+ * DFT of every block with the 1/size normalisation. */
+int cfft_block_inverse(cfft_bplan* plan, const double* in, double* out);
+int cfft_block_plan_destroy(cfft_bplan* plan);
+
+/* ---- full-grid centring pre-fold KRFFT ------------------------------------ */
+typedef struct cfft_pplan cfft_pplan;
+
+/* plan_krfft_centered(ops, N), src/centering_prefold.jl:175-205: I/F/C/A
+ * pre-fold of the full N-grid into 4/2/2/2 channels on the halved grid, one
+ * FFT per channel and the merge table onto the spectral ASU of ops
+ * (src/centering_prefold.jl:207-230).  Errors: CFFT_ERR_CENTERING for CentP. */
+int cfft_prefold_plan_create(const int32_t* N, const int32_t* ops, int32_t nops,
+                             cfft_pplan** out);
+/* info[6] = {n_spec, n_channels, centering, Nsub0, Nsub1, Nsub2};
+ * idx_out (n_spec x 3, may be NULL) = the spectral-ASU points. */
+int cfft_prefold_plan_info(const cfft_pplan* plan, int32_t* info, int32_t* idx_out);
+/* execute_centered_krfft!, src/centering_prefold.jl:236-257: u real full grid
+ * (N0*N1*N2) -> F_spec (n_spec interleaved complex; unmapped points 0). */
+int cfft_prefold_forward(cfft_pplan* plan, const double* u, double* F_spec);
+int cfft_prefold_plan_destroy(cfft_pplan* plan);
+
 #ifdef __cplusplus
 }
 #endif

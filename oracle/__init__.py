@@ -12,3 +12,4 @@ listed in the module docstrings and exercised by ``tests/test_oracle_*.py``.
 """
 from .krfft import *  # noqa: F401,F403
 from .scft import *   # noqa: F401,F403
+from .asu import calc_asu, pack_asu, block_fft, block_ifft  # noqa: F401
