@@ -43,8 +43,12 @@ EXPORTED = [
     "cfft_calc_asu", "cfft_pack_asu", "cfft_block_plan_create", "cfft_block_plan_info",
     "cfft_block_forward", "cfft_block_inverse", "cfft_block_plan_destroy",
     "cfft_prefold_plan_create", "cfft_prefold_plan_info", "cfft_prefold_forward",
-    "cfft_prefold_plan_destroy",
+    "cfft_prefold_plan_destroy", "cfft_sweep_hist_dev", "cfft_scft_create", "cfft_scft_set_mixing",
+    "cfft_scft_set_fields", "cfft_scft_iterate", "cfft_scft_solve", "cfft_scft_get",
+    "cfft_scft_propagators", "cfft_scft_destroy",
 ]
+
+SCFT_W, SCFT_PHI, SCFT_DEV = 0, 1, 2
 
 OPT_PERSISTENT = 1
 
@@ -119,6 +123,15 @@ def _declare(lib):
         "cfft_prefold_plan_info": (C.c_int, [P, PI32, PI32]),
         "cfft_prefold_forward": (C.c_int, [P, PD, PD]),
         "cfft_prefold_plan_destroy": (C.c_int, [P]),
+        "cfft_sweep_hist_dev": (C.c_int, [P, P, C.POINTER(P), C.POINTER(P), PI32, I32, I32, P]),
+        "cfft_scft_create": (C.c_int, [P, D, D, I32, I32, C.POINTER(P)]),
+        "cfft_scft_set_mixing": (C.c_int, [P, D, D, D]),
+        "cfft_scft_set_fields": (C.c_int, [P, PD, PD]),
+        "cfft_scft_iterate": (C.c_int, [P, PD]),
+        "cfft_scft_solve": (C.c_int, [P, D, I32, PI32, PD]),
+        "cfft_scft_get": (C.c_int, [P, I32, PD]),
+        "cfft_scft_propagators": (C.c_int, [P, C.POINTER(P)]),
+        "cfft_scft_destroy": (C.c_int, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -11,6 +11,8 @@ and w_B elsewhere.  Q = <q(1)> = <q+(1)> (grid mean; on the subgrid because the
 full grid is the union of prod(L) images of it).
 
 Device memory comes from torch (plumbing only); every kernel is the library's.
+``SCFTSolver`` keeps the whole iteration (sweep, densities, free energy, field
+update, Anderson mixing) inside the library's ``cfft_scft`` context.
 """
 from __future__ import annotations
 
@@ -112,3 +114,79 @@ class PropagatorSweep:
     def fields(self):
         out = self.q.to(self.torch.float64).cpu().numpy()
         return [out[b].reshape(self.M, order="F") for b in range(2)]
+
+
+class SCFTSolver:
+    """Device SCFT of an AB diblock melt on the stride-L subgrid (C context
+    ``cfft_scft``): sweep (q, q+ batched, every contour point kept), Simpson
+    densities, free energy, field update and Anderson mixing, FP64 except the
+    propagator arithmetic in mixed precision.
+
+    ``ops_shifted`` are the magic-shifted ops; fields are subgrid arrays of
+    shape M (x fastest).  ``iterate()`` returns a dict with Q, Q+ (Qd), F, the
+    residual, <phiA + phiB>, <phiA>, and whether Anderson mixing was used.
+    """
+
+    KEYS = ("Q", "Qd", "F", "res", "phi_total", "phiA_mean", "anderson", "hist")
+
+    def __init__(self, N, ops_shifted, lattice, f, chiN, ns, precision=L.FP64, path=L.PATH_AUTO,
+                 m_hist=10):
+        self.ns = int(ns)
+        self.f = float(f)
+        self.chiN = float(chiN)
+        self.plan = DevicePlan(N, ops_shifted, 1.0 / self.ns, lattice, path, precision, batch=2)
+        self.M = self.plan.M
+        self.M3 = int(np.prod(self.M))
+        h = C.c_void_p()
+        L.check(L.lib().cfft_scft_create(self.plan.handle, self.f, self.chiN, self.ns, int(m_hist),
+                                         C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                L.lib().cfft_scft_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def set_mixing(self, lam=1.0, lam_simple=0.1, simple_above=1e-2):
+        L.check(L.lib().cfft_scft_set_mixing(self._h, float(lam), float(lam_simple), float(simple_above)))
+
+    def set_fields(self, wA, wB):
+        a = np.ascontiguousarray(np.asarray(wA, dtype=np.float64).ravel(order="F"))
+        b = np.ascontiguousarray(np.asarray(wB, dtype=np.float64).ravel(order="F"))
+        if a.size != self.M3 or b.size != self.M3:
+            raise L.CFFTError(L.CFFT_ERR_INVALID, "fields must have the subgrid size")
+        L.check(L.lib().cfft_scft_set_fields(self._h, L.f64p(a), L.f64p(b)))
+
+    def _out(self, o):
+        return {k: float(v) for k, v in zip(self.KEYS, o)}
+
+    def iterate(self):
+        o = np.zeros(8)
+        L.check(L.lib().cfft_scft_iterate(self._h, L.f64p(o)))
+        return self._out(o)
+
+    def solve(self, tol=1e-8, max_iter=1000):
+        o = np.zeros(8)
+        it = C.c_int32()
+        L.check(L.lib().cfft_scft_solve(self._h, float(tol), int(max_iter), C.byref(it), L.f64p(o)))
+        return int(it.value), self._out(o)
+
+    def _get(self, which):
+        out = np.zeros(2 * self.M3)
+        L.check(L.lib().cfft_scft_get(self._h, which, L.f64p(out)))
+        return [out[i * self.M3:(i + 1) * self.M3].reshape(self.M, order="F") for i in range(2)]
+
+    def fields(self):
+        """[wA, wB] (the fields the next iteration will use)."""
+        return self._get(L.SCFT_W)
+
+    def densities(self):
+        """[phiA, phiB] of the last iteration."""
+        return self._get(L.SCFT_PHI)
+
+    def deviations(self):
+        return self._get(L.SCFT_DEV)

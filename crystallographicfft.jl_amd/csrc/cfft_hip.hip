@@ -535,12 +535,12 @@ void run_backward(cfft_plan* p, const PipeP& pp, const cx<T>* F, T* f0, int step
   launch_p5<T>(p, pp, B, step, s);
 }
 
-// One diffusion operator application per step on f0 (device), with the
-// interaction factors of `pp` on both sides: f0 <- E D (E f0).
+// One diffusion operator application per step, with the interaction factors
+// of `pp` on both sides: out <- E D (E in).  `in` may equal `out`.
 template <typename T>
-void run_diffuse(cfft_plan* p, T* f0, const PipeP& pp, int step, hipStream_t s) {
+void run_diffuse(cfft_plan* p, const T* in, T* out_f0, const PipeP& pp, int step, hipStream_t s) {
   if (is_q_path(p)) {
-    run_forward_channels<T>(p, f0, pp, step, s);
+    run_forward_channels<T>(p, in, pp, step, s);
     cx<T>* out = static_cast<cx<T>*>(p->d_ch2.p);
     {
       StageTimer st(p, s, CFFT_STAGE_QMUL);
@@ -549,7 +549,7 @@ void run_diffuse(cfft_plan* p, T* f0, const PipeP& pp, int step, hipStream_t s) 
                          static_cast<const int*>(p->d_gidx.p), p->d, p->n_alive, p->batch);
       launch_check<T>("k_qmul");
     }
-    PipeBufs B = pipe_bufs(p, nullptr, f0, nullptr, false);
+    PipeBufs B = pipe_bufs(p, nullptr, out_f0, nullptr, false);
     if (cubic(p)) {
       StageTimer st(p, s, CFFT_STAGE_IFFT_Z);
       const unsigned g = (unsigned)n_planes(p);
@@ -567,9 +567,9 @@ void run_diffuse(cfft_plan* p, T* f0, const PipeP& pp, int step, hipStream_t s) 
     return;
   }
   cx<T>* F = static_cast<cx<T>*>(p->d_F.p);
-  run_forward_channels<T>(p, f0, pp, step, s);
+  run_forward_channels<T>(p, in, pp, step, s);
   run_recon<T>(p, pp, F, true, s);
-  run_backward<T>(p, pp, F, f0, step, s);
+  run_backward<T>(p, pp, F, out_f0, step, s);
 }
 
 // Persistent cooperative sweep of nsteps Strang steps (cubic H-grids, fwd+K+bwd
@@ -662,7 +662,7 @@ template <typename T>
 void diffuse_host(cfft_plan* p, double* f0) {
   to_dev<T>(p, p->d_f0, f0, p->M3 * p->batch);
   PipeP pp = pipe_params(p, ESel{});
-  run_diffuse<T>(p, static_cast<T*>(p->d_f0.p), pp, 0, p->stream);
+  run_diffuse<T>(p, static_cast<T*>(p->d_f0.p), static_cast<T*>(p->d_f0.p), pp, 0, p->stream);
   from_dev<T>(p, p->d_f0, f0, p->M3 * p->batch);
 }
 
@@ -677,7 +677,16 @@ template <typename T>
 void run_steps(cfft_plan* p, T* q, const ESel& es, int step0, int nsteps, hipStream_t s) {
   PipeP pp = pipe_params(p, es);
   if (nsteps > 0 && run_sweep_persistent<T>(p, q, pp, step0, nsteps, s)) return;
-  for (int i = 0; i < nsteps; ++i) run_diffuse<T>(p, q, pp, step0 + i, s);
+  for (int i = 0; i < nsteps; ++i) run_diffuse<T>(p, q, q, pp, step0 + i, s);
+}
+
+// nsteps Strang steps keeping every contour point: step i reads slice i of
+// `hist` and writes slice i + 1 (slices of batch * M3 entries).
+template <typename T>
+void run_steps_hist(cfft_plan* p, T* hist, const ESel& es, int step0, int nsteps, hipStream_t s) {
+  PipeP pp = pipe_params(p, es);
+  const long slice = p->M3 * p->batch;
+  for (int i = 0; i < nsteps; ++i) run_diffuse<T>(p, hist + i * slice, hist + (i + 1) * slice, pp, step0 + i, s);
 }
 
 template <typename T>
@@ -990,7 +999,7 @@ int cfft_diffuse_dev(cfft_plan* p, void* f0, void* stream) {
     auto run = [&](auto tag) {
       using T = decltype(tag);
       PipeP pp = pipe_params(p, ESel{});
-      run_diffuse<T>(p, static_cast<T*>(f0), pp, 0, s);
+      run_diffuse<T>(p, static_cast<T*>(f0), static_cast<T*>(f0), pp, 0, s);
     };
     if (p->prec == CFFT_FP64) run(double{}); else run(float{});
   });
@@ -1042,6 +1051,24 @@ int cfft_sweep_dev(cfft_plan* p, void* q, const void* const* E1, const void* con
       run_steps<double>(p, static_cast<double*>(q), es, step0, nsteps, s);
     else
       run_steps<float>(p, static_cast<float*>(q), es, step0, nsteps, s);
+  });
+}
+
+int cfft_sweep_hist_dev(cfft_plan* p, void* hist, const void* const* E1, const void* const* E2,
+                        const int32_t* sw, int32_t step0, int32_t nsteps, void* stream) {
+  return guarded([&] {
+    check_plan(p);
+    require(hist && E1 && E2 && sw && nsteps >= 0, CFFT_ERR_INVALID, "bad argument");
+    hipStream_t s = pick(p, stream);
+    ESel es;
+    es.mode = 2;
+    es.E1 = E1;
+    es.E2 = E2;
+    es.sw = sw;
+    if (p->prec == CFFT_FP64)
+      run_steps_hist<double>(p, static_cast<double*>(hist), es, step0, nsteps, s);
+    else
+      run_steps_hist<float>(p, static_cast<float*>(hist), es, step0, nsteps, s);
   });
 }
 

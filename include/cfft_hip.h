@@ -199,6 +199,13 @@ int cfft_profile(cfft_plan* plan, int32_t enable);
  * counts per stage since the last read; clears the record. */
 int cfft_profile_read(cfft_plan* plan, double* ms_out, int64_t* count_out);
 
+/* Propagator sweep keeping every contour point (SCFT densities; no
+ * counterpart in the reference).  hist holds nsteps + 1 slices of batch
+ * fields (batch * M0*M1*M2 entries each); step i reads slice i and writes
+ * slice i + 1 with the factors of cfft_sweep_dev. */
+int cfft_sweep_hist_dev(cfft_plan* plan, void* hist, const void* const* E1, const void* const* E2,
+                        const int32_t* sw, int32_t step0, int32_t nsteps, void* stream);
+
 /* ---- plan options -------------------------------------------------------- */
 /* value != 0: run cfft_sweep_dev / cfft_step_dev on cubic H-grids as one
  * cooperative launch with grid barriers between phases (default 0: one
@@ -260,6 +267,39 @@ int cfft_prefold_plan_info(const cfft_pplan* plan, int32_t* info, int32_t* idx_o
  * (N0*N1*N2) -> F_spec (n_spec interleaved complex; unmapped points 0). */
 int cfft_prefold_forward(cfft_pplan* plan, const double* u, double* F_spec);
 int cfft_prefold_plan_destroy(cfft_pplan* plan);
+
+/* ---- SCFT driver (AB diblock melt on the subgrid) -------------------------- */
+/* No counterpart in the reference, which stops at the diffusion step
+ * (src/diffusion_solver.jl:254-282).  A context borrows a plan created with
+ * batch = 2 (q and q+ advance together, ds = 1/ns) and owns the propagator
+ * slices (ns + 1 of them), the FP64 fields w = [wA, wB], densities
+ * phi = [phiA, phiB], deviations d = t - w and the Anderson history (m_hist
+ * previous iterates).  One plan per context. */
+typedef struct cfft_scft cfft_scft;
+
+#define CFFT_SCFT_W 0   /* [wA, wB]   (2 x M0*M1*M2 doubles) */
+#define CFFT_SCFT_PHI 1 /* [phiA, phiB] of the last iteration  */
+#define CFFT_SCFT_DEV 2 /* [tA - wA, tB - wB] of the last iteration */
+
+int cfft_scft_create(cfft_plan* plan, double f, double chiN, int32_t ns, int32_t m_hist,
+                     cfft_scft** out);
+/* Anderson step length lam, simple-mixing step lam_simple, and the residual
+ * above which simple mixing is used (defaults 1.0, 0.1, 1e-2). */
+int cfft_scft_set_mixing(cfft_scft* ctx, double lam, double lam_simple, double simple_above);
+/* Fields on the subgrid (M0*M1*M2 doubles each, x fastest); clears the
+ * mixing history. */
+int cfft_scft_set_fields(cfft_scft* ctx, const double* wA, const double* wB);
+/* One iteration: sweep, densities (Simpson), F, field update and mixing.
+ * out[8] = {Q, Q+, F, residual, <phiA + phiB>, <phiA>, 1 if Anderson else 0,
+ * history entries used}. */
+int cfft_scft_iterate(cfft_scft* ctx, double* out);
+/* Iterate until residual < tol or max_iter; out[8] as cfft_scft_iterate. */
+int cfft_scft_solve(cfft_scft* ctx, double tol, int32_t max_iter, int32_t* iters_out, double* out);
+/* Copy a quantity (CFFT_SCFT_*) to the host: 2 x M0*M1*M2 doubles. */
+int cfft_scft_get(cfft_scft* ctx, int32_t which, double* out);
+/* Device pointer to the propagator slices [ns + 1][q, q+][M0*M1*M2] (plan precision). */
+int cfft_scft_propagators(cfft_scft* ctx, void** hist_out);
+int cfft_scft_destroy(cfft_scft* ctx);
 
 #ifdef __cplusplus
 }

@@ -27,12 +27,14 @@ from __future__ import annotations
 import numpy as np
 
 __all__ = ["contour_weights", "propagate", "sweep", "field_update", "free_energy",
-           "disordered_free_energy", "homogeneous_Q"]
+           "disordered_free_energy", "homogeneous_Q", "AndersonMixer"]
 
 
 def contour_weights(n):
     """Simpson weights on n+1 points (trapezoid on an odd remainder)."""
     w = np.zeros(n + 1)
+    if n == 0:
+        return w
     if n % 2 == 0:
         w[0] = w[n] = 1.0
         w[1:n:2] = 4.0
@@ -106,3 +108,39 @@ def disordered_free_energy(f, chiN):
 
 def homogeneous_Q(wA, wB, f):
     return float(np.exp(-(f * wA + (1.0 - f) * wB)))
+
+
+class AndersonMixer:
+    """Anderson mixing of the field iteration (Matsen, Eur. Phys. J. E 30,
+    361 (2009), eqs. 31-35), FP64.
+
+    History of inputs w_k and deviations d_k = w_k^out - w_k^in (both fields
+    stacked).  With U_ij = <(d_n - d_{n-i}).(d_n - d_{n-j})>, V_i =
+    <(d_n - d_{n-i}).d_n>, U c = V (least squares):
+      w' = w_n + sum_i c_i (w_{n-i} - w_n),  d' = d_n + sum_i c_i (d_{n-i} - d_n),
+      w_{n+1} = w' + lam d'.
+    Before `start_after` history entries exist (or while the residual is
+    above `simple_above`) it falls back to simple mixing w + lam_s d.
+    """
+
+    def __init__(self, m=10, lam=1.0, lam_simple=0.1, simple_above=1e-2):
+        self.m, self.lam, self.lam_simple, self.simple_above = m, lam, lam_simple, simple_above
+        self.W, self.D = [], []
+
+    def step(self, w, d, res):
+        self.W.append(w.copy())
+        self.D.append(d.copy())
+        if len(self.W) > self.m + 1:
+            self.W.pop(0)
+            self.D.pop(0)
+        k = len(self.W) - 1
+        if res > self.simple_above or k == 0:
+            return w + self.lam_simple * d
+        dn, wn = self.D[-1], self.W[-1]
+        dd = [dn - self.D[-1 - i] for i in range(1, k + 1)]
+        U = np.array([[np.mean(a * b) for b in dd] for a in dd])
+        V = np.array([np.mean(a * dn) for a in dd])
+        c = np.linalg.lstsq(U, V, rcond=None)[0]
+        wp = wn + sum(c[i] * (self.W[-2 - i] - wn) for i in range(k))
+        dp = dn + sum(c[i] * (self.D[-2 - i] - dn) for i in range(k))
+        return wp + self.lam * dp

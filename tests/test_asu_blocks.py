@@ -164,20 +164,27 @@ def test_block_cfft_vs_numpy(sg, N):
 
 
 @pytest.mark.gpu
-def test_block_cfft_full_block_spectral_consistency():
-    """A block spanning the whole grid (P1) transforms to the full FFT
-    (test/test_cfft.jl:37-68 checks blocks with size*step == N)."""
-    N = (8, 8, 8)
-    plan = cf.plan_cfft(N, 1)
-    blocks = plan.asu.blocks()
-    assert len(blocks) == 1 and blocks[0].data.shape == N
-    rng = np.random.default_rng(3)
-    x = plan.asu.similar(np.complex128)
-    u = rng.standard_normal(N) + 1j * rng.standard_normal(N)
-    x.blocks()[0].data[...] = u
-    y = plan.asu.similar(np.complex128)
-    cf.mul(y, plan, x)
-    assert _rel(y.blocks()[0].data, np.fft.fftn(u)) < 1e-12
+def test_block_cfft_parseval_and_linearity():
+    """Size-independent properties at 64^3 Im-3m: Parseval per block
+    (sum |X|^2 = n sum |x|^2) and linearity of mul."""
+    N = (64, 64, 64)
+    plan = cf.plan_cfft(N, 229)
+    rng = np.random.default_rng(5)
+    x1, x2 = plan.asu.similar(np.complex128), plan.asu.similar(np.complex128)
+    for b1, b2 in zip(x1.blocks(), x2.blocks()):
+        b1.data[...] = rng.standard_normal(b1.data.shape)
+        b2.data[...] = rng.standard_normal(b2.data.shape) * 1j
+    xs = plan.asu.similar(np.complex128)
+    for bs, b1, b2 in zip(xs.blocks(), x1.blocks(), x2.blocks()):
+        bs.data[...] = 2.0 * b1.data - 3.0 * b2.data
+    y1, y2, ys = (plan.asu.similar(np.complex128) for _ in range(3))
+    cf.mul(y1, plan, x1)
+    cf.mul(y2, plan, x2)
+    cf.mul(ys, plan, xs)
+    for a, b, c, x in zip(y1.blocks(), y2.blocks(), ys.blocks(), x1.blocks()):
+        assert _rel(c.data, 2.0 * a.data - 3.0 * b.data) < 1e-12
+        n = x.data.size
+        assert abs(np.sum(np.abs(a.data) ** 2) - n * np.sum(np.abs(x.data) ** 2)) <= 1e-10 * n * np.sum(np.abs(x.data) ** 2)
 
 
 @pytest.mark.gpu
