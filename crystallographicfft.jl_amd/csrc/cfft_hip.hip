@@ -43,6 +43,7 @@ struct cfft_plan {
   std::vector<int32_t> fmap_host;
   // options (cfft_plan_set_option)
   int persistent = 0;  // contour sweeps as one cooperative launch
+  int fuse = 1;        // fuse P5 of a step with P1 of the next
   // per-stage hipEvent timer (cfft_profile / cfft_profile_read)
   bool prof = false;
   struct Mark {
@@ -513,6 +514,17 @@ void launch_p5(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipSt
   launch_check<T>("k_p5");
 }
 
+template <typename T>
+void launch_p51(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipStream_t s) {
+  StageTimer st(p, s, CFFT_STAGE_UNFOLD_FOLD);
+  const unsigned g = (unsigned)n_tiles(p, pp);
+#define CFFT_P51(L) \
+  case L: hipLaunchKernelGGL((k_p51<T, double, L>), dim3(g), dim3(kNT), 0, s, pp, B, step); break;
+  CFFT_LEN_SWITCH(p->H[0], CFFT_P51)
+#undef CFFT_P51
+  launch_check<T>("k_p51");
+}
+
 // f0 -> channel spectra (P1, P2).
 template <typename T>
 void run_forward_channels(cfft_plan* p, const T* f0, const PipeP& pp, int step, hipStream_t s) {
@@ -672,11 +684,38 @@ void exp_dev(cfft_plan* p, const T* w, T* E, double factor, long n, hipStream_t 
   launch_check<T>("k_exp");
 }
 
+// nsteps Strang steps of the fwd + K + bwd paths as one launch chain: P1, P2
+// for the first step, then per step P3, P4 and either P5 fused with the next
+// step's P1 (followed by its P2) or, after the last step, P5.  Step i reads
+// `q + i * stride` and writes `q + (i + 1) * stride` (stride 0: in place).
+template <typename T>
+void run_chain(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int nsteps, hipStream_t s) {
+  cx<T>* F = static_cast<cx<T>*>(p->d_F.p);
+  run_forward_channels<T>(p, q, pp, step0, s);
+  for (int i = 0; i < nsteps; ++i) {
+    run_recon<T>(p, pp, F, true, s);
+    PipeBufs B = pipe_bufs(p, nullptr, q + (i + 1) * stride, F, false);
+    launch_p4<T>(p, pp, B, s);
+    if (i + 1 < nsteps && p->fuse) {
+      launch_p51<T>(p, pp, B, step0 + i, s);
+      launch_p2<T>(p, pp, B, s);
+    } else {
+      launch_p5<T>(p, pp, B, step0 + i, s);
+      if (i + 1 < nsteps) run_forward_channels<T>(p, q + (i + 1) * stride, pp, step0 + i + 1, s);
+    }
+  }
+}
+
 // nsteps Strang steps with the interaction factors of `es`.
 template <typename T>
 void run_steps(cfft_plan* p, T* q, const ESel& es, int step0, int nsteps, hipStream_t s) {
   PipeP pp = pipe_params(p, es);
-  if (nsteps > 0 && run_sweep_persistent<T>(p, q, pp, step0, nsteps, s)) return;
+  if (nsteps <= 0) return;
+  if (run_sweep_persistent<T>(p, q, pp, step0, nsteps, s)) return;
+  if (!is_q_path(p)) {
+    run_chain<T>(p, q, 0, pp, step0, nsteps, s);
+    return;
+  }
   for (int i = 0; i < nsteps; ++i) run_diffuse<T>(p, q, q, pp, step0 + i, s);
 }
 
@@ -686,6 +725,11 @@ template <typename T>
 void run_steps_hist(cfft_plan* p, T* hist, const ESel& es, int step0, int nsteps, hipStream_t s) {
   PipeP pp = pipe_params(p, es);
   const long slice = p->M3 * p->batch;
+  if (nsteps <= 0) return;
+  if (!is_q_path(p)) {
+    run_chain<T>(p, hist, slice, pp, step0, nsteps, s);
+    return;
+  }
   for (int i = 0; i < nsteps; ++i) run_diffuse<T>(p, hist + i * slice, hist + (i + 1) * slice, pp, step0 + i, s);
 }
 
@@ -1077,6 +1121,7 @@ int cfft_plan_set_option(cfft_plan* p, int32_t key, int64_t value) {
     check_plan(p);
     switch (key) {
       case CFFT_OPT_PERSISTENT: p->persistent = value != 0; break;
+      case CFFT_OPT_FUSE: p->fuse = value != 0; break;
       default: throw Error{CFFT_ERR_INVALID, "unknown option " + std::to_string(key)};
     }
   });

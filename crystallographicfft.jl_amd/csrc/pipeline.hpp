@@ -297,6 +297,76 @@ __device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, in
   __syncthreads();
 }
 
+// ---- P5 of step s fused with P1 of step s + 1 ------------------------------
+// IFFT x + unfold + E_s writes q(s + 1) to f0_out; the same thread then folds
+// E_{s+1} q(s + 1) at the same tile points, so q(s + 1) is not read back.
+template <typename T, typename TW, int L0>
+__device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds,
+                               cx<T>* xch) {
+  const int H0 = p.H[0], H1 = p.H[1], H2 = p.H[2], YB = p.YB;
+  const int nyb = H1 / YB;
+  const int yb = (int)(item % nyb);
+  const int z = (int)((item / nyb) % H2);
+  const int b = (int)(item / ((long)nyb * H2));
+  const int y0 = yb * YB;
+  const int rs = H0 + 1;
+  cx<T>* ch = static_cast<cx<T>*>(B.ch);
+  const int n = p.n_ch * H0 * YB;
+  for (int i = threadIdx.x; i < n; i += kNT) {
+    const int yy = i % YB, x = (i / YB) % H0, c = i / (YB * H0);
+    lds[(c * YB + yy) * rs + x] = ch[(((long)b * p.n_ch + c) * H0 + x) * ((long)H2 * H1) + (long)z * H1 + y0 + yy];
+  }
+  __syncthreads();
+  lds_fft<T, TW, L0, +1>(lds, p.n_ch * YB, rs, 1, xch, static_cast<const cx<TW>*>(B.tw0));
+  const long M3 = (long)p.M[0] * p.M[1] * p.M[2];
+  T* fb = static_cast<T*>(B.f0_out) + b * M3;
+  const T* Eo = pipe_E<T>(p, b, step, M3);
+  const T* En = pipe_E<T>(p, b, step + 1, M3);
+  const cx<TW>* twM = static_cast<const cx<TW>*>(B.twM);
+  for (int pt = threadIdx.x; pt < H0 * YB; pt += kNT) {
+    const int x = pt % H0, yy = pt / H0, y = y0 + yy;
+    T v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = T(0);
+    for (int c = 0; c < p.n_ch; ++c) {
+      const int o = p.offs[c];
+      const cx<TW> tw = fold_twiddle(o, twM, x, y, z, p.M[0], p.M[1]);
+      const cx<T> g = lds[(c * YB + yy) * rs + x];
+      const T val = (T)((TW)g.x * tw.x + (TW)g.y * tw.y);
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        if (e == o) v[e] = val;
+    }
+    wht8(v);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const long idx = corner_index(p, x, y, z, e);
+      T r = (T)((TW)v[e] * p.scale_unfold);
+      if (Eo) r *= Eo[idx];
+      fb[idx] = r;
+      v[e] = En ? r * En[idx] : r;
+    }
+    wht8(v);
+    // the entries of this thread's point only: no barrier needed before reuse
+    for (int c = 0; c < p.n_ch; ++c) {
+      const int o = p.offs[c];
+      T val = T(0);
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        if (e == o) val = v[e];
+      const cx<TW> tw = fold_twiddle(o, twM, x, y, z, p.M[0], p.M[1]);
+      lds[(c * YB + yy) * rs + x] = {(T)((TW)val * tw.x), (T)((TW)val * tw.y)};
+    }
+  }
+  __syncthreads();
+  lds_fft<T, TW, L0, -1>(lds, p.n_ch * YB, rs, 1, xch, static_cast<const cx<TW>*>(B.tw0));
+  for (int i = threadIdx.x; i < n; i += kNT) {
+    const int yy = i % YB, x = (i / YB) % H0, c = i / (YB * H0);
+    ch[(((long)b * p.n_ch + c) * H0 + x) * ((long)H2 * H1) + (long)z * H1 + y0 + yy] = lds[(c * YB + yy) * rs + x];
+  }
+  __syncthreads();
+}
+
 // ---- kernels: one per phase -------------------------------------------------
 template <typename T, typename TW, int L0>
 __global__ void __launch_bounds__(kNT) k_p1(PipeP p, PipeBufs B, int step) {
@@ -340,6 +410,13 @@ __global__ void __launch_bounds__(kNT) k_p5(PipeP p, PipeBufs B, int step) {
   __shared__ cx<T> lds[64 * (L0 + 1)];
   __shared__ cx<T> xch[PipeG<L0>::XCH];
   ph_ifftx_unfold<T, TW, L0>(p, B, blockIdx.x, step, lds, xch);
+}
+
+template <typename T, typename TW, int L0>
+__global__ void __launch_bounds__(kNT) k_p51(PipeP p, PipeBufs B, int step) {
+  __shared__ cx<T> lds[64 * (L0 + 1)];
+  __shared__ cx<T> xch[PipeG<L0>::XCH];
+  ph_unfold_fold<T, TW, L0>(p, B, blockIdx.x, step, lds, xch);
 }
 
 // ---- persistent sweep: nsteps Strang steps in one cooperative launch ---------

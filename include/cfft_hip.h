@@ -190,7 +190,8 @@ int cfft_exp_dev(cfft_plan* plan, const void* w, void* E, double factor, int64_t
 #define CFFT_STAGE_UNFOLD 9 /* P5: IFFT x + unfold + E               */
 #define CFFT_STAGE_QMUL 10
 #define CFFT_STAGE_SWEEP 11 /* persistent sweep (all phases, all steps) */
-#define CFFT_N_STAGES 12
+#define CFFT_STAGE_UNFOLD_FOLD 12 /* P5 of step s fused with P1 of s + 1 */
+#define CFFT_N_STAGES 13
 /* No counterpart in the reference (its benches time stages with @elapsed,
  * test/profile_centering_fold.jl:48-107).  enable != 0 brackets every
  * subsequent kernel launch of the plan with a pair of hipEvents. */
@@ -211,6 +212,9 @@ int cfft_sweep_hist_dev(cfft_plan* plan, void* hist, const void* const* E1, cons
  * cooperative launch with grid barriers between phases (default 0: one
  * launch per phase and step). */
 #define CFFT_OPT_PERSISTENT 1
+/* value != 0 (default 1): in multi-step calls, fuse the unfold of a step with
+ * the fold of the next (one tile kernel, q(s + 1) is not read back). */
+#define CFFT_OPT_FUSE 2
 /* No counterpart in the reference. */
 int cfft_plan_set_option(cfft_plan* plan, int32_t key, int64_t value);
 

@@ -183,12 +183,37 @@ class SpectralASU:
     N: tuple
 
 
+def _rot_closed(Rs):
+    """True when the rotation parts form a group (closed under products)."""
+    keys = {tuple(r.ravel()) for r in Rs}
+    return all(tuple((a @ b).ravel()) in keys for a in Rs for b in Rs)
+
+
+def _orbit_min_closed(Rs, N):
+    """For a closed rotation group: orbit minimum (column-major linear index)
+    and orbit size of every grid point, orbit = {R h mod N : R in Rs}."""
+    n_total = int(np.prod(N))
+    X = _coords(n_total, N)
+    Nv = np.asarray(N, dtype=np.int64)
+    lin = np.arange(n_total, dtype=np.int64)
+    mn = lin.copy()
+    stab = np.zeros(n_total, dtype=np.int64)
+    for Rg in Rs:
+        img = lin_index(np.mod(X @ Rg.T, Nv), N)
+        np.minimum(mn, img, out=mn)
+        stab += img == lin
+    # orbit size = |G| / |stabiliser| (each op counted once per distinct matrix)
+    return mn, len(Rs) // stab
+
+
 def calc_spectral_asu(R, t, N):
     """src/spectral_indexing.jl:22-176 (direct orbit enumeration).
 
     Orbits of [0,N)^D under R* = (R^-1)^T; representative = smallest
     column-major linear index; kept when |sum_{g in Stab} exp(-2 pi i k.t_g/N)|
-    > 1e-5; sorted lexicographically by (k1, k2, ..., kD).
+    > 1e-5; sorted lexicographically by (k1, k2, ..., kD).  A closed rotation
+    set takes the orbit as {R* h} directly; otherwise the closure is the
+    connected components of the op graph (as the reference's worklist).
     """
     N = tuple(int(n) for n in N)
     D = len(N)
@@ -196,15 +221,17 @@ def calc_spectral_asu(R, t, N):
     n_total = int(np.prod(N))
     Nv = np.asarray(N, dtype=np.int64)
     X = _coords(n_total, N)
-    # orbit minimum and orbit size: group closure taken from the op set by
-    # connected components (works for any op list, as the reference's worklist)
-    labels = _orbit_components(Rs, np.zeros_like(t), N)
-    sizes = np.bincount(labels)
-    # min linear index per component
-    minlin = np.full(sizes.size, n_total, dtype=np.int64)
-    np.minimum.at(minlin, labels, np.arange(n_total, dtype=np.int64))
-    reps = minlin
-    mult = sizes
+    if _rot_closed(Rs):
+        mn, size = _orbit_min_closed(np.unique(Rs, axis=0), N)
+        reps = np.flatnonzero(mn == np.arange(n_total))
+        mult = size[reps]
+    else:
+        labels = _orbit_components(Rs, np.zeros_like(t), N)
+        sizes = np.bincount(labels)
+        minlin = np.full(sizes.size, n_total, dtype=np.int64)
+        np.minimum.at(minlin, labels, np.arange(n_total, dtype=np.int64))
+        reps = minlin
+        mult = sizes
     K = X[reps]
     # extinction filter
     keep = np.zeros(reps.size, dtype=bool)

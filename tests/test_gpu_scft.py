@@ -10,7 +10,7 @@ from conftest import group_setup
 pytestmark = pytest.mark.gpu
 
 
-def sweep_case(sg, N, f, ns, prec, seed=3, persistent=0):
+def sweep_case(sg, N, f, ns, prec, seed=3, persistent=0, fuse=1):
     import torch
     from cfft_hip.scft import PropagatorSweep
     ops, ops_s, shift, R, t = group_setup(sg, N)
@@ -20,6 +20,7 @@ def sweep_case(sg, N, f, ns, prec, seed=3, persistent=0):
     wB = -1.5 * O.extract_subgrid(O.make_symmetric(R, t, N, seed=seed + 1), Lv)
     sw = PropagatorSweep(N, ops_s, lat, f, ns, prec)
     sw.plan.set_option(cf._lib.OPT_PERSISTENT, persistent)
+    sw.plan.set_option(cf._lib.OPT_FUSE, fuse)
     sw.set_fields(wA, wB)
     sw.reset()
     sw.run()
@@ -36,12 +37,12 @@ def sweep_case(sg, N, f, ns, prec, seed=3, persistent=0):
     return q, qd, ref_q, ref_qd, sw.Q()
 
 
-@pytest.mark.parametrize("persistent", [0, 1])
+@pytest.mark.parametrize("persistent,fuse", [(0, 0), (0, 1), (1, 1)])
 @pytest.mark.parametrize("sg,N", [(229, (32, 32, 32)), (230, (32, 32, 32)), (47, (32, 32, 32)),
-                                  (70, (32, 32, 32)), (10, (16, 16, 16))])
-def test_sweep_fp64_vs_oracle(sg, N, persistent):
-    print(f"sweep sg={sg} N={N} persistent={persistent}", flush=True)
-    q, qd, rq, rqd, (Q1, Q2) = sweep_case(sg, N, 0.3, 20, cf.FP64, persistent=persistent)
+                                  (70, (32, 32, 32)), (10, (16, 16, 16)), (71, (24, 32, 48))])
+def test_sweep_fp64_vs_oracle(sg, N, persistent, fuse):
+    print(f"sweep sg={sg} N={N} persistent={persistent} fuse={fuse}", flush=True)
+    q, qd, rq, rqd, (Q1, Q2) = sweep_case(sg, N, 0.3, 20, cf.FP64, persistent=persistent, fuse=fuse)
     assert np.max(np.abs(q - rq)) / np.max(np.abs(rq)) < 1e-12
     assert np.max(np.abs(qd - rqd)) / np.max(np.abs(rqd)) < 1e-12
     # Q from q(1) and from q+(1) agree (symmetry of the chain)
