@@ -153,6 +153,103 @@ def cpu_baseline(ops_s, wA, Lv, budget_s):
     return out
 
 
+def gyroid_fields(cf, N, sg, cell, amp, noise, seed):
+    """Ia-3d start: amp * (sin x cos y + sin y cos z + sin z cos x) (x in units
+    of 2 pi / cell) + seeded noise, symmetrised; on the subgrid."""
+    from cfft_hip.scft import gyroid_first_harmonic, symmetrize_subgrid
+    ops = cf.get_ops(sg, 3, N)
+    shift, ops_s = cf.find_optimal_shift(ops, N)
+    Lv = cf.auto_L(ops_s)
+    b = [s * n for s, n in zip(shift, N)]
+    rng = np.random.default_rng(seed)
+    g = gyroid_first_harmonic(N, b, cell)
+    sub = tuple(slice(0, None, int(l)) for l in Lv)
+    g0 = np.ascontiguousarray(g[sub])
+    nA = symmetrize_subgrid(rng.normal(0.0, noise, N), ops_s, Lv)
+    nB = symmetrize_subgrid(rng.normal(0.0, noise, N), ops_s, Lv)
+    return ops_s, Lv, amp * g0 + nA, -amp * g0 + nB
+
+
+def scft_im3m(cf, ops_s, wA, wB, tol, max_iter):
+    """Plain cold start of the Im-3m 128^3 melt (f 0.3, chiN 20, Ns 100) on the
+    device, FP64: first-iteration Q, F, densities; wall time to `tol`."""
+    lat = CELL * np.eye(3)
+    solver = cf.scft.SCFTSolver(N, ops_s, lat, F_A, CHIN, NS, cf.FP64, m_hist=10)
+    solver.set_fields(wA, wB)
+    first = solver.iterate()
+    phi0 = solver.densities()
+    solver.set_fields(wA, wB)
+    t0 = time.perf_counter()
+    it, out = solver.solve(tol=tol, max_iter=max_iter)
+    el = time.perf_counter() - t0
+    return first, phi0, it, out, el
+
+
+def scft_ia3d(cf, tol64, tol32, max_iter):
+    """Ia-3d 128^3, cell 3.8, f 0.35, chiN 18, Ns 200 from the gyroid first
+    harmonic: solve in FP64 and in mixed precision from the same start; errors
+    of Q, F and the densities of the mixed solution against the FP64 one."""
+    n3 = (128, 128, 128)
+    ops_s, Lv, wA, wB = gyroid_fields(cf, n3, 230, 3.8, 4.0, 0.05, 7)
+    lat = 3.8 * np.eye(3)
+    out = {}
+    for name, prec, tol in (("fp64", cf.FP64, tol64), ("mixed", cf.MIXED, tol32)):
+        solver = cf.scft.SCFTSolver(n3, ops_s, lat, 0.35, 18.0, 200, prec, m_hist=10)
+        solver.set_fields(wA, wB)
+        t0 = time.perf_counter()
+        it, o = solver.solve(tol=tol, max_iter=max_iter)
+        el = time.perf_counter() - t0
+        out[name] = (it, o, el, solver.densities())
+        del solver
+    i64, o64, e64, p64 = out["fp64"]
+    i32, o32, e32, p32 = out["mixed"]
+    return {
+        "fp64": {"iters": i64, "res": o64["res"], "F": o64["F"], "Q": o64["Q"], "wall_s": e64},
+        "mixed": {"iters": i32, "res": o32["res"], "F": o32["F"], "Q": o32["Q"], "wall_s": e32},
+        "abs_dF": abs(o32["F"] - o64["F"]),
+        "rel_dQ": abs(o32["Q"] - o64["Q"]) / abs(o64["Q"]),
+        "max_abs_dphi": float(max(np.max(np.abs(a - b)) for a, b in zip(p32, p64))),
+    }
+
+
+def cpu_scft_reference(ops_s, wA, wB):
+    """Q, F and densities of the first iteration on the CPU: the C restatement
+    of the Strang step for the sweeps, Simpson densities and F from the SCFT
+    oracle (FP64)."""
+    import oracle as O
+    from oracle import cstep
+    import cfft_hip as cf
+    R, t = cf.ops_arrays(ops_s)
+    if not os.path.exists(cstep.LIB):
+        cstep.build()
+    ds = 1.0 / NS
+    cs = cstep.CStep(R, t, N, ds, CELL * np.eye(3), nthreads=cpu_threads())
+    EA = np.asfortranarray(np.exp(-wA * ds / 2.0))
+    EB = np.asfortranarray(np.exp(-wB * ds / 2.0))
+    nA = int(round(F_A * NS))
+    q = [np.ones(cs.M, order="F")]
+    for j in range(NS):
+        q.append(cs.step(q[-1], EA if j < nA else EB))
+    qd = np.ones(cs.M, order="F")
+    wa, wb = O.contour_weights(nA), O.contour_weights(NS - nA)
+    phiA = np.zeros(cs.M)
+    phiB = np.zeros(cs.M)
+    for j in range(NS + 1):
+        # qd holds q+(j'); the density pairs q(ns - j') with q+(j')
+        k = NS - j
+        if k <= nA:
+            phiA += wa[k] * q[k] * qd
+        if k >= nA:
+            phiB += wb[k - nA] * q[k] * qd
+        if j < NS:
+            qd = cs.step(qd, EB if j < NS - nA else EA)
+    Q = float(np.mean(q[NS]))
+    phiA *= ds / Q
+    phiB *= ds / Q
+    F = O.free_energy(wA, wB, phiA, phiB, Q, CHIN)
+    return Q, F, phiA, phiB
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +257,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-scft", action="store_true")
+    ap.add_argument("--scft-iters", type=int, default=600)
     args = ap.parse_args()
     if args.gpus > 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -208,6 +307,21 @@ def main():
                          achieved=achieved, plan=sw.plan, sbytes=nb,
                          step_bytes=step_bytes(sw.plan, e))
         del sw
+
+    # ---- SCFT: Im-3m 128^3 plain cold start, Ia-3d 128^3 mixed vs FP64 ----
+    scft = None
+    if not args.no_scft:
+        first, phi0, it, sout, el = scft_im3m(cf, ops_s, wA, wB, 1e-8, args.scft_iters)
+        scft = {"im3m_128": {"first_iteration": {"Q": first["Q"], "F": first["F"]},
+                             "cold_start": {"iters": it, "res": sout["res"], "F": sout["F"],
+                                            "wall_s": el}}}
+        if not args.no_cpu:
+            Qc, Fc, pA, pB = cpu_scft_reference(ops_s, wA, wB)
+            scft["im3m_128"]["vs_cpu_reference"] = {
+                "Q_cpu": Qc, "F_cpu": Fc, "rel_dQ": abs(first["Q"] - Qc) / Qc,
+                "abs_dF": abs(first["F"] - Fc),
+                "max_abs_dphi": float(max(np.max(np.abs(phi0[0] - pA)), np.max(np.abs(phi0[1] - pB))))}
+        scft["ia3d_128_mixed"] = scft_ia3d(cf, 1e-8, 1e-6, args.scft_iters)
 
     r64, r32 = res["fp64"], res["mixed"]
     K = args.steps
@@ -260,6 +374,7 @@ def main():
         "launch_strategy": variants,
         "step_alg_bytes_fp64": r64["step_bytes"],
         "cpu_baseline": cpu,
+        "scft": scft,
     }
     print(json.dumps(line), flush=True)
 

@@ -128,7 +128,7 @@ struct cfft_pplan {
   std::vector<std::array<int, 3>> pars;
   SpecASU spec;
   std::vector<long> merge;
-  DevBuf d_u, d_ch, d_F, d_merge, d_tw[3];
+  DevBuf d_u, d_ch, d_F, d_merge;
   hipStream_t stream = nullptr;
   ~cfft_pplan() {
     if (stream) (void)hipStreamDestroy(stream);
@@ -155,10 +155,9 @@ void block_run(cfft_bplan* p, const double* in, double* out, int dir) {
 }
 
 template <int LEN>
-void fft_pass(cfft_pplan* p, cx<double>* data, long n_tr, long inner, long outer, long stride,
-              const cx<double>* tw) {
+void fft_pass(cfft_pplan* p, cx<double>* data, long n_tr, long inner, long outer, long stride) {
   hipLaunchKernelGGL((k_fft_ml<double, double, LEN, -1>), dim3(nblk(n_tr, MLGeom<LEN>::G)),
-                     dim3(MLGeom<LEN>::WG), 0, p->stream, data, n_tr, inner, outer, stride, tw);
+                     dim3(MLGeom<LEN>::WG), 0, p->stream, data, n_tr, inner, outer, stride);
   hip_check(hipGetLastError(), "k_fft_ml");
 }
 
@@ -173,20 +172,19 @@ void fft_axis_pre(cfft_pplan* p, int axis) {
     n_tr = nb * S0 * S1; inner = S0 * S1; outer = S0 * S1 * S2; stride = S0 * S1;
   }
   cx<double>* data = static_cast<cx<double>*>(p->d_ch.p);
-  const cx<double>* tw = static_cast<const cx<double>*>(p->d_tw[axis].p);
   switch (p->Ns[axis]) {
     case 1: return;
-    case 2: fft_pass<2>(p, data, n_tr, inner, outer, stride, tw); break;
-    case 3: fft_pass<3>(p, data, n_tr, inner, outer, stride, tw); break;
-    case 4: fft_pass<4>(p, data, n_tr, inner, outer, stride, tw); break;
-    case 6: fft_pass<6>(p, data, n_tr, inner, outer, stride, tw); break;
-    case 8: fft_pass<8>(p, data, n_tr, inner, outer, stride, tw); break;
-    case 12: fft_pass<12>(p, data, n_tr, inner, outer, stride, tw); break;
-    case 16: fft_pass<16>(p, data, n_tr, inner, outer, stride, tw); break;
-    case 24: fft_pass<24>(p, data, n_tr, inner, outer, stride, tw); break;
-    case 32: fft_pass<32>(p, data, n_tr, inner, outer, stride, tw); break;
-    case 48: fft_pass<48>(p, data, n_tr, inner, outer, stride, tw); break;
-    case 64: fft_pass<64>(p, data, n_tr, inner, outer, stride, tw); break;
+    case 2: fft_pass<2>(p, data, n_tr, inner, outer, stride); break;
+    case 3: fft_pass<3>(p, data, n_tr, inner, outer, stride); break;
+    case 4: fft_pass<4>(p, data, n_tr, inner, outer, stride); break;
+    case 6: fft_pass<6>(p, data, n_tr, inner, outer, stride); break;
+    case 8: fft_pass<8>(p, data, n_tr, inner, outer, stride); break;
+    case 12: fft_pass<12>(p, data, n_tr, inner, outer, stride); break;
+    case 16: fft_pass<16>(p, data, n_tr, inner, outer, stride); break;
+    case 24: fft_pass<24>(p, data, n_tr, inner, outer, stride); break;
+    case 32: fft_pass<32>(p, data, n_tr, inner, outer, stride); break;
+    case 48: fft_pass<48>(p, data, n_tr, inner, outer, stride); break;
+    case 64: fft_pass<64>(p, data, n_tr, inner, outer, stride); break;
     default: throw Error{CFFT_ERR_UNSUPPORTED, "FFT length not supported"};
   }
 }
@@ -374,12 +372,6 @@ int cfft_prefold_plan_create(const int32_t* N, const int32_t* ops, int32_t nops,
     p->d_merge.alloc(8 * std::max<size_t>(1, p->merge.size()));
     if (!p->merge.empty())
       hip_check(hipMemcpy(p->d_merge.p, p->merge.data(), 8 * p->merge.size(), hipMemcpyHostToDevice), "upload");
-    for (int d = 0; d < 3; ++d) {
-      std::vector<cx<double>> tw(p->Ns[d]);
-      for (int m = 0; m < p->Ns[d]; ++m) cis2pi(-m, p->Ns[d], tw[m].x, tw[m].y);
-      p->d_tw[d].alloc(16 * tw.size());
-      hip_check(hipMemcpy(p->d_tw[d].p, tw.data(), 16 * tw.size(), hipMemcpyHostToDevice), "upload");
-    }
     *out = p.release();
   });
 }

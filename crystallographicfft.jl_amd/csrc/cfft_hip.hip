@@ -37,13 +37,20 @@ struct cfft_plan {
   size_t esz = 8;  // element size of the plan's real type
   hipStream_t stream = nullptr;
   // device tables
-  DevBuf d_hs, d_K, d_W, d_twM, d_twF[3], d_btab, d_Q, d_gidx, d_fmap, d_chmap;
+  DevBuf d_hs, d_K, d_W, d_twM, d_btab, d_Q, d_gidx, d_fmap, d_chmap;
   // device work buffers
   DevBuf d_f0, d_ch, d_ch2, d_F, d_E, d_u;
   std::vector<int32_t> fmap_host;
   // options (cfft_plan_set_option)
   int persistent = 0;  // contour sweeps as one cooperative launch
   int fuse = 1;        // fuse P5 of a step with P1 of the next
+  int graph = 1;       // replay multi-step chains from captured hipGraphs
+  hipStream_t cap_stream = nullptr;  // capture stream of the chains
+  struct Cached {
+    std::string key;
+    hipGraphExec_t exec;
+  };
+  std::vector<Cached> graphs;  // most recently used last
   // per-stage hipEvent timer (cfft_profile / cfft_profile_read)
   bool prof = false;
   struct Mark {
@@ -52,6 +59,8 @@ struct cfft_plan {
   };
   std::vector<Mark> marks;
   ~cfft_plan() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
     for (auto& m : marks) {
       (void)hipEventDestroy(m.a);
       (void)hipEventDestroy(m.b);
@@ -84,7 +93,8 @@ void upload_bytes(DevBuf& buf, const void* src, size_t n) {
   if (n) hip_check(hipMemcpy(buf.p, src, n, hipMemcpyHostToDevice), "upload");
 }
 
-// Phase table W[j] = exp(2 pi i j/P) and fold / FFT twiddle tables.
+// Phase table W[j] = exp(2 pi i j/P) and fold twiddle tables (the FFT
+// twiddles are compile-time constants).
 template <typename T>
 void build_tables(cfft_plan* p) {
   std::vector<double> re, im;
@@ -103,11 +113,6 @@ void build_tables(cfft_plan* p) {
       im.push_back(b);
     }
   upload_complex<T>(p->d_twM, re, im);
-  for (int d = 0; d < 3; ++d) {
-    std::vector<double> fr(p->H[d]), fi(p->H[d]);
-    for (int m = 0; m < p->H[d]; ++m) cis2pi(-m, p->H[d], fr[m], fi[m]);
-    upload_complex<T>(p->d_twF[d], fr, fi);
-  }
   upload_real<T>(p->d_K, p->K);
 }
 
@@ -333,13 +338,12 @@ void fft_axis(cfft_plan* p, cx<T>* data, int axis, hipStream_t s) {
   } else {
     n_tr = nb * H0 * H1; inner = H1; outer = H1 * H2; stride = H1;
   }
-  const cx<double>* tw = static_cast<const cx<double>*>(p->d_twF[axis].p);
   StageTimer st(p, s, (DIR < 0 ? CFFT_STAGE_FFT_X : CFFT_STAGE_IFFT_X) + axis);
 #define CFFT_FFT_CASE(LEN)                                                                      \
   case LEN:                                                                                     \
     hipLaunchKernelGGL((k_fft_ml<T, double, LEN, DIR>),                                          \
                        dim3(nblk(n_tr, MLGeom<LEN>::G)), dim3(MLGeom<LEN>::WG), 0, s, data, n_tr, \
-                       inner, outer, stride, tw);                                               \
+                       inner, outer, stride);                                                   \
     break;
   CFFT_LEN_SWITCH(len, CFFT_FFT_CASE)
 #undef CFFT_FFT_CASE
@@ -376,7 +380,8 @@ struct ESel {
 };
 
 PipeP pipe_params(cfft_plan* p, const ESel& es) {
-  PipeP pp{};
+  PipeP pp;
+  std::memset(&pp, 0, sizeof(pp));  // padding too: the bytes key the graph cache
   for (int d = 0; d < 3; ++d) {
     pp.M[d] = p->M[d];
     pp.H[d] = p->H[d];
@@ -429,9 +434,6 @@ PipeBufs pipe_bufs(cfft_plan* p, const void* f0_in, void* f0_out, void* F, bool 
   B.K = withK ? static_cast<const double*>(p->d_K.p) : nullptr;
   B.W = p->d_W.p;
   B.twM = p->d_twM.p;
-  B.tw0 = p->d_twF[0].p;
-  B.tw1 = p->d_twF[1].p;
-  B.tw2 = p->d_twF[2].p;
   B.btab = static_cast<const uint32_t*>(p->d_btab.p);
   return B;
 }
@@ -706,17 +708,74 @@ void run_chain(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int 
   }
 }
 
+// Launch sequence of nsteps steps (fused chain, or one diffusion per step on
+// the Q paths).
+template <typename T>
+void launch_steps(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int nsteps, hipStream_t s) {
+  if (!is_q_path(p)) {
+    run_chain<T>(p, q, stride, pp, step0, nsteps, s);
+    return;
+  }
+  for (int i = 0; i < nsteps; ++i) run_diffuse<T>(p, q + i * stride, q + (i + 1) * stride, pp, step0 + i, s);
+}
+
+constexpr size_t kMaxGraphs = 16;
+
+// The launch sequence of (q, stride, factors, step0, nsteps) is captured once
+// into a hipGraph on the plan's capture stream and replayed on `s`.  Kernel
+// arguments are baked into the graph, so the cache key holds every pointer
+// and scalar the sequence depends on.  The chain contains kernel nodes only.
+template <typename T>
+void run_steps_graph(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int nsteps, hipStream_t s) {
+  std::string key(reinterpret_cast<const char*>(&pp), sizeof(PipeP));
+  const long hdr[5] = {reinterpret_cast<long>(q), stride, step0, nsteps, p->fuse};
+  key.append(reinterpret_cast<const char*>(hdr), sizeof(hdr));
+  for (size_t i = 0; i < p->graphs.size(); ++i)
+    if (p->graphs[i].key == key) {
+      cfft_plan::Cached c = p->graphs[i];
+      p->graphs.erase(p->graphs.begin() + i);
+      p->graphs.push_back(c);
+      hip_check(hipGraphLaunch(c.exec, s), "hipGraphLaunch");
+      return;
+    }
+  if (!p->cap_stream) hip_check(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking), "hipStreamCreate");
+  hipGraph_t g = nullptr;
+  hip_check(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  try {
+    launch_steps<T>(p, q, stride, pp, step0, nsteps, p->cap_stream);
+  } catch (...) {
+    (void)hipStreamEndCapture(p->cap_stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  hip_check(hipStreamEndCapture(p->cap_stream, &g), "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  hip_check(e, "hipGraphInstantiate");
+  if (p->graphs.size() >= kMaxGraphs) {
+    (void)hipGraphExecDestroy(p->graphs.front().exec);
+    p->graphs.erase(p->graphs.begin());
+  }
+  p->graphs.push_back({key, exec});
+  hip_check(hipGraphLaunch(exec, s), "hipGraphLaunch");
+}
+
+template <typename T>
+void dispatch_steps(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int nsteps, hipStream_t s) {
+  if (p->graph && !p->prof && nsteps > 1)
+    run_steps_graph<T>(p, q, stride, pp, step0, nsteps, s);
+  else
+    launch_steps<T>(p, q, stride, pp, step0, nsteps, s);
+}
+
 // nsteps Strang steps with the interaction factors of `es`.
 template <typename T>
 void run_steps(cfft_plan* p, T* q, const ESel& es, int step0, int nsteps, hipStream_t s) {
   PipeP pp = pipe_params(p, es);
   if (nsteps <= 0) return;
   if (run_sweep_persistent<T>(p, q, pp, step0, nsteps, s)) return;
-  if (!is_q_path(p)) {
-    run_chain<T>(p, q, 0, pp, step0, nsteps, s);
-    return;
-  }
-  for (int i = 0; i < nsteps; ++i) run_diffuse<T>(p, q, q, pp, step0 + i, s);
+  dispatch_steps<T>(p, q, 0, pp, step0, nsteps, s);
 }
 
 // nsteps Strang steps keeping every contour point: step i reads slice i of
@@ -724,13 +783,8 @@ void run_steps(cfft_plan* p, T* q, const ESel& es, int step0, int nsteps, hipStr
 template <typename T>
 void run_steps_hist(cfft_plan* p, T* hist, const ESel& es, int step0, int nsteps, hipStream_t s) {
   PipeP pp = pipe_params(p, es);
-  const long slice = p->M3 * p->batch;
   if (nsteps <= 0) return;
-  if (!is_q_path(p)) {
-    run_chain<T>(p, hist, slice, pp, step0, nsteps, s);
-    return;
-  }
-  for (int i = 0; i < nsteps; ++i) run_diffuse<T>(p, hist + i * slice, hist + (i + 1) * slice, pp, step0 + i, s);
+  dispatch_steps<T>(p, hist, p->M3 * p->batch, pp, step0, nsteps, s);
 }
 
 template <typename T>
@@ -1122,6 +1176,7 @@ int cfft_plan_set_option(cfft_plan* p, int32_t key, int64_t value) {
     switch (key) {
       case CFFT_OPT_PERSISTENT: p->persistent = value != 0; break;
       case CFFT_OPT_FUSE: p->fuse = value != 0; break;
+      case CFFT_OPT_GRAPH: p->graph = value != 0; break;
       default: throw Error{CFFT_ERR_INVALID, "unknown option " + std::to_string(key)};
     }
   });

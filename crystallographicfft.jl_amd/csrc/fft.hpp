@@ -3,13 +3,15 @@
 // One lane owns one length-LEN transform held entirely in VGPRs.  The
 // transform is an iterative Stockham autosort (radix 4, then 2, then 3 passes)
 // whose loops are fully unrolled at compile time, so every register index and
-// every twiddle index is a constant: twiddles are uniform loads from a small
-// FP64-accurate table (exp(-2 pi i m/LEN), m < LEN) that the compiler turns
-// into scalar loads.  Forward = exp(-2 pi i jk/n) unnormalised, inverse =
+// every twiddle index is a constant: twiddles come from compile-time FP64
+// tables (twiddles.hpp) and fold into literal operands, with no memory
+// traffic.  Forward = exp(-2 pi i jk/n) unnormalised, inverse =
 // exp(+2 pi i jk/n) unnormalised (scaling is folded into the callers).
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include "twiddles.hpp"
 
 namespace cfft {
 
@@ -44,6 +46,17 @@ __host__ __device__ constexpr bool fft_len_ok(int len) {
   while (n % 2 == 0) n /= 2;
   while (n % 3 == 0) n /= 3;
   return n == 1;
+}
+
+// exp(-2 pi i m / LEN), 0 <= m < LEN, for LEN dividing 64 or 48.  With m a
+// compile-time constant after unrolling, both parts are literals.
+template <int LEN, typename TW>
+__device__ __forceinline__ cx<TW> tw_c(int m) {
+  static_assert(64 % LEN == 0 || 48 % LEN == 0, "FFT length must divide 64 or 48");
+  if constexpr (64 % LEN == 0)
+    return {(TW)kTwRe64[m * (64 / LEN)], (TW)kTwIm64[m * (64 / LEN)]};
+  else
+    return {(TW)kTwRe48[m * (48 / LEN)], (TW)kTwIm48[m * (48 / LEN)]};
 }
 
 // x * w (DIR < 0) or x * conj(w) (DIR > 0), the product formed in the
@@ -99,10 +112,9 @@ __device__ __forceinline__ void dft_small(cx<T>* v) {
 }
 
 // Stockham pass(es) starting at sub-transform length NS on LEN values held in
-// registers; twiddle exp(-2 pi i m/LEN) is tw[m * TWS] (a table of a longer
-// transform can be shared by passing its stride).
-template <typename T, typename TW, int LEN, int DIR, int NS, int TWS>
-__device__ __forceinline__ void stockham(cx<T> (&a)[LEN], const cx<TW>* __restrict__ tw) {
+// registers; twiddle exp(-2 pi i m/LEN).
+template <typename T, typename TW, int LEN, int DIR, int NS>
+__device__ __forceinline__ void stockham(cx<T> (&a)[LEN]) {
   if constexpr (NS < LEN) {
     constexpr int R = fft_radix(LEN, NS);
     constexpr int J = LEN / R;
@@ -116,7 +128,7 @@ __device__ __forceinline__ void stockham(cx<T> (&a)[LEN], const cx<TW>* __restri
         v[r] = a[j + r * J];
         if (r > 0 && k > 0) {
           const int m = (r * k * (LEN / (NS * R))) % LEN;
-          v[r] = twmul<T, TW, DIR>(v[r], tw[m * TWS]);
+          v[r] = twmul<T, TW, DIR>(v[r], tw_c<LEN, TW>(m));
         }
       }
       dft_small<R, DIR, T, TW>(v);
@@ -125,14 +137,14 @@ __device__ __forceinline__ void stockham(cx<T> (&a)[LEN], const cx<TW>* __restri
     }
 #pragma unroll
     for (int i = 0; i < LEN; ++i) a[i] = b[i];
-    stockham<T, TW, LEN, DIR, NS * R, TWS>(a, tw);
+    stockham<T, TW, LEN, DIR, NS * R>(a);
   }
 }
 
 template <typename T, int LEN, int DIR>
-__device__ __forceinline__ void fft_reg(cx<T> (&a)[LEN], const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void fft_reg(cx<T> (&a)[LEN]) {
   static_assert(fft_len_ok(LEN), "FFT length must be 2^a 3^b");
-  stockham<T, T, LEN, DIR, 1, 1>(a, tw);
+  stockham<T, T, LEN, DIR, 1>(a);
 }
 
 // Split of a length-LEN transform over R1 cooperating lanes with R2 = LEN/R1
@@ -150,13 +162,19 @@ struct MLS {
 // xch is the workgroup's exchange area of R2 * R1 * GP entries (GP = groups per
 // workgroup + 1); every lane of the workgroup must call this function.
 template <typename T, typename TW, int LEN, int DIR, int GP>
-__device__ __forceinline__ void fft_ml(cx<T> (&v)[MLS<LEN>::R2], int r, int g, cx<T>* xch,
-                                       const cx<TW>* __restrict__ tw) {
+__device__ __forceinline__ void fft_ml(cx<T> (&v)[MLS<LEN>::R2], int r, int g, cx<T>* xch) {
   constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2;
-  stockham<T, TW, R2, DIR, 1, R1>(v, tw);
+  stockham<T, TW, R2, DIR, 1>(v);
   if constexpr (R1 > 1) {
+    // w^(r k) for the lane index r < R1: a select among R1 literals
 #pragma unroll
-    for (int k = 1; k < R2; ++k) v[k] = twmul<T, TW, DIR>(v[k], tw[(r * k) % LEN]);
+    for (int k = 1; k < R2; ++k) {
+      cx<TW> w = tw_c<LEN, TW>(0);
+#pragma unroll
+      for (int s = 1; s < R1; ++s)
+        if (r == s) w = tw_c<LEN, TW>((s * k) % LEN);
+      v[k] = twmul<T, TW, DIR>(v[k], w);
+    }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < R2; ++k) xch[(k * R1 + r) * GP + g] = v[k];

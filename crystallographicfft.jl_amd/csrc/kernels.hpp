@@ -95,8 +95,7 @@ struct MLGeom {
 // base + j*stride (j < LEN), base = (i % inner) + (i / inner) * outer.
 template <typename T, typename TW, int LEN, int DIR>
 __global__ void __launch_bounds__(MLGeom<LEN>::WG)
-    k_fft_ml(cx<T>* __restrict__ data, long n_tr, long inner, long outer, long stride,
-             const cx<TW>* __restrict__ tw) {
+    k_fft_ml(cx<T>* __restrict__ data, long n_tr, long inner, long outer, long stride) {
   constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2, G = MLGeom<LEN>::G, GP = MLGeom<LEN>::GP;
   __shared__ cx<T> xch[MLGeom<LEN>::XCH];
   const int r = threadIdx.x % R1, g = threadIdx.x / R1;
@@ -106,7 +105,7 @@ __global__ void __launch_bounds__(MLGeom<LEN>::WG)
   cx<T> a[R2];
 #pragma unroll
   for (int m = 0; m < R2; ++m) a[m] = valid ? data[base + (long)(r + R1 * m) * stride] : cx<T>{T(0), T(0)};
-  fft_ml<T, TW, LEN, DIR, GP>(a, r, g, xch, tw);
+  fft_ml<T, TW, LEN, DIR, GP>(a, r, g, xch);
   if (valid) {
 #pragma unroll
     for (int q = 0; q < R2; ++q) data[base + (long)ml_out_pos<LEN>(r, q) * stride] = a[q];

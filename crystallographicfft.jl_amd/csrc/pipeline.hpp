@@ -49,9 +49,6 @@ struct PipeBufs {
   const double* K;    // kernel (nullptr: 1)
   const void* W;      // exp(2 pi i j / P)
   const void* twM;    // fold twiddles (M0 + M1 + M2)
-  const void* tw0;    // FFT twiddles per axis
-  const void* tw1;
-  const void* tw2;
   const uint32_t* btab;  // inverse reconstruction table [e][8]
 };
 
@@ -66,8 +63,7 @@ struct PipeG {
 // FFT of n_tr length-LEN transforms held in LDS: element j of transform t is
 // buf[t * ts + j * es].  Every thread of the workgroup must call.
 template <typename T, typename TW, int LEN, int DIR>
-__device__ __forceinline__ void lds_fft(cx<T>* buf, int n_tr, int ts, int es, cx<T>* xch,
-                                        const cx<TW>* __restrict__ tw) {
+__device__ __forceinline__ void lds_fft(cx<T>* buf, int n_tr, int ts, int es, cx<T>* xch) {
   if constexpr (LEN > 1) {
     constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2, G = PipeG<LEN>::G, GP = PipeG<LEN>::GP;
     const int r = threadIdx.x % R1, g = threadIdx.x / R1;
@@ -77,7 +73,7 @@ __device__ __forceinline__ void lds_fft(cx<T>* buf, int n_tr, int ts, int es, cx
       cx<T> v[R2];
 #pragma unroll
       for (int m = 0; m < R2; ++m) v[m] = ok ? buf[t * ts + (r + R1 * m) * es] : cx<T>{T(0), T(0)};
-      fft_ml<T, TW, LEN, DIR, GP>(v, r, g, xch, tw);
+      fft_ml<T, TW, LEN, DIR, GP>(v, r, g, xch);
       if (ok) {
 #pragma unroll
         for (int q = 0; q < R2; ++q) buf[t * ts + ml_out_pos<LEN>(r, q) * es] = v[q];
@@ -96,6 +92,78 @@ __device__ __forceinline__ const T* pipe_E(const PipeP& p, int b, int step, long
 __device__ __forceinline__ long corner_index(const PipeP& p, int x, int y, int z, int e) {
   return (x + (e & 1) * p.H[0]) +
          (long)p.M[0] * ((y + ((e >> 1) & 1) * p.H[1]) + (long)p.M[1] * (z + ((e >> 2) & 1) * p.H[2]));
+}
+
+// Plane of L1 x L2 entries (y fastest, contiguous) into LDS rows of stride
+// L1 + 1.  The trip count is a compile-time constant, so every global load is
+// issued before the first LDS write.
+template <typename T, int L1, int L2>
+__device__ __forceinline__ void plane_to_lds(const cx<T>* __restrict__ src, cx<T>* lds) {
+  constexpr int NP = L1 * L2, NIT = (NP + kNT - 1) / kNT;
+  cx<T> r[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int i = threadIdx.x + it * kNT;
+    if (i < NP) r[it] = src[i];
+  }
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int i = threadIdx.x + it * kNT;
+    if (i < NP) lds[(i / L1) * (L1 + 1) + i % L1] = r[it];
+  }
+}
+
+template <typename T, int L1, int L2>
+__device__ __forceinline__ void lds_to_plane(const cx<T>* lds, cx<T>* __restrict__ dst) {
+  constexpr int NP = L1 * L2, NIT = (NP + kNT - 1) / kNT;
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int i = threadIdx.x + it * kNT;
+    if (i < NP) dst[i] = lds[(i / L1) * (L1 + 1) + i % L1];
+  }
+}
+
+// Tile (b, z, y0 .. y0 + YB) of every channel into LDS A[c][yy][x] (row
+// stride L0 + 1); n_ch * YB <= 64 bounds the compile-time trip count.
+template <typename T, int L0>
+__device__ __forceinline__ void tile_to_lds(const PipeP& p, const cx<T>* __restrict__ ch, int b, int z,
+                                            int y0, cx<T>* lds) {
+  constexpr int MAXIT = (64 * L0 + kNT - 1) / kNT;
+  const int YB = p.YB, n = p.n_ch * L0 * YB;
+  const long HH = (long)p.H[2] * p.H[1];
+  cx<T> r[MAXIT];
+#pragma unroll
+  for (int it = 0; it < MAXIT; ++it) {
+    const int i = threadIdx.x + it * kNT;
+    if (i < n) {
+      const int yy = i % YB, x = (i / YB) % L0, c = i / (YB * L0);
+      r[it] = ch[(((long)b * p.n_ch + c) * L0 + x) * HH + (long)z * p.H[1] + y0 + yy];
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < MAXIT; ++it) {
+    const int i = threadIdx.x + it * kNT;
+    if (i < n) {
+      const int yy = i % YB, x = (i / YB) % L0, c = i / (YB * L0);
+      lds[(c * YB + yy) * (L0 + 1) + x] = r[it];
+    }
+  }
+}
+
+template <typename T, int L0>
+__device__ __forceinline__ void lds_to_tile(const PipeP& p, const cx<T>* lds, cx<T>* __restrict__ ch, int b,
+                                            int z, int y0) {
+  constexpr int MAXIT = (64 * L0 + kNT - 1) / kNT;
+  const int YB = p.YB, n = p.n_ch * L0 * YB;
+  const long HH = (long)p.H[2] * p.H[1];
+#pragma unroll
+  for (int it = 0; it < MAXIT; ++it) {
+    const int i = threadIdx.x + it * kNT;
+    if (i < n) {
+      const int yy = i % YB, x = (i / YB) % L0, c = i / (YB * L0);
+      ch[(((long)b * p.n_ch + c) * L0 + x) * HH + (long)z * p.H[1] + y0 + yy] = lds[(c * YB + yy) * (L0 + 1) + x];
+    }
+  }
 }
 
 // ---- P1: fold + E + FFT x --------------------------------------------------
@@ -134,29 +202,22 @@ __device__ void ph_fold_fftx(const PipeP& p, const PipeBufs& B, long item, int s
     }
   }
   __syncthreads();
-  lds_fft<T, TW, L0, -1>(lds, p.n_ch * YB, rs, 1, xch, static_cast<const cx<TW>*>(B.tw0));
+  lds_fft<T, TW, L0, -1>(lds, p.n_ch * YB, rs, 1, xch);
   // store to ch[b][c][x][z][y] (y fastest within a tile row)
-  cx<T>* ch = static_cast<cx<T>*>(B.ch);
-  const int n = p.n_ch * H0 * YB;
-  for (int i = threadIdx.x; i < n; i += kNT) {
-    const int yy = i % YB, x = (i / YB) % H0, c = i / (YB * H0);
-    ch[(((long)b * p.n_ch + c) * H0 + x) * ((long)H2 * H1) + (long)z * H1 + y0 + yy] =
-        lds[(c * YB + yy) * rs + x];
-  }
+  lds_to_tile<T, L0>(p, lds, static_cast<cx<T>*>(B.ch), b, z, y0);
   __syncthreads();
 }
 
 // ---- P2: FFT y then FFT z of one (b, c, x) plane ----------------------------
 template <typename T, typename TW, int L1, int L2>
 __device__ void ph_fft_yz(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds, cx<T>* xch) {
-  const int H1 = p.H[1], H2 = p.H[2];
-  const int rs = H1 + 1;
-  cx<T>* plane = static_cast<cx<T>*>(B.ch) + item * ((long)H1 * H2);
-  for (int i = threadIdx.x; i < H1 * H2; i += kNT) lds[(i / H1) * rs + (i % H1)] = plane[i];
+  constexpr int rs = L1 + 1;
+  cx<T>* plane = static_cast<cx<T>*>(B.ch) + item * ((long)L1 * L2);
+  plane_to_lds<T, L1, L2>(plane, lds);
   __syncthreads();
-  lds_fft<T, TW, L1, -1>(lds, H2, rs, 1, xch, static_cast<const cx<TW>*>(B.tw1));
-  lds_fft<T, TW, L2, -1>(lds, H1, 1, rs, xch, static_cast<const cx<TW>*>(B.tw2));
-  for (int i = threadIdx.x; i < H1 * H2; i += kNT) plane[i] = lds[(i / H1) * rs + (i % H1)];
+  lds_fft<T, TW, L1, -1>(lds, L2, rs, 1, xch);
+  lds_fft<T, TW, L2, -1>(lds, L1, 1, rs, xch);
+  lds_to_plane<T, L1, L2>(lds, plane);
   __syncthreads();
 }
 
@@ -197,19 +258,29 @@ __device__ __forceinline__ void ph_recon_point(const PipeP& p, const PipeBufs& B
 template <typename T, typename TW, int L1, int L2>
 __device__ void ph_irecon_ifft_zy(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds,
                                   cx<T>* xch) {
-  const int H1 = p.H[1], H2 = p.H[2];
-  const int rs = H1 + 1;
-  const long H3 = (long)p.H[0] * H1 * H2;
+  constexpr int rs = L1 + 1, NP = L1 * L2, NIT = (NP + kNT - 1) / kNT;
+  const long H3 = (long)p.H[0] * NP;
   const long n_alive = (long)p.n_ch * H3;
   const int b = (int)(item / ((long)p.n_ch * p.H[0]));
-  const long e0 = item * ((long)H1 * H2) - (long)b * n_alive;  // channel-space start
+  const long e0 = item * (long)NP - (long)b * n_alive;  // channel-space start
   const cx<T>* Fb = static_cast<const cx<T>*>(B.F) + (long)b * p.n_spec;
   const cx<TW>* W = static_cast<const cx<TW>*>(B.W);
   const uint4* tab4 = reinterpret_cast<const uint4*>(B.btab);
-  for (int i = threadIdx.x; i < H1 * H2; i += kNT) {
-    const long e = e0 + i;
-    const uint4 ta = tab4[2 * e], tb = tab4[2 * e + 1];
-    const uint32_t ent[8] = {ta.x, ta.y, ta.z, ta.w, tb.x, tb.y, tb.z, tb.w};
+  // all table rows of this thread first, then the gathers
+  uint4 ta[NIT], tb[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int i = threadIdx.x + it * kNT;
+    if (i < NP) {
+      ta[it] = tab4[2 * (e0 + i)];
+      tb[it] = tab4[2 * (e0 + i) + 1];
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int i = threadIdx.x + it * kNT;
+    if (i >= NP) continue;
+    const uint32_t ent[8] = {ta[it].x, ta[it].y, ta[it].z, ta[it].w, tb[it].x, tb[it].y, tb[it].z, tb[it].w};
     cx<T> acc = {T(0), T(0)};
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
@@ -220,13 +291,12 @@ __device__ void ph_irecon_ifft_zy(const PipeP& p, const PipeBufs& B, long item, 
       const cx<T> prod = twmul<T, TW, -1>(f, W[(ent[a] >> 20) & 0x7FFu]);
       if (ok) acc = cadd(acc, prod);
     }
-    lds[(i / H1) * rs + (i % H1)] = {(T)((TW)acc.x * p.scale_irecon), (T)((TW)acc.y * p.scale_irecon)};
+    lds[(i / L1) * rs + (i % L1)] = {(T)((TW)acc.x * p.scale_irecon), (T)((TW)acc.y * p.scale_irecon)};
   }
   __syncthreads();
-  lds_fft<T, TW, L2, +1>(lds, H1, 1, rs, xch, static_cast<const cx<TW>*>(B.tw2));
-  lds_fft<T, TW, L1, +1>(lds, H2, rs, 1, xch, static_cast<const cx<TW>*>(B.tw1));
-  cx<T>* plane = static_cast<cx<T>*>(B.ch) + item * ((long)H1 * H2);
-  for (int i = threadIdx.x; i < H1 * H2; i += kNT) plane[i] = lds[(i / H1) * rs + (i % H1)];
+  lds_fft<T, TW, L2, +1>(lds, L1, 1, rs, xch);
+  lds_fft<T, TW, L1, +1>(lds, L2, rs, 1, xch);
+  lds_to_plane<T, L1, L2>(lds, static_cast<cx<T>*>(B.ch) + item * (long)NP);
   __syncthreads();
 }
 
@@ -234,15 +304,12 @@ __device__ void ph_irecon_ifft_zy(const PipeP& p, const PipeBufs& B, long item, 
 template <typename T, typename TW, int L1, int L2>
 __device__ void ph_ifft_zy(const PipeP& p, const PipeBufs& B, const cx<T>* src, long item,
                            cx<T>* lds, cx<T>* xch) {
-  const int H1 = p.H[1], H2 = p.H[2];
-  const int rs = H1 + 1;
-  const cx<T>* in = src + item * ((long)H1 * H2);
-  for (int i = threadIdx.x; i < H1 * H2; i += kNT) lds[(i / H1) * rs + (i % H1)] = in[i];
+  constexpr int rs = L1 + 1;
+  plane_to_lds<T, L1, L2>(src + item * ((long)L1 * L2), lds);
   __syncthreads();
-  lds_fft<T, TW, L2, +1>(lds, H1, 1, rs, xch, static_cast<const cx<TW>*>(B.tw2));
-  lds_fft<T, TW, L1, +1>(lds, H2, rs, 1, xch, static_cast<const cx<TW>*>(B.tw1));
-  cx<T>* plane = static_cast<cx<T>*>(B.ch) + item * ((long)H1 * H2);
-  for (int i = threadIdx.x; i < H1 * H2; i += kNT) plane[i] = lds[(i / H1) * rs + (i % H1)];
+  lds_fft<T, TW, L2, +1>(lds, L1, 1, rs, xch);
+  lds_fft<T, TW, L1, +1>(lds, L2, rs, 1, xch);
+  lds_to_plane<T, L1, L2>(lds, static_cast<cx<T>*>(B.ch) + item * ((long)L1 * L2));
   __syncthreads();
 }
 
@@ -259,15 +326,9 @@ __device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, in
   const int b = (int)(item / ((long)nyb * H2));
   const int y0 = yb * YB;
   const int rs = H0 + 1;
-  const cx<T>* ch = static_cast<const cx<T>*>(B.ch);
-  const int n = p.n_ch * H0 * YB;
-  for (int i = threadIdx.x; i < n; i += kNT) {
-    const int yy = i % YB, x = (i / YB) % H0, c = i / (YB * H0);
-    lds[(c * YB + yy) * rs + x] =
-        ch[(((long)b * p.n_ch + c) * H0 + x) * ((long)H2 * H1) + (long)z * H1 + y0 + yy];
-  }
+  tile_to_lds<T, L0>(p, static_cast<const cx<T>*>(B.ch), b, z, y0, lds);
   __syncthreads();
-  lds_fft<T, TW, L0, +1>(lds, p.n_ch * YB, rs, 1, xch, static_cast<const cx<TW>*>(B.tw0));
+  lds_fft<T, TW, L0, +1>(lds, p.n_ch * YB, rs, 1, xch);
   const long M3 = (long)p.M[0] * p.M[1] * p.M[2];
   T* fb = static_cast<T*>(B.f0_out) + b * M3;
   const T* Eb = pipe_E<T>(p, b, step, M3);
@@ -311,13 +372,9 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
   const int y0 = yb * YB;
   const int rs = H0 + 1;
   cx<T>* ch = static_cast<cx<T>*>(B.ch);
-  const int n = p.n_ch * H0 * YB;
-  for (int i = threadIdx.x; i < n; i += kNT) {
-    const int yy = i % YB, x = (i / YB) % H0, c = i / (YB * H0);
-    lds[(c * YB + yy) * rs + x] = ch[(((long)b * p.n_ch + c) * H0 + x) * ((long)H2 * H1) + (long)z * H1 + y0 + yy];
-  }
+  tile_to_lds<T, L0>(p, ch, b, z, y0, lds);
   __syncthreads();
-  lds_fft<T, TW, L0, +1>(lds, p.n_ch * YB, rs, 1, xch, static_cast<const cx<TW>*>(B.tw0));
+  lds_fft<T, TW, L0, +1>(lds, p.n_ch * YB, rs, 1, xch);
   const long M3 = (long)p.M[0] * p.M[1] * p.M[2];
   T* fb = static_cast<T*>(B.f0_out) + b * M3;
   const T* Eo = pipe_E<T>(p, b, step, M3);
@@ -359,11 +416,8 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
     }
   }
   __syncthreads();
-  lds_fft<T, TW, L0, -1>(lds, p.n_ch * YB, rs, 1, xch, static_cast<const cx<TW>*>(B.tw0));
-  for (int i = threadIdx.x; i < n; i += kNT) {
-    const int yy = i % YB, x = (i / YB) % H0, c = i / (YB * H0);
-    ch[(((long)b * p.n_ch + c) * H0 + x) * ((long)H2 * H1) + (long)z * H1 + y0 + yy] = lds[(c * YB + yy) * rs + x];
-  }
+  lds_fft<T, TW, L0, -1>(lds, p.n_ch * YB, rs, 1, xch);
+  lds_to_tile<T, L0>(p, lds, ch, b, z, y0);
   __syncthreads();
 }
 

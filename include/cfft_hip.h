@@ -215,6 +215,11 @@ int cfft_sweep_hist_dev(cfft_plan* plan, void* hist, const void* const* E1, cons
 /* value != 0 (default 1): in multi-step calls, fuse the unfold of a step with
  * the fold of the next (one tile kernel, q(s + 1) is not read back). */
 #define CFFT_OPT_FUSE 2
+/* value != 0 (default 1): multi-step calls (cfft_step*, cfft_sweep*) capture
+ * their launch sequence into a hipGraph once per argument set (pointers,
+ * factors, step range) and replay it on the caller's stream; at most 16
+ * graphs are cached per plan.  Not used while profiling is enabled. */
+#define CFFT_OPT_GRAPH 3
 /* No counterpart in the reference. */
 int cfft_plan_set_option(cfft_plan* plan, int32_t key, int64_t value);
 

@@ -240,3 +240,29 @@ def test_full_size_im3m_128_properties():
     # the M7 and M2 routes agree on the config grid
     p2 = cf.DevicePlan(N, ops_s, DS, LAT4, cf.PATH_M2)
     assert rel(p.diffuse(f0), p2.diffuse(f0)) < TOL64
+
+
+@pytest.mark.parametrize("sg,N", [(229, (32, 32, 32)), (230, (32, 32, 32)), (70, (32, 32, 32))])
+def test_star_coefficients(sg, N):
+    """Star coefficients: c_0 = cell mean; encode/decode of all stars is the
+    identity; truncation to the first n stars equals the full-grid FFT with
+    every wave vector beyond those stars' |k|^2 shells zeroed."""
+    from cfft_hip import stars
+    ops_s, R, t, u, Lv = field(sg, N)
+    plan = cf.DevicePlan(N, ops_s, 0.0, LAT4, cf.PATH_AUTO)
+    f0 = O.extract_subgrid(u, Lv)
+    order, k2 = stars.star_order(plan)
+    c_all = stars.encode(plan, f0, plan.n_spec, order)
+    assert abs(c_all[0] - np.mean(u)) < 1e-12 * np.max(np.abs(u))
+    assert rel(stars.decode(plan, c_all, order), f0) < TOL64
+    # truncate at a shell boundary: keep all stars with |k|^2 <= k2[n-1]
+    n = int(np.searchsorted(k2, k2[min(40, len(k2) - 1)], side="right"))
+    approx = stars.decode(plan, stars.encode(plan, f0, n, order), order)
+    Fu = np.fft.fftn(u)
+    B = 2 * np.pi * np.linalg.inv(LAT4).T
+    kk = [np.fft.fftfreq(m, d=1.0 / m) for m in N]
+    KX, KY, KZ = np.meshgrid(*kk, indexing="ij")
+    kv = np.stack([KX, KY, KZ], axis=-1) @ B.T
+    K2 = np.sum(kv * kv, axis=-1)
+    ref = np.real(np.fft.ifftn(np.where(K2 <= k2[n - 1] + 1e-9, Fu, 0.0)))
+    assert rel(approx, O.extract_subgrid(ref, Lv)) < TOL64

