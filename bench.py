@@ -51,14 +51,20 @@ def stage_bytes(plan, e):
     M3 = int(np.prod(plan.M))
     chan = plan.n_ch * int(np.prod(plan.H)) * 2 * e
     spec = plan.n_spec * 2 * e
+    rep = plan.n_rep * 2 * e
+    qr = plan.path == 5  # orbit-reduced Q chain
     return {
         "p1_fold_fftx": B * (2 * M3 * e + chan),          # f0, E -> channels
         "fft_x": B * 2 * chan, "p2_fft_yz": B * 2 * chan, "fft_z": B * 2 * chan,
         "p3_recon": B * (chan + spec),                     # channels -> F (+K)
-        "p4_irecon_ifft_zy": B * (spec + chan),            # F -> channels
+        # F -> channels, or (QR) representative values -> channels
+        "p4_irecon_ifft_zy": B * ((rep if qr else spec) + chan),
         "ifft_x": B * 2 * chan, "ifft_y": B * 2 * chan, "ifft_z": B * 2 * chan,
         "p5_ifftx_unfold": B * (chan + 2 * M3 * e),        # channels, E -> f0
-        "qmul": B * 2 * chan,
+        # (QR) channels gathered at the representatives -> representative values
+        "qmul": B * ((min(chan, plan.n_rep * plan.d * 2 * e) + rep) if qr else 2 * chan),
+        # channels, E -> f0 (q(s+1)) -> channels
+        "p51_unfold_fold": B * (2 * chan + 2 * M3 * e),
     }
 
 
@@ -71,7 +77,7 @@ def launch_bytes(plan, e, stage, steps_per_launch):
 
 
 def step_bytes(plan, e):
-    """SURVEY-style algorithmic bytes of one Strang step per field:
+    """Algorithmic bytes of one Strang step per field:
     4 M^3 e + 8 n_ch H^3 2e + 2 n_spec 2e."""
     M3 = int(np.prod(plan.M))
     return 4 * M3 * e + 8 * plan.n_ch * int(np.prod(plan.H)) * 2 * e + 2 * plan.n_spec * 2 * e
@@ -262,7 +268,8 @@ def main():
     args = ap.parse_args()
     if args.gpus > 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+              "(a single field is not sharded; independent instances would be spread one "
+              "process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -278,19 +285,25 @@ def main():
     res = {}
     variants = {}
     for name, prec, e in (("fp64", cf.FP64, 8), ("mixed", cf.MIXED, 4)):
-        sw = cf.scft.PropagatorSweep(N, ops_s, lat, F_A, NS, prec)
-        sw.set_fields(wA, wB)
-        # launch strategy: one launch per phase vs one cooperative launch per
-        # sweep; the faster one is used for the timed region
+        # launch strategies: the centred fwd+K+bwd chain (one launch per phase,
+        # hipGraph replay) or one cooperative launch per sweep, and the
+        # orbit-reduced Q chain; the fastest one is used for the timed region
         trial = {}
-        for persist in (0, 1):
+        sweeps = {}
+        for label, path, persist in (("m7_chain", cf.PATH_AUTO, 0), ("m7_persistent", cf.PATH_AUTO, 1),
+                                     ("qr_chain", cf.PATH_QR, 0)):
+            sw = cf.scft.PropagatorSweep(N, ops_s, lat, F_A, NS, prec, path=path)
+            sw.set_fields(wA, wB)
             sw.plan.set_option(cf._lib.OPT_PERSISTENT, persist)
-            trial[persist] = time_sweeps(sw, min(args.steps, NS), 1, torch)
+            trial[label] = time_sweeps(sw, min(args.steps, NS), 1, torch)
+            sweeps[label] = sw
         best = min(trial, key=trial.get)
-        sw.plan.set_option(cf._lib.OPT_PERSISTENT, best)
-        variants[name] = {"per_phase_ms_per_step": trial[0] / min(args.steps, NS) * 1e3,
-                          "persistent_ms_per_step": trial[1] / min(args.steps, NS) * 1e3,
-                          "used": "persistent" if best else "per_phase"}
+        sw = sweeps[best]
+        for label in list(sweeps):
+            if label != best:
+                del sweeps[label]
+        variants[name] = {k + "_ms_per_step": v / min(args.steps, NS) * 1e3 for k, v in trial.items()}
+        variants[name]["used"] = best
         # full sweep once for Q
         sw.reset()
         sw.run()

@@ -28,7 +28,7 @@ CFFT_ERR_BOUNDS = 10
 CFFT_ERR_UNSUPPORTED = 11
 CFFT_ERR_HIP = 12
 
-PATH_AUTO, PATH_M2, PATH_M7, PATH_M2Q, PATH_M7Q = 0, 1, 2, 3, 4
+PATH_AUTO, PATH_M2, PATH_M7, PATH_M2Q, PATH_M7Q, PATH_QR = 0, 1, 2, 3, 4, 5
 FP64, MIXED = 0, 1
 
 EXPORTED = [

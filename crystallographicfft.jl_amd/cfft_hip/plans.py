@@ -221,7 +221,7 @@ class DevicePlan:
                                          len(self.ops), float(ds), lat.ctypes.data_as(L.PD),
                                          int(path), int(precision), int(batch), C.byref(h)))
         self._h = h
-        info = np.zeros(16, dtype=np.int32)
+        info = np.zeros(18, dtype=np.int32)
         L.check(L.lib().cfft_plan_info(self._h, L.i32p(info)))
         self.M = tuple(int(v) for v in info[0:3])
         self.L = tuple(int(v) for v in info[3:6])
@@ -233,6 +233,8 @@ class DevicePlan:
         self.precision = int(info[11])
         self.batch = int(info[12])
         self.H = tuple(int(v) for v in info[13:16])
+        self.n_rep = int(info[16])
+        self.n_alive = int(info[17])
         self.ds = float(ds)
 
     @property

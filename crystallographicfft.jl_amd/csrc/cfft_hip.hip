@@ -34,12 +34,14 @@ struct cfft_plan {
   std::vector<std::array<int, 3>> offs;
   int chmap[8];
   long H3 = 0, M3 = 0, n_alive = 0, N3 = 0;
+  long n_rep = 0;  // orbit representatives of the alive M-grid (path QR)
   size_t esz = 8;  // element size of the plan's real type
   hipStream_t stream = nullptr;
   // device tables
   DevBuf d_hs, d_K, d_W, d_twM, d_btab, d_Q, d_gidx, d_fmap, d_chmap;
   // device work buffers
   DevBuf d_f0, d_ch, d_ch2, d_F, d_E, d_u;
+  DevBuf d_qmap, d_grep;  // path QR: element -> (rep, phase); rep values
   std::vector<int32_t> fmap_host;
   // options (cfft_plan_set_option)
   int persistent = 0;  // contour sweeps as one cooperative launch
@@ -71,7 +73,10 @@ struct cfft_plan {
 
 namespace {
 
-bool is_q_path(const cfft_plan* p) { return p->path == CFFT_PATH_M2Q || p->path == CFFT_PATH_M7Q; }
+bool is_qr(const cfft_plan* p) { return p->path == CFFT_PATH_QR; }
+bool is_q_path(const cfft_plan* p) {
+  return p->path == CFFT_PATH_M2Q || p->path == CFFT_PATH_M7Q || p->path == CFFT_PATH_QR;
+}
 
 template <typename T>
 void upload_complex(DevBuf& buf, const std::vector<double>& re, const std::vector<double>& im) {
@@ -120,7 +125,7 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
                 const double* lattice, int path, int prec, int batch) {
   require(batch >= 1, CFFT_ERR_INVALID, "batch must be >= 1");
   require(prec == CFFT_FP64 || prec == CFFT_MIXED, CFFT_ERR_INVALID, "unknown precision");
-  require(path >= CFFT_PATH_AUTO && path <= CFFT_PATH_M7Q, CFFT_ERR_UNKNOWN_METHOD,
+  require(path >= CFFT_PATH_AUTO && path <= CFFT_PATH_QR, CFFT_ERR_UNKNOWN_METHOD,
           "Unknown method: path " + std::to_string(path));
   for (int d = 0; d < 3; ++d) require(N[d] >= 1, CFFT_ERR_INVALID, "grid size must be positive");
   p->ops = ops;
@@ -156,8 +161,9 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
     require(fft_supported(p->H[d]), CFFT_ERR_UNSUPPORTED,
             "FFT length " + std::to_string(p->H[d]) + " not supported (need 2^a 3^b <= 64)");
   }
-  p->offs = (path == CFFT_PATH_M7 || path == CFFT_PATH_M7Q) ? alive_offsets(p->cent)
-                                                             : alive_offsets(CENT_P);
+  const bool centred_fold = path == CFFT_PATH_M7 || path == CFFT_PATH_M7Q ||
+                            (path == CFFT_PATH_QR && p->cent != CENT_P && allL2);
+  p->offs = centred_fold ? alive_offsets(p->cent) : alive_offsets(CENT_P);
   p->n_ch = (int)p->offs.size();
   for (int i = 0; i < 8; ++i) p->chmap[i] = -1;
   for (int c = 0; c < p->n_ch; ++c) {
@@ -235,10 +241,65 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
     double det = A[0][0] * c[0][0] + A[0][1] * c[0][1] + A[0][2] * c[0][2];
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) B[i][j] = 2.0 * M_PI * (c[i][j] / det);
-    std::vector<double> qr((size_t)p->d * p->n_alive), qi((size_t)p->d * p->n_alive);
-    std::vector<int32_t> gidx((size_t)p->d * p->n_alive);
+    // path QR: orbits of the alive M-grid points under the ops that keep the
+    // stride-L subgrid (t = 0 mod L, s = t/L): G0(R^T q) = exp(2 pi i q.s/M) G0(q)
+    // (the orbit expansion of src/centering_fold.jl:630-731).  Q is evaluated at
+    // the representatives only; every element keeps (rep, phase exponent).
+    std::vector<long> reps;
+    std::vector<uint32_t> qmap;
+    if (is_qr(p)) {
+      std::vector<const Op*> rem;
+      for (const Op& g : ops) {
+        bool ok = true;
+        for (int dd = 0; dd < 3; ++dd) ok = ok && (((g.t[dd] % p->L[dd]) + p->L[dd]) % p->L[dd] == 0);
+        if (ok) rem.push_back(&g);
+      }
+      std::vector<int32_t> rep_of(p->n_alive, -1), jph(p->n_alive, 0);
+      auto elem_of = [&](const int* q) -> long {
+        const int c = p->chmap[(q[0] & 1) | ((q[1] & 1) << 1) | ((q[2] & 1) << 2)];
+        if (c < 0) return -1;
+        return c * p->H3 + ((q[0] >> 1) * 1L * p->H[2] + (q[2] >> 1)) * p->H[1] + (q[1] >> 1);
+      };
+      std::vector<long> queue;
+      for (long e = 0; e < p->n_alive; ++e) {
+        if (rep_of[e] >= 0) continue;
+        const int r = (int)reps.size();
+        reps.push_back(e);
+        rep_of[e] = r;
+        jph[e] = 0;
+        queue.assign(1, e);
+        for (size_t qi = 0; qi < queue.size(); ++qi) {
+          const long cur = queue[qi];
+          int q[3];
+          q_of(cur, q);
+          for (const Op* g : rem) {
+            int rq[3];
+            long long jj = 0;
+            for (int d1 = 0; d1 < 3; ++d1) {
+              long v = 0;
+              for (int d2 = 0; d2 < 3; ++d2) v += 1L * g->R[d2][d1] * q[d2];
+              rq[d1] = (int)(((v % p->M[d1]) + p->M[d1]) % p->M[d1]);
+              jj += 1LL * q[d1] * (g->t[d1] / p->L[d1]) * (P / p->M[d1]);
+            }
+            const long e2 = elem_of(rq);
+            require(e2 >= 0, CFFT_ERR_INVALID, "orbit leaves the alive parities");
+            if (rep_of[e2] >= 0) continue;
+            rep_of[e2] = r;
+            jph[e2] = (int32_t)((((jph[cur] + jj) % P) + P) % P);
+            queue.push_back(e2);
+          }
+        }
+      }
+      require((long)reps.size() < (1L << 20), CFFT_ERR_UNSUPPORTED, "too many orbit representatives");
+      qmap.resize(p->n_alive);
+      for (long e = 0; e < p->n_alive; ++e) qmap[e] = (uint32_t)rep_of[e] | ((uint32_t)jph[e] << 20);
+    }
+    const long n_q = is_qr(p) ? (long)reps.size() : p->n_alive;
+    std::vector<double> qr((size_t)p->d * n_q), qi((size_t)p->d * n_q);
+    std::vector<int32_t> gidx((size_t)p->d * n_q);
     std::vector<double> Kh(p->d);
-    for (long e = 0; e < p->n_alive; ++e) {
+    for (long ei = 0; ei < n_q; ++ei) {
+      const long e = is_qr(p) ? reps[ei] : ei;
       int q[3];
       q_of(e, q);
       for (int a = 0; a < p->d; ++a) {
@@ -264,8 +325,8 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
           sr += Kh[a] * cr;
           si += Kh[a] * ci;
         }
-        qr[(size_t)b * p->n_alive + e] = sr / p->d;
-        qi[(size_t)b * p->n_alive + e] = si / p->d;
+        qr[(size_t)b * n_q + ei] = sr / p->d;
+        qi[(size_t)b * n_q + ei] = si / p->d;
         int rq[3];
         for (int d1 = 0; d1 < 3; ++d1) {
           long v = 0;
@@ -273,22 +334,27 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
           rq[d1] = (int)(((v % p->M[d1]) + p->M[d1]) % p->M[d1]);
         }
         int cc = p->chmap[(rq[0] & 1) | ((rq[1] & 1) << 1) | ((rq[2] & 1) << 2)];
-        gidx[(size_t)b * p->n_alive + e] =
+        gidx[(size_t)b * n_q + ei] =
             cc < 0 ? -1
                    : (int32_t)(cc * p->H3 + ((rq[0] >> 1) * 1L * p->H[2] + (rq[2] >> 1)) * p->H[1] + (rq[1] >> 1));
       }
     }
-    if (prec == CFFT_FP64)
-      upload_complex<double>(p->d_Q, qr, qi);
+    if (prec == CFFT_FP64 || is_qr(p))
+      upload_complex<double>(p->d_Q, qr, qi);  // QR: FP64 coefficients in both precisions
     else
       upload_complex<float>(p->d_Q, qr, qi);
     upload_bytes(p->d_gidx, gidx.data(), gidx.size() * 4);
+    if (is_qr(p)) {
+      p->n_rep = (long)reps.size();
+      upload_bytes(p->d_qmap, qmap.data(), qmap.size() * 4);
+      p->d_grep.alloc(2 * p->esz * (size_t)p->n_rep * batch);
+    }
   }
 
   const size_t es = p->esz;
   p->d_f0.alloc(es * p->M3 * batch);
   p->d_ch.alloc(2 * es * p->n_alive * batch);
-  if (is_q_path(p)) p->d_ch2.alloc(2 * es * p->n_alive * batch);
+  if (is_q_path(p) && !is_qr(p)) p->d_ch2.alloc(2 * es * p->n_alive * batch);
   p->d_F.alloc(2 * es * (size_t)std::max(p->n_spec, 1) * batch);
   p->d_E.alloc(es * p->M3 * batch);
 }
@@ -421,6 +487,7 @@ PipeP pipe_params(cfft_plan* p, const ESel& es) {
   }
   pp.scale_unfold = 1.0 / (8.0 * (double)p->H3);
   pp.scale_irecon = 1.0 / p->d;
+  pp.n_rep = p->n_rep;
   return pp;
 }
 
@@ -435,6 +502,8 @@ PipeBufs pipe_bufs(cfft_plan* p, const void* f0_in, void* f0_out, void* F, bool 
   B.W = p->d_W.p;
   B.twM = p->d_twM.p;
   B.btab = static_cast<const uint32_t*>(p->d_btab.p);
+  B.grep = p->d_grep.p;
+  B.qmap = static_cast<const uint32_t*>(p->d_qmap.p);
   return B;
 }
 
@@ -516,6 +585,40 @@ void launch_p5(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipSt
   launch_check<T>("k_p5");
 }
 
+// Path QR middle of a step: Q at the representatives, orbit expansion and
+// IFFT z, IFFT y (one plane kernel on cubic H-grids).
+template <typename T>
+void launch_qr(cfft_plan* p, const PipeP& pp, const PipeBufs& B, hipStream_t s) {
+  {
+    StageTimer st(p, s, CFFT_STAGE_QMUL);
+    hipLaunchKernelGGL(k_qrep<T>, dim3(nblk(p->n_rep * p->batch, 256)), dim3(256), 0, s,
+                       static_cast<const cx<T>*>(B.ch), static_cast<cx<T>*>(p->d_grep.p),
+                       static_cast<const cx<double>*>(p->d_Q.p), static_cast<const int*>(p->d_gidx.p), p->d,
+                       p->n_rep, p->n_alive, p->batch);
+    launch_check<T>("k_qrep");
+  }
+  if (!cubic(p)) {
+    {
+      StageTimer st(p, s, CFFT_STAGE_IRECON);
+      hipLaunchKernelGGL(k_expand<T>, dim3(nblk(p->n_alive * p->batch, 256)), dim3(256), 0, s,
+                         static_cast<const cx<T>*>(p->d_grep.p), static_cast<cx<T>*>(B.ch),
+                         static_cast<const uint32_t*>(p->d_qmap.p), static_cast<const cx<double>*>(p->d_W.p),
+                         p->n_rep, p->n_alive, p->batch);
+      launch_check<T>("k_expand");
+    }
+    fft_axis<T, +1>(p, static_cast<cx<T>*>(B.ch), 2, s);
+    fft_axis<T, +1>(p, static_cast<cx<T>*>(B.ch), 1, s);
+    return;
+  }
+  StageTimer st(p, s, CFFT_STAGE_IRECON);
+  const unsigned g = (unsigned)n_planes(p);
+#define CFFT_P4R(L) \
+  case L: hipLaunchKernelGGL((k_p4r<T, double, L, L>), dim3(g), dim3(kNT), 0, s, pp, B); break;
+  CFFT_LEN_SWITCH(p->H[1], CFFT_P4R)
+#undef CFFT_P4R
+  launch_check<T>("k_p4r");
+}
+
 template <typename T>
 void launch_p51(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipStream_t s) {
   StageTimer st(p, s, CFFT_STAGE_UNFOLD_FOLD);
@@ -553,6 +656,13 @@ void run_backward(cfft_plan* p, const PipeP& pp, const cx<T>* F, T* f0, int step
 // of `pp` on both sides: out <- E D (E in).  `in` may equal `out`.
 template <typename T>
 void run_diffuse(cfft_plan* p, const T* in, T* out_f0, const PipeP& pp, int step, hipStream_t s) {
+  if (is_qr(p)) {
+    run_forward_channels<T>(p, in, pp, step, s);
+    PipeBufs B = pipe_bufs(p, nullptr, out_f0, nullptr, false);
+    launch_qr<T>(p, pp, B, s);
+    launch_p5<T>(p, pp, B, step, s);
+    return;
+  }
   if (is_q_path(p)) {
     run_forward_channels<T>(p, in, pp, step, s);
     cx<T>* out = static_cast<cx<T>*>(p->d_ch2.p);
@@ -695,9 +805,13 @@ void run_chain(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int 
   cx<T>* F = static_cast<cx<T>*>(p->d_F.p);
   run_forward_channels<T>(p, q, pp, step0, s);
   for (int i = 0; i < nsteps; ++i) {
-    run_recon<T>(p, pp, F, true, s);
     PipeBufs B = pipe_bufs(p, nullptr, q + (i + 1) * stride, F, false);
-    launch_p4<T>(p, pp, B, s);
+    if (is_qr(p)) {
+      launch_qr<T>(p, pp, B, s);
+    } else {
+      run_recon<T>(p, pp, F, true, s);
+      launch_p4<T>(p, pp, B, s);
+    }
     if (i + 1 < nsteps && p->fuse) {
       launch_p51<T>(p, pp, B, step0 + i, s);
       launch_p2<T>(p, pp, B, s);
@@ -712,7 +826,7 @@ void run_chain(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int 
 // the Q paths).
 template <typename T>
 void launch_steps(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int nsteps, hipStream_t s) {
-  if (!is_q_path(p)) {
+  if (!is_q_path(p) || is_qr(p)) {
     run_chain<T>(p, q, stride, pp, step0, nsteps, s);
     return;
   }
@@ -932,8 +1046,9 @@ int cfft_plan_info(const cfft_plan* p, int32_t* info) {
   return guarded([&] {
     check_plan(p);
     require(info != nullptr, CFFT_ERR_INVALID, "null argument");
-    int32_t v[16] = {p->M[0], p->M[1], p->M[2], p->L[0], p->L[1], p->L[2], p->n_spec, p->n_ch,
-                     p->cent, p->d,    p->path, p->prec, p->batch, p->H[0], p->H[1], p->H[2]};
+    int32_t v[18] = {p->M[0], p->M[1], p->M[2], p->L[0], p->L[1], p->L[2], p->n_spec, p->n_ch,
+                     p->cent, p->d,    p->path, p->prec, p->batch, p->H[0], p->H[1], p->H[2],
+                     (int32_t)p->n_rep, (int32_t)p->n_alive};
     std::memcpy(info, v, sizeof(v));
   });
 }
@@ -956,7 +1071,7 @@ int cfft_plan_tables(const cfft_plan* p, int32_t* idx_out, int32_t* mult_out, do
 int cfft_update_kernel(cfft_plan* p, double ds, const double* lattice) {
   return guarded([&] {
     check_plan(p);
-    require(!(p->path == CFFT_PATH_M2Q || p->path == CFFT_PATH_M7Q), CFFT_ERR_INVALID,
+    require(!is_q_path(p), CFFT_ERR_INVALID,
             "update_kernel applies to the fwd+K+bwd paths; replan the Q-fused paths");
     const double* lat = lattice ? lattice : p->lattice;
     p->K = kernel_spec(p->spec, ds, lat);

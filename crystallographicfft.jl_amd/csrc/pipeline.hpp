@@ -38,6 +38,7 @@ struct PipeP {
   int sw[kMaxEPtr];
   double scale_unfold;  // 1 / (8 H0 H1 H2)
   double scale_irecon;  // 1 / d
+  long n_rep;           // orbit representatives (path QR)
 };
 
 struct PipeBufs {
@@ -50,6 +51,8 @@ struct PipeBufs {
   const void* W;      // exp(2 pi i j / P)
   const void* twM;    // fold twiddles (M0 + M1 + M2)
   const uint32_t* btab;  // inverse reconstruction table [e][8]
+  const void* grep;      // path QR: Q-multiplied values at the orbit representatives
+  const uint32_t* qmap;  // path QR: element -> rep (bits 0-19) | phase exponent << 20
 };
 
 template <int LEN>
@@ -300,6 +303,35 @@ __device__ void ph_irecon_ifft_zy(const PipeP& p, const PipeBufs& B, long item, 
   __syncthreads();
 }
 
+// Orbit expansion G0(e) = exp(2 pi i j_e / P) G0'(rep_e), then IFFT z, IFFT y
+// of one (b, c, x) plane (path QR).
+template <typename T, typename TW, int L1, int L2>
+__device__ void ph_expand_ifft_zy(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds, cx<T>* xch) {
+  constexpr int rs = L1 + 1, NP = L1 * L2, NIT = (NP + kNT - 1) / kNT;
+  const long n_alive = (long)p.n_ch * p.H[0] * NP;
+  const int b = (int)(item / ((long)p.n_ch * p.H[0]));
+  const long e0 = item * (long)NP - (long)b * n_alive;
+  const cx<T>* G = static_cast<const cx<T>*>(B.grep) + (long)b * p.n_rep;
+  const cx<TW>* W = static_cast<const cx<TW>*>(B.W);
+  uint32_t ent[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int i = threadIdx.x + it * kNT;
+    if (i < NP) ent[it] = B.qmap[e0 + i];
+  }
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int i = threadIdx.x + it * kNT;
+    if (i < NP)
+      lds[(i / L1) * rs + (i % L1)] = twmul<T, TW, -1>(G[ent[it] & 0xFFFFFu], W[ent[it] >> 20]);
+  }
+  __syncthreads();
+  lds_fft<T, TW, L2, +1>(lds, L1, 1, rs, xch);
+  lds_fft<T, TW, L1, +1>(lds, L2, rs, 1, xch);
+  lds_to_plane<T, L1, L2>(lds, static_cast<cx<T>*>(B.ch) + item * (long)NP);
+  __syncthreads();
+}
+
 // IFFT z, IFFT y of one plane of `src` written to `dst` (Q-fused path).
 template <typename T, typename TW, int L1, int L2>
 __device__ void ph_ifft_zy(const PipeP& p, const PipeBufs& B, const cx<T>* src, long item,
@@ -464,6 +496,50 @@ __global__ void __launch_bounds__(kNT) k_p5(PipeP p, PipeBufs B, int step) {
   __shared__ cx<T> lds[64 * (L0 + 1)];
   __shared__ cx<T> xch[PipeG<L0>::XCH];
   ph_ifftx_unfold<T, TW, L0>(p, B, blockIdx.x, step, lds, xch);
+}
+
+template <typename T, typename TW, int L1, int L2>
+__global__ void __launch_bounds__(kNT) k_p4r(PipeP p, PipeBufs B) {
+  constexpr int X = PipeG<L1>::XCH > PipeG<L2>::XCH ? PipeG<L1>::XCH : PipeG<L2>::XCH;
+  __shared__ cx<T> lds[L2 * (L1 + 1)];
+  __shared__ cx<T> xch[X];
+  ph_expand_ifft_zy<T, TW, L1, L2>(p, B, blockIdx.x, lds, xch);
+}
+
+// Q multiply at the orbit representatives (path QR):
+// G0'(rep) = sum_b Q_b(rep) G0(R_b^T rep), coefficients in FP64.
+template <typename T>
+__global__ void __launch_bounds__(256) k_qrep(const cx<T>* __restrict__ ch, cx<T>* __restrict__ grep,
+                                              const cx<double>* __restrict__ Q, const int* __restrict__ gidx,
+                                              int d, long n_rep, long n_alive, int batch) {
+  const long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (tid >= n_rep * batch) return;
+  const int b = (int)(tid / n_rep);
+  const long r = tid - (long)b * n_rep;
+  const cx<T>* cb = ch + (long)b * n_alive;
+  double ax = 0.0, ay = 0.0;
+  for (int a = 0; a < d; ++a) {
+    const int g = gidx[a * n_rep + r];
+    if (g < 0) continue;
+    const cx<double> q = Q[a * n_rep + r];
+    const cx<T> v = cb[g];
+    ax += q.x * (double)v.x - q.y * (double)v.y;
+    ay += q.x * (double)v.y + q.y * (double)v.x;
+  }
+  grep[tid] = {(T)ax, (T)ay};
+}
+
+// Orbit expansion on the channel array (non-cubic path QR).
+template <typename T>
+__global__ void __launch_bounds__(256) k_expand(const cx<T>* __restrict__ grep, cx<T>* __restrict__ ch,
+                                                const uint32_t* __restrict__ qmap,
+                                                const cx<double>* __restrict__ W, long n_rep, long n_alive,
+                                                int batch) {
+  const long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (tid >= n_alive * batch) return;
+  const int b = (int)(tid / n_alive);
+  const uint32_t ent = qmap[tid - (long)b * n_alive];
+  ch[tid] = twmul<T, double, -1>(grep[(long)b * n_rep + (ent & 0xFFFFFu)], W[ent >> 20]);
 }
 
 template <typename T, typename TW, int L0>

@@ -356,7 +356,7 @@ int cfft_scft_create(cfft_plan* plan, double f, double chiN, int32_t ns, int32_t
     require(ns >= 2 && f > 0.0 && f < 1.0, CFFT_ERR_INVALID, "need ns >= 2 and 0 < f < 1");
     require(m_hist >= 0 && m_hist < kMaxHist, CFFT_ERR_INVALID, "Anderson history must be < 32");
     *out = nullptr;
-    int32_t info[16];
+    int32_t info[18];
     int rc = cfft_plan_info(plan, info);
     if (rc != CFFT_OK) throw Error{rc, g_last_error};
     require(info[12] == 2, CFFT_ERR_INVALID, "the SCFT plan must have batch = 2 (q and q+)");

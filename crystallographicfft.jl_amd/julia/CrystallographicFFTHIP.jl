@@ -60,7 +60,7 @@ function DevicePlan(N::NTuple{3,Int}, ops_shifted, ds::Float64, lattice::Abstrac
                  Ptr{Ptr{Cvoid}}),
                 Int32[N...], pack_ops(ops_shifted), Int32(length(ops_shifted)), ds, lat,
                 Int32(path), Int32(precision), Int32(batch), h))
-    info = zeros(Int32, 16)
+    info = zeros(Int32, 18)
     check(ccall((:cfft_plan_info, libcfft), Cint, (Ptr{Cvoid}, Ptr{Int32}), h[], info))
     p = DevicePlan(h[], (info[1], info[2], info[3]), (info[4], info[5], info[6]),
                    Int(info[7]), Int(info[13]))

@@ -55,6 +55,10 @@ extern "C" {
 #define CFFT_PATH_M7 2   /* centred fwd + K + bwd, plan_centered_scft          */
 #define CFFT_PATH_M2Q 3  /* Q-fused multiply on the M-grid, plan_m2_q          */
 #define CFFT_PATH_M7Q 4  /* centring fold + Q multiply, plan_m7_scft           */
+#define CFFT_PATH_QR 5   /* fold + Q multiply at the M-grid orbit representatives
+                          * under the subgrid-preserving ops, then the orbit
+                          * expansion of src/centering_fold.jl:630-731; diffusion
+                          * and steps only (no F_spec output)                   */
 
 #define CFFT_FP64 0  /* all arithmetic FP64                                   */
 #define CFFT_MIXED 1 /* FP32 data/FFT/propagator, FP64 tables and reductions  */
@@ -109,8 +113,9 @@ int cfft_plan_create(const int32_t* N, const int32_t* ops, int32_t nops, double 
                      const double* lattice, int32_t path, int32_t precision, int32_t batch,
                      cfft_plan** out);
 
-/* Plan metadata: info[16] = {M0, M1, M2, L0, L1, L2, n_spec, n_ch, centering,
- * d, path, precision, batch, H0, H1, H2}. */
+/* Plan metadata: info[18] = {M0, M1, M2, L0, L1, L2, n_spec, n_ch, centering,
+ * d, path, precision, batch, H0, H1, H2, n_rep (path QR orbit representatives,
+ * else 0), n_alive (channel elements per field)}. */
 int cfft_plan_info(const cfft_plan* plan, int32_t* info);
 
 /* Spectral-ASU points of the plan: idx_out (n_spec x 3), mult_out (n_spec);

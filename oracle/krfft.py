@@ -39,7 +39,7 @@ __all__ = [
     "disassemble_G0", "centered_krfft_forward", "CenteredBackward",
     "centered_backward_plan", "centered_backward", "centered_scft",
     "m2q_plan", "m2q_execute", "m7_scft", "fill_map", "subgrid_to_fullgrid",
-    "fullgrid_to_subgrid", "centering_prefold", "prefold_merge",
+    "fullgrid_to_subgrid", "centering_prefold", "prefold_merge", "subgrid_orbits", "qr_scft",
     "strang_step",
 ]
 
@@ -816,6 +816,57 @@ def m7_scft(f0, plan, cent):
     G0 = assemble_G0(fft_channels(centering_fold(f0, cent)), cent, M).reshape(-1, order="F")
     Gn = np.sum(plan["Q"] * G0[plan["gather"]], axis=0).reshape(M, order="F")
     return centering_unfold(ifft_channels(disassemble_G0(Gn, cent)), cent, M)
+
+
+def subgrid_orbits(R, t, N, L):
+    """Orbits of the M-grid under the ops that keep the stride-L subgrid
+    (t = 0 mod L, s = t/L): G0(R^T q) = exp(2 pi i q.s/M) G0(q) for a
+    symmetric subgrid field (the orbit expansion of
+    src/centering_fold.jl:630-731).  Returns (rep (Mvol,), phase (Mvol,),
+    reps): every q's representative (linear index) and the phase with
+    G0(q) = phase * G0(rep)."""
+    N = tuple(N)
+    M = tuple(int(N[d] // L[d]) for d in range(3))
+    Mv = np.asarray(M, dtype=np.int64)
+    Mvol = int(np.prod(M))
+    rem = [(R[i], t[i] // np.asarray(L)) for i in range(R.shape[0]) if np.all(t[i] % np.asarray(L) == 0)]
+    rep = np.full(Mvol, -1, dtype=np.int64)
+    phase = np.zeros(Mvol, dtype=np.complex128)
+    reps = []
+    for m in range(Mvol):
+        if rep[m] >= 0:
+            continue
+        reps.append(m)
+        rep[m] = m
+        phase[m] = 1.0
+        queue = [m]
+        qi = 0
+        while qi < len(queue):
+            cur = queue[qi]
+            qi += 1
+            qv = np.array([cur % M[0], (cur // M[0]) % M[1], cur // (M[0] * M[1])], dtype=np.int64)
+            for Rg, s_ in rem:
+                rq = np.mod(Rg.T @ qv, Mv)
+                rl = int(rq[0] + M[0] * (rq[1] + M[1] * rq[2]))
+                if rep[rl] < 0:
+                    rep[rl] = m
+                    phase[rl] = phase[cur] * np.exp(2j * np.pi * float(np.sum(qv * s_ / Mv)))
+                    queue.append(rl)
+    return rep, phase, np.array(reps, dtype=np.int64)
+
+
+def qr_scft(f0, qplan, orbits):
+    """Q multiply evaluated at the orbit representatives only, then expanded
+    with the orbit phases (the device path QR); equals m2q_execute for a
+    symmetric f0."""
+    M = f0.shape
+    rep, phase, reps = orbits
+    Y = np.fft.fftn(f0.astype(np.complex128)).reshape(-1, order="F")
+    Yr = np.sum(qplan["Q"][:, reps] * Y[qplan["gather"][:, reps]], axis=0)
+    full = np.zeros(Y.size, dtype=np.complex128)
+    full[reps] = Yr
+    Yn = (phase * full[rep]).reshape(M, order="F")
+    return np.real(np.fft.ifftn(Yn))
 
 
 def fill_map(R, t, L, N):

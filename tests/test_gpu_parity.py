@@ -81,6 +81,38 @@ def test_q_fused_fp64(sg, path):
     assert rel(plan.diffuse(f0), O.fullgrid_reference(u, N, DS, LAT4, Lv)) < 1e-11
 
 
+@pytest.mark.parametrize("sg,N", [(229, (16, 16, 16)), (229, (32, 32, 32)), (230, (24, 24, 24)),
+                                  (70, (16, 16, 16)), (47, (16, 16, 16)), (221, (24, 24, 24)),
+                                  (225, (32, 32, 32)), (10, (16, 16, 16)), (71, (24, 32, 48))])
+def test_orbit_reduced_q_fp64(sg, N):
+    """Path QR (Q multiply at the subgrid orbit representatives + orbit
+    expansion) = the oracle's restatement = full-grid FFT.K.IFFT; Strang steps
+    (fused chain) = the oracle's steps; mixed precision within 2e-5."""
+    print(f"QR sg={sg} N={N}", flush=True)
+    ops_s, R, t, u, Lv = field(sg, N, seed=4)
+    f0 = O.extract_subgrid(u, Lv)
+    plan = cf.DevicePlan(N, ops_s, DS, LAT4, cf.PATH_QR)
+    qp = O.m2q_plan(R, t, N, DS, LAT4)
+    orb = O.subgrid_orbits(R, t, N, Lv)
+    d = plan.diffuse(f0)
+    assert rel(d, O.qr_scft(f0, qp, orb)) < TOL64
+    assert rel(d, O.fullgrid_reference(u, N, DS, LAT4, Lv)) < TOL64
+    w = 2.0 * O.extract_subgrid(O.make_symmetric(R, t, N, seed=8), Lv)
+    q = f0 + 1.0
+    out = plan.step(q, w, DS, nsteps=4)
+    ref = q.copy()
+    for _ in range(4):
+        ref = O.strang_step(ref, w, DS, lambda x: O.qr_scft(x, qp, orb))
+    assert rel(out, ref) < TOL64
+    p32 = cf.DevicePlan(N, ops_s, DS, LAT4, cf.PATH_QR, cf.MIXED)
+    assert rel(p32.diffuse(f0), d) < TOL32
+    spec = O.calc_spectral_asu(R, t, N)
+    F = plan.forward(f0)  # the forward transform is shared with the other paths
+    assert rel(F, np.fft.fftn(u)[tuple(spec.idx[:, k] for k in range(3))]) < TOL64
+    with pytest.raises(cf.CFFTError):
+        plan.backward(F)
+
+
 @pytest.mark.parametrize("sg,N", [(229, (32, 32, 32)), (230, (32, 32, 32)), (70, (32, 32, 32)),
                                   (47, (32, 32, 32)), (223, (48, 48, 48)), (229, (48, 48, 48)),
                                   (47, (24, 24, 24)), (230, (96, 96, 96))])

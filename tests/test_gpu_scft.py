@@ -190,7 +190,7 @@ def test_graph_replays_match_eager(prec):
         got = sw.fields()
         for a, b in zip(got, eager):
             assert np.array_equal(a, b), f"replay {rep} differs"
-        sw.plan.forward(f0)  # eager work on the same plan between replays
+        sw.plan.forward(np.stack([f0, f0]))  # eager work on the same plan between replays
     # split runs (two graphs, different step ranges) equal the whole sweep
     sw.reset()
     sw.run(12, start=0)
