@@ -27,11 +27,12 @@ _lib = None
 
 def build(verbose=False):
     """gcc -O3 (x86-64-v3, OpenMP) -> oracle/libcstep.so."""
-    cmd = ["gcc", "-O3", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-shared", "-o", LIB + ".tmp", SRC, "-lm"]
+    tmp = f"{LIB}.tmp{os.getpid()}"
+    cmd = ["gcc", "-O3", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-shared", "-o", tmp, SRC, "-lm"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(LIB + ".tmp", LIB)
+    os.replace(tmp, LIB)
     return LIB
 
 
