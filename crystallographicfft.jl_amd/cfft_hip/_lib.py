@@ -45,7 +45,9 @@ EXPORTED = [
     "cfft_prefold_plan_create", "cfft_prefold_plan_info", "cfft_prefold_forward",
     "cfft_prefold_plan_destroy", "cfft_sweep_hist_dev", "cfft_scft_create", "cfft_scft_set_mixing",
     "cfft_scft_set_fields", "cfft_scft_iterate", "cfft_scft_solve", "cfft_scft_get",
-    "cfft_scft_propagators", "cfft_scft_destroy",
+    "cfft_scft_propagators", "cfft_scft_destroy", "cfft_mlp_create", "cfft_mlp_info",
+    "cfft_mlp_get_params", "cfft_mlp_set_params", "cfft_mlp_forward", "cfft_mlp_gradient",
+    "cfft_mlp_train", "cfft_mlp_destroy",
 ]
 
 SCFT_W, SCFT_PHI, SCFT_DEV = 0, 1, 2
@@ -79,6 +81,7 @@ D = C.c_double
 PI32 = C.POINTER(C.c_int32)
 PI64 = C.POINTER(C.c_int64)
 PD = C.POINTER(C.c_double)
+PF = C.POINTER(C.c_float)
 
 
 def _declare(lib):
@@ -134,6 +137,14 @@ def _declare(lib):
         "cfft_scft_get": (C.c_int, [P, I32, PD]),
         "cfft_scft_propagators": (C.c_int, [P, C.POINTER(P)]),
         "cfft_scft_destroy": (C.c_int, [P]),
+        "cfft_mlp_create": (C.c_int, [I32, PI32, C.c_uint64, C.POINTER(P)]),
+        "cfft_mlp_info": (C.c_int, [P, PI64]),
+        "cfft_mlp_get_params": (C.c_int, [P, PF]),
+        "cfft_mlp_set_params": (C.c_int, [P, PF]),
+        "cfft_mlp_forward": (C.c_int, [P, PF, I32, PF]),
+        "cfft_mlp_gradient": (C.c_int, [P, PF, PF, I32, PD, PF]),
+        "cfft_mlp_train": (C.c_int, [P, PF, PF, I64, I32, I32, C.c_float, C.c_uint64, PD]),
+        "cfft_mlp_destroy": (C.c_int, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

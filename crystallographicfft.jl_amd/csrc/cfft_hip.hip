@@ -144,6 +144,7 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
   }
   p->rep = select_rep_ops(3, ops, p->L, N);
   p->d = (int)p->rep.size();
+  require(p->d <= 8, CFFT_ERR_UNSUPPORTED, "more than 8 subgrid cosets (L beyond 2 per axis)");
   p->cent = detect_centering(3, N, ops);
   bool allL2 = p->L[0] == 2 && p->L[1] == 2 && p->L[2] == 2;
   if (path == CFFT_PATH_AUTO) path = (p->cent != CENT_P && allL2) ? CFFT_PATH_M7 : CFFT_PATH_M2;

@@ -104,15 +104,17 @@ template <typename T, int L1, int L2>
 __device__ __forceinline__ void plane_to_lds(const cx<T>* __restrict__ src, cx<T>* lds) {
   constexpr int NP = L1 * L2, NIT = (NP + kNT - 1) / kNT;
   cx<T> r[NIT];
+  // loads are unconditional (a clamped index on the ragged last pass) so that
+  // none of them waits behind a branch
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
     const int i = threadIdx.x + it * kNT;
-    if (i < NP) r[it] = src[i];
+    r[it] = src[(NP % kNT == 0 || i < NP) ? i : NP - 1];
   }
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
     const int i = threadIdx.x + it * kNT;
-    if (i < NP) lds[(i / L1) * (L1 + 1) + i % L1] = r[it];
+    if (NP % kNT == 0 || i < NP) lds[(i / L1) * (L1 + 1) + i % L1] = r[it];
   }
 }
 
@@ -122,7 +124,7 @@ __device__ __forceinline__ void lds_to_plane(const cx<T>* lds, cx<T>* __restrict
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
     const int i = threadIdx.x + it * kNT;
-    if (i < NP) dst[i] = lds[(i / L1) * (L1 + 1) + i % L1];
+    if (NP % kNT == 0 || i < NP) dst[i] = lds[(i / L1) * (L1 + 1) + i % L1];
   }
 }
 
@@ -137,11 +139,10 @@ __device__ __forceinline__ void tile_to_lds(const PipeP& p, const cx<T>* __restr
   cx<T> r[MAXIT];
 #pragma unroll
   for (int it = 0; it < MAXIT; ++it) {
-    const int i = threadIdx.x + it * kNT;
-    if (i < n) {
-      const int yy = i % YB, x = (i / YB) % L0, c = i / (YB * L0);
-      r[it] = ch[(((long)b * p.n_ch + c) * L0 + x) * HH + (long)z * p.H[1] + y0 + yy];
-    }
+    const int i0 = threadIdx.x + it * kNT;
+    const int i = i0 < n ? i0 : n - 1;  // unconditional load, clamped index
+    const int yy = i % YB, x = (i / YB) % L0, c = i / (YB * L0);
+    r[it] = ch[(((long)b * p.n_ch + c) * L0 + x) * HH + (long)z * p.H[1] + y0 + yy];
   }
 #pragma unroll
   for (int it = 0; it < MAXIT; ++it) {
@@ -169,6 +170,37 @@ __device__ __forceinline__ void lds_to_tile(const PipeP& p, const cx<T>* lds, cx
   }
 }
 
+// Per-thread tile points (x, yy) of a (b, z, y-block) tile: NPT covers
+// L0 * YB <= L0 * 8 points.  The interaction factors and the three fold
+// twiddle factors of every point are loaded up front, so their latency
+// overlaps the tile loads instead of following them.
+template <int L0>
+struct TilePts {
+  static constexpr int NPT = (L0 * 8 + kNT - 1) / kNT;
+};
+
+template <typename TW>
+struct FoldTw {
+  cx<TW> x, y, z;
+};
+
+// exp(-2 pi i off.(x,y,z)/M) from the prefetched axis factors (same products
+// as fold_twiddle).
+template <typename TW>
+__device__ __forceinline__ cx<TW> fold_tw(int o, const FoldTw<TW>& f) {
+  cx<TW> t = {TW(1), TW(0)};
+  if (o & 1) t = f.x;
+  if (o & 2) t = cmul(t, f.y);
+  if (o & 4) t = cmul(t, f.z);
+  return t;
+}
+
+template <typename TW>
+__device__ __forceinline__ FoldTw<TW> load_fold_tw(const cx<TW>* __restrict__ twM, int x, int y, int z,
+                                                   const PipeP& p) {
+  return {twM[x], twM[p.M[0] + y], twM[p.M[0] + p.M[1] + z]};
+}
+
 // ---- P1: fold + E + FFT x --------------------------------------------------
 // tile item = (b, z, y-block); LDS A[c][yy][x] with row stride H0 + 1.
 template <typename T, typename TW, int L0>
@@ -185,22 +217,34 @@ __device__ void ph_fold_fftx(const PipeP& p, const PipeBufs& B, long item, int s
   const T* Eb = pipe_E<T>(p, b, step, M3);
   const cx<TW>* twM = static_cast<const cx<TW>*>(B.twM);
   const int rs = H0 + 1;
-  for (int pt = threadIdx.x; pt < H0 * YB; pt += kNT) {
-    const int x = pt % H0, yy = pt / H0, y = y0 + yy;
-    T v[8];
+  constexpr int NPT = TilePts<L0>::NPT;
+  T v[NPT][8];
+  FoldTw<TW> ft[NPT];
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int pt0 = threadIdx.x + k * kNT;
+    const int pt = pt0 < H0 * YB ? pt0 : H0 * YB - 1;  // unconditional loads
+    const int x = pt % H0, y = y0 + pt / H0;
+    ft[k] = load_fold_tw(twM, x, y, z, p);
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const long idx = corner_index(p, x, y, z, e);
-      v[e] = Eb ? fb[idx] * Eb[idx] : fb[idx];
+      v[k][e] = Eb ? fb[idx] * Eb[idx] : fb[idx];
     }
-    wht8(v);
+  }
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int pt = threadIdx.x + k * kNT;
+    if (pt >= H0 * YB) continue;
+    const int x = pt % H0, yy = pt / H0;
+    wht8(v[k]);
     for (int c = 0; c < p.n_ch; ++c) {
       const int o = p.offs[c];
       T val = T(0);
 #pragma unroll
       for (int e = 0; e < 8; ++e)
-        if (e == o) val = v[e];
-      const cx<TW> tw = fold_twiddle(o, twM, x, y, z, p.M[0], p.M[1]);
+        if (e == o) val = v[k][e];
+      const cx<TW> tw = fold_tw(o, ft[k]);
       lds[(c * YB + yy) * rs + x] = {(T)((TW)val * tw.x), (T)((TW)val * tw.y)};
     }
   }
@@ -358,21 +402,36 @@ __device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, in
   const int b = (int)(item / ((long)nyb * H2));
   const int y0 = yb * YB;
   const int rs = H0 + 1;
-  tile_to_lds<T, L0>(p, static_cast<const cx<T>*>(B.ch), b, z, y0, lds);
-  __syncthreads();
-  lds_fft<T, TW, L0, +1>(lds, p.n_ch * YB, rs, 1, xch);
   const long M3 = (long)p.M[0] * p.M[1] * p.M[2];
   T* fb = static_cast<T*>(B.f0_out) + b * M3;
   const T* Eb = pipe_E<T>(p, b, step, M3);
   const cx<TW>* twM = static_cast<const cx<TW>*>(B.twM);
-  for (int pt = threadIdx.x; pt < H0 * YB; pt += kNT) {
+  constexpr int NPT = TilePts<L0>::NPT;
+  T ev[NPT][8];
+  FoldTw<TW> ft[NPT];
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int pt0 = threadIdx.x + k * kNT;
+    const int pt = pt0 < H0 * YB ? pt0 : H0 * YB - 1;  // unconditional loads
+    const int x = pt % H0, y = y0 + pt / H0;
+    ft[k] = load_fold_tw(twM, x, y, z, p);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) ev[k][e] = Eb ? Eb[corner_index(p, x, y, z, e)] : T(1);
+  }
+  tile_to_lds<T, L0>(p, static_cast<const cx<T>*>(B.ch), b, z, y0, lds);
+  __syncthreads();
+  lds_fft<T, TW, L0, +1>(lds, p.n_ch * YB, rs, 1, xch);
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int pt = threadIdx.x + k * kNT;
+    if (pt >= H0 * YB) continue;
     const int x = pt % H0, yy = pt / H0, y = y0 + yy;
     T v[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) v[e] = T(0);
     for (int c = 0; c < p.n_ch; ++c) {
       const int o = p.offs[c];
-      const cx<TW> tw = fold_twiddle(o, twM, x, y, z, p.M[0], p.M[1]);
+      const cx<TW> tw = fold_tw(o, ft[k]);
       const cx<T> g = lds[(c * YB + yy) * rs + x];
       const T val = (T)((TW)g.x * tw.x + (TW)g.y * tw.y);  // Re(g conj(tw))
 #pragma unroll
@@ -382,9 +441,8 @@ __device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, in
     wht8(v);
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const long idx = corner_index(p, x, y, z, e);
       const T r = (T)((TW)v[e] * p.scale_unfold);
-      fb[idx] = Eb ? r * Eb[idx] : r;
+      fb[corner_index(p, x, y, z, e)] = Eb ? r * ev[k][e] : r;
     }
   }
   __syncthreads();
@@ -404,22 +462,42 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
   const int y0 = yb * YB;
   const int rs = H0 + 1;
   cx<T>* ch = static_cast<cx<T>*>(B.ch);
-  tile_to_lds<T, L0>(p, ch, b, z, y0, lds);
-  __syncthreads();
-  lds_fft<T, TW, L0, +1>(lds, p.n_ch * YB, rs, 1, xch);
   const long M3 = (long)p.M[0] * p.M[1] * p.M[2];
   T* fb = static_cast<T*>(B.f0_out) + b * M3;
   const T* Eo = pipe_E<T>(p, b, step, M3);
   const T* En = pipe_E<T>(p, b, step + 1, M3);
   const cx<TW>* twM = static_cast<const cx<TW>*>(B.twM);
-  for (int pt = threadIdx.x; pt < H0 * YB; pt += kNT) {
+  constexpr int NPT = TilePts<L0>::NPT;
+  T eo[NPT][8], en[NPT][8];
+  FoldTw<TW> ft[NPT];
+  const bool same_E = En == Eo;
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int pt0 = threadIdx.x + k * kNT;
+    const int pt = pt0 < H0 * YB ? pt0 : H0 * YB - 1;  // unconditional loads
+    const int x = pt % H0, y = y0 + pt / H0;
+    ft[k] = load_fold_tw(twM, x, y, z, p);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const long idx = corner_index(p, x, y, z, e);
+      eo[k][e] = Eo ? Eo[idx] : T(1);
+      en[k][e] = same_E ? eo[k][e] : (En ? En[idx] : T(1));
+    }
+  }
+  tile_to_lds<T, L0>(p, ch, b, z, y0, lds);
+  __syncthreads();
+  lds_fft<T, TW, L0, +1>(lds, p.n_ch * YB, rs, 1, xch);
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int pt = threadIdx.x + k * kNT;
+    if (pt >= H0 * YB) continue;
     const int x = pt % H0, yy = pt / H0, y = y0 + yy;
     T v[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) v[e] = T(0);
     for (int c = 0; c < p.n_ch; ++c) {
       const int o = p.offs[c];
-      const cx<TW> tw = fold_twiddle(o, twM, x, y, z, p.M[0], p.M[1]);
+      const cx<TW> tw = fold_tw(o, ft[k]);
       const cx<T> g = lds[(c * YB + yy) * rs + x];
       const T val = (T)((TW)g.x * tw.x + (TW)g.y * tw.y);
 #pragma unroll
@@ -429,11 +507,10 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
     wht8(v);
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const long idx = corner_index(p, x, y, z, e);
       T r = (T)((TW)v[e] * p.scale_unfold);
-      if (Eo) r *= Eo[idx];
-      fb[idx] = r;
-      v[e] = En ? r * En[idx] : r;
+      if (Eo) r *= eo[k][e];
+      fb[corner_index(p, x, y, z, e)] = r;
+      v[e] = En ? r * en[k][e] : r;
     }
     wht8(v);
     // the entries of this thread's point only: no barrier needed before reuse
@@ -443,7 +520,7 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
 #pragma unroll
       for (int e = 0; e < 8; ++e)
         if (e == o) val = v[e];
-      const cx<TW> tw = fold_twiddle(o, twM, x, y, z, p.M[0], p.M[1]);
+      const cx<TW> tw = fold_tw(o, ft[k]);
       lds[(c * YB + yy) * rs + x] = {(T)((TW)val * tw.x), (T)((TW)val * tw.y)};
     }
   }
@@ -517,14 +594,22 @@ __global__ void __launch_bounds__(256) k_qrep(const cx<T>* __restrict__ ch, cx<T
   const int b = (int)(tid / n_rep);
   const long r = tid - (long)b * n_rep;
   const cx<T>* cb = ch + (long)b * n_alive;
+  // d <= 8 (L in {1,2}^3): all index and coefficient loads first, then the gathers
+  int g[8];
+  cx<double> q[8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    g[a] = a < d ? gidx[a * n_rep + r] : -1;
+    q[a] = a < d ? Q[a * n_rep + r] : cx<double>{0.0, 0.0};
+  }
+  cx<T> v[8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) v[a] = g[a] >= 0 ? cb[g[a]] : cx<T>{T(0), T(0)};
   double ax = 0.0, ay = 0.0;
-  for (int a = 0; a < d; ++a) {
-    const int g = gidx[a * n_rep + r];
-    if (g < 0) continue;
-    const cx<double> q = Q[a * n_rep + r];
-    const cx<T> v = cb[g];
-    ax += q.x * (double)v.x - q.y * (double)v.y;
-    ay += q.x * (double)v.y + q.y * (double)v.x;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    ax += q[a].x * (double)v[a].x - q[a].y * (double)v[a].y;
+    ay += q[a].x * (double)v[a].y + q[a].y * (double)v[a].x;
   }
   grep[tid] = {(T)ax, (T)ay};
 }

@@ -315,6 +315,29 @@ int cfft_scft_get(cfft_scft* ctx, int32_t which, double* out);
 int cfft_scft_propagators(cfft_scft* ctx, void** hist_out);
 int cfft_scft_destroy(cfft_scft* ctx);
 
+/* ---- warm-start network (multilayer perceptron) ---------------------------- */
+/* No counterpart in the reference.  dims[0..n_layers]: input width, hidden
+ * widths, output width.  SiLU hidden activations, linear output.  Parameters:
+ * FP32 master copy, flat as W_0 (dims[0] x dims[1], row-major), ..., W_{L-1},
+ * then b_0, ..., b_{L-1}; the GEMMs read a BF16 copy and accumulate in FP32
+ * on the matrix cores.  Host pointers, row-major FP32 rows. */
+typedef struct cfft_mlp cfft_mlp;
+
+int cfft_mlp_create(int32_t n_layers, const int32_t* dims, uint64_t seed, cfft_mlp** out);
+int cfft_mlp_info(const cfft_mlp* mlp, int64_t* n_params);
+int cfft_mlp_get_params(cfft_mlp* mlp, float* out);
+/* Also resets the optimiser state. */
+int cfft_mlp_set_params(cfft_mlp* mlp, const float* in);
+int cfft_mlp_forward(cfft_mlp* mlp, const float* x, int32_t rows, float* y);
+/* Mean-squared-error loss over rows x dims[L] and its gradient (n_params). */
+int cfft_mlp_gradient(cfft_mlp* mlp, const float* x, const float* t, int32_t rows, double* loss,
+                      float* grad);
+/* Adam (0.9, 0.999, 1e-8) over `epochs` shuffled passes of mini-batches;
+ * loss_per_epoch[epochs] (may be NULL) = mean training loss of each epoch. */
+int cfft_mlp_train(cfft_mlp* mlp, const float* X, const float* T, int64_t n, int32_t epochs,
+                   int32_t batch, float lr, uint64_t seed, double* loss_per_epoch);
+int cfft_mlp_destroy(cfft_mlp* mlp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -198,3 +198,17 @@ def test_graph_replays_match_eager(prec):
     torch.cuda.synchronize()
     for a, b in zip(sw.fields(), eager):
         assert np.array_equal(a, b)
+
+
+def test_batched_instances_match_sequential():
+    """Concurrent instances (own plan, context and stream each, hipGraph sweeps)
+    give the same solutions as solving them one at a time."""
+    from cfft_hip import batch
+    insts = batch.draw_instances(6, seed=5, sizes=(16, 24, 32), ns=40)
+    res, t_plan, t_solve = batch.solve_batch(insts, precision=cf.FP64, tol=1e-8, max_iter=300, workers=6)
+    print(f"  batch of {len(insts)}: plan {t_plan:.2f}s solve {t_solve:.2f}s", flush=True)
+    for inst, r in zip(insts, res):
+        assert r["res"] < 1e-8, (inst, r)
+        seq, _, _ = batch.solve_batch([inst], precision=cf.FP64, tol=1e-8, max_iter=300, workers=1)
+        assert seq[0]["iters"] == r["iters"]
+        assert abs(seq[0]["F"] - r["F"]) < 1e-12 * max(1.0, abs(r["F"]))
