@@ -44,7 +44,7 @@ EXPORTED = [
     "cfft_block_forward", "cfft_block_inverse", "cfft_block_plan_destroy",
     "cfft_prefold_plan_create", "cfft_prefold_plan_info", "cfft_prefold_forward",
     "cfft_prefold_plan_destroy", "cfft_sweep_hist_dev", "cfft_scft_create", "cfft_scft_set_mixing",
-    "cfft_scft_set_fields", "cfft_scft_iterate", "cfft_scft_solve", "cfft_scft_get",
+    "cfft_scft_set_fields", "cfft_scft_set_params", "cfft_scft_iterate", "cfft_scft_solve", "cfft_scft_get",
     "cfft_scft_propagators", "cfft_scft_destroy", "cfft_mlp_create", "cfft_mlp_info",
     "cfft_mlp_get_params", "cfft_mlp_set_params", "cfft_mlp_forward", "cfft_mlp_gradient",
     "cfft_mlp_train", "cfft_mlp_destroy",
@@ -132,6 +132,7 @@ def _declare(lib):
         "cfft_scft_create": (C.c_int, [P, D, D, I32, I32, C.POINTER(P)]),
         "cfft_scft_set_mixing": (C.c_int, [P, D, D, D]),
         "cfft_scft_set_fields": (C.c_int, [P, PD, PD]),
+        "cfft_scft_set_params": (C.c_int, [P, D, D]),
         "cfft_scft_iterate": (C.c_int, [P, PD]),
         "cfft_scft_solve": (C.c_int, [P, D, I32, PI32, PD]),
         "cfft_scft_get": (C.c_int, [P, I32, PD]),

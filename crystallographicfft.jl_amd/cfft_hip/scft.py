@@ -151,6 +151,16 @@ class SCFTSolver:
                 pass
             self._h = None
 
+    def update(self, lattice=None, f=None, chiN=None):
+        """Reuse the plan and context for another cell (kernel update,
+        src/centering_fold.jl:939-953) and/or another f, chiN."""
+        if lattice is not None:
+            self.plan.update_kernel(1.0 / self.ns, lattice)
+        if f is not None or chiN is not None:
+            self.f = self.f if f is None else float(f)
+            self.chiN = self.chiN if chiN is None else float(chiN)
+            L.check(L.lib().cfft_scft_set_params(self._h, self.f, self.chiN))
+
     def set_mixing(self, lam=1.0, lam_simple=0.1, simple_above=1e-2):
         L.check(L.lib().cfft_scft_set_mixing(self._h, float(lam), float(lam_simple), float(simple_above)))
 

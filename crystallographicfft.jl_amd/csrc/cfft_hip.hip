@@ -1078,8 +1078,12 @@ int cfft_update_kernel(cfft_plan* p, double ds, const double* lattice) {
     p->K = kernel_spec(p->spec, ds, lat);
     p->ds = ds;
     if (lattice) std::memcpy(p->lattice, lattice, sizeof(double) * 9);
-    hip_check(hipStreamSynchronize(p->stream), "sync");
-    upload_real<double>(p->d_K, p->K);
+    // work of this plan may be in flight on any stream; the new values go into
+    // the same allocation, so captured chains keep pointing at valid memory
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    require(p->d_K.bytes == p->K.size() * sizeof(double), CFFT_ERR_INVALID, "kernel table size changed");
+    if (!p->K.empty())
+      hip_check(hipMemcpy(p->d_K.p, p->K.data(), p->d_K.bytes, hipMemcpyHostToDevice), "upload K");
   });
 }
 

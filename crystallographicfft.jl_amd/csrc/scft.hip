@@ -395,6 +395,23 @@ int cfft_scft_create(cfft_plan* plan, double f, double chiN, int32_t ns, int32_t
   });
 }
 
+int cfft_scft_set_params(cfft_scft* c, double f, double chiN) {
+  return guarded([&] {
+    require(c != nullptr, CFFT_ERR_INVALID, "null context");
+    require(f > 0.0 && f < 1.0, CFFT_ERR_INVALID, "need 0 < f < 1");
+    const int nA = (int)std::lround(f * c->ns);
+    require(nA >= 1 && nA < c->ns, CFFT_ERR_INVALID, "f * ns must round into [1, ns - 1]");
+    c->f = f;
+    c->chiN = chiN;
+    c->nA = nA;
+    const std::vector<double> wa = simpson(c->nA), wb = simpson(c->ns - c->nA);
+    c->wa.alloc(8 * wa.size());
+    c->wb.alloc(8 * wb.size());
+    hip_check(hipMemcpy(c->wa.p, wa.data(), 8 * wa.size(), hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(c->wb.p, wb.data(), 8 * wb.size(), hipMemcpyHostToDevice), "upload");
+  });
+}
+
 int cfft_scft_set_mixing(cfft_scft* c, double lam, double lam_simple, double simple_above) {
   return guarded([&] {
     require(c != nullptr, CFFT_ERR_INVALID, "null context");

@@ -297,6 +297,9 @@ typedef struct cfft_scft cfft_scft;
 
 int cfft_scft_create(cfft_plan* plan, double f, double chiN, int32_t ns, int32_t m_hist,
                      cfft_scft** out);
+/* New block fraction and chiN for the next iterations (the contour
+ * discretisation ns is fixed). */
+int cfft_scft_set_params(cfft_scft* ctx, double f, double chiN);
 /* Anderson step length lam, simple-mixing step lam_simple, and the residual
  * above which simple mixing is used (defaults 1.0, 0.1, 1e-2). */
 int cfft_scft_set_mixing(cfft_scft* ctx, double lam, double lam_simple, double simple_above);

@@ -212,3 +212,38 @@ def test_batched_instances_match_sequential():
         seq, _, _ = batch.solve_batch([inst], precision=cf.FP64, tol=1e-8, max_iter=300, workers=1)
         assert seq[0]["iters"] == r["iters"]
         assert abs(seq[0]["F"] - r["F"]) < 1e-12 * max(1.0, abs(r["F"]))
+
+
+def test_solver_update_equals_fresh_solver():
+    """Re-pointing a solver at another cell, f and chiN (kernel update +
+    context parameters) gives the fresh solver's iterations bit for bit."""
+    from cfft_hip.scft import SCFTSolver
+    N = (16, 16, 16)
+    ops_s, R, t, Lv, wA, wB = _bcc_fields(229, N)
+    s = SCFTSolver(N, ops_s, 4.0 * np.eye(3), 0.3, 20.0, 40, cf.FP64)
+    s.set_fields(wA, wB)
+    s.solve(tol=1e-6, max_iter=50)
+    s.update(lattice=3.7 * np.eye(3), f=0.34, chiN=17.0)
+    s.set_fields(wA, wB)
+    a = [s.iterate() for _ in range(5)]
+    fresh = SCFTSolver(N, ops_s, 3.7 * np.eye(3), 0.34, 17.0, 40, cf.FP64)
+    fresh.set_fields(wA, wB)
+    b = [fresh.iterate() for _ in range(5)]
+    for x, y in zip(a, b):
+        assert x == y
+
+
+def test_warm_start_pipeline_small():
+    """End-to-end at 16^3 with a small network: generate converged solutions,
+    train, predict, decode and solve from the prediction (fallback counted)."""
+    from cfft_hip import warmstart as W
+    ps = W.draw(24, seed=3)
+    X, T, ok, its, els, groups = W.generate(ps, (16, 16, 16), workers=4, max_iter=400)
+    assert ok.sum() >= 12
+    ws = W.WarmStart(dims=(6, 64, 64, 64, 4 * W.N_STARS), seed=1)
+    hist = ws.fit(X[ok], T[ok], epochs=200, batch=8, lr=2e-3)
+    assert hist[-1] < hist[0]
+    rows = W.evaluate(ws, W.draw(4, seed=4), groups, max_iter=400, workers=2)
+    summ = W.summary(rows)
+    print("  warm start (16^3, small):", summ, flush=True)
+    assert summ["n"] == 4
