@@ -207,15 +207,110 @@ def scft_ia3d(cf, tol64, tol32, max_iter):
         el = time.perf_counter() - t0
         out[name] = (it, o, el, solver.densities())
         del solver
+    # mixed precision to 1e-6, then an FP64 finish to 1e-8 from those fields
+    s32 = cf.scft.SCFTSolver(n3, ops_s, lat, 0.35, 18.0, 200, cf.MIXED, m_hist=10)
+    s32.set_fields(wA, wB)
+    t0 = time.perf_counter()
+    ia, oa = s32.solve(tol=tol32, max_iter=max_iter)
+    fin = cf.scft.SCFTSolver(n3, ops_s, lat, 0.35, 18.0, 200, cf.FP64, m_hist=10)
+    fin.set_fields(*s32.fields())
+    ib, ob = fin.solve(tol=tol64, max_iter=max_iter)
+    e_mf = time.perf_counter() - t0
+    del s32, fin
     i64, o64, e64, p64 = out["fp64"]
     i32, o32, e32, p32 = out["mixed"]
+    refine = {"iters_mixed": ia, "iters_fp64": ib, "res": ob["res"], "F": ob["F"], "wall_s": e_mf,
+              "abs_dF_vs_fp64": abs(ob["F"] - o64["F"]), "speedup_vs_fp64": e64 / e_mf}
     return {
+        "mixed_then_fp64": refine,
         "fp64": {"iters": i64, "res": o64["res"], "F": o64["F"], "Q": o64["Q"], "wall_s": e64},
         "mixed": {"iters": i32, "res": o32["res"], "F": o32["F"], "Q": o32["Q"], "wall_s": e32},
         "abs_dF": abs(o32["F"] - o64["F"]),
         "rel_dQ": abs(o32["Q"] - o64["Q"]) / abs(o64["Q"]),
         "max_abs_dphi": float(max(np.max(np.abs(a - b)) for a, b in zip(p32, p64))),
     }
+
+
+def batched_fields(cf, torch, ops_s, wA, prec, e, batch, steps, path):
+    """Field-steps/s and the dominant kernel's bandwidth when one plan carries
+    `batch` Im-3m 128^3 fields (the same workload's q, replicated): the same
+    kernels once there is enough work per launch to leave the latency floor."""
+    import ctypes as C
+    lat = CELL * np.eye(3)
+    plan = cf.DevicePlan(N, ops_s, 1.0 / NS, lat, path, prec, batch=batch)
+    M3 = int(np.prod(plan.M))
+    dt = torch.float64 if prec == cf.FP64 else torch.float32
+    w = torch.from_numpy(np.asarray(wA, dtype=np.float64).ravel(order="F")).to("cuda", dt)
+    E = torch.exp(-w * (0.5 / NS)).repeat(batch)
+    q = torch.ones(batch * M3, dtype=dt, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    plan.step_dev(q.data_ptr(), E.data_ptr(), 10, s)  # warm-up (captures the graph)
+    torch.cuda.synchronize()
+    q.fill_(1.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.step_dev(q.data_ptr(), E.data_ptr(), steps, s)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    stages = {}
+    import cfft_hip._lib as L
+    lib = L.lib()
+    L.check(lib.cfft_profile(plan.handle, 1))
+    plan.step_dev(q.data_ptr(), E.data_ptr(), 20, s)
+    torch.cuda.synchronize()
+    ms = np.zeros(len(L.STAGES))
+    cnt = np.zeros(len(L.STAGES), dtype=np.int64)
+    L.check(lib.cfft_profile_read(plan.handle, ms.ctypes.data_as(L.PD), cnt.ctypes.data_as(L.PI64)))
+    L.check(lib.cfft_profile(plan.handle, 0))
+    stages = {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(L.STAGES) if cnt[i]}
+    dom = max(stages, key=lambda k: stages[k][0])
+    avg = stages[dom][0] / stages[dom][1]
+    nb = stage_bytes(plan, e)[dom]
+    return {"batch": batch, "value": batch * steps / el, "unit": "diffusion steps/s",
+            "ms_per_step": el / steps * 1e3, "dominant_kernel": dom, "avg_launch_ms": avg,
+            "achieved_GBs": nb / (avg * 1e-3) / 1e9, "frac": nb / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
+def batched_instances(cf, n, seed, workers):
+    """64 (default) independent instances, sizes {32,48,64,96}^3 and groups
+    {47,70,223,229,230} drawn from a seed, f ~ U(.25,.45), chiN ~ U(12,30),
+    Ns 100: solved concurrently in mixed precision (tol 1e-6) and in FP64
+    (tol 1e-8); instances/s of the mixed batch and the per-instance free-energy
+    error against FP64."""
+    from cfft_hip import batch
+    insts = batch.draw_instances(n, seed)
+    r32, p32, s32 = batch.solve_batch(insts, precision=cf.MIXED, tol=1e-6, max_iter=400, workers=workers)
+    r64, p64, s64 = batch.solve_batch(insts, precision=cf.FP64, tol=1e-8, max_iter=400, workers=workers)
+    dF = [abs(a["F"] - b["F"]) for a, b in zip(r32, r64)]
+    conv = [a["res"] < 1e-6 for a in r32]
+    return {
+        "instances": n, "workers": workers,
+        "mixed": {"instances_per_s": n / s32, "solve_s": s32, "plan_s": p32, "converged": int(sum(conv)),
+                  "iters_mean": float(np.mean([a["iters"] for a in r32]))},
+        "fp64": {"instances_per_s": n / s64, "solve_s": s64, "plan_s": p64,
+                 "converged": int(sum(a["res"] < 1e-8 for a in r64)),
+                 "iters_mean": float(np.mean([a["iters"] for a in r64]))},
+        "abs_dF_vs_fp64": {"max": float(np.max(dF)), "median": float(np.median(dF))},
+    }
+
+
+def learned_warm_start(n_eval, workers):
+    """Cold start vs the trained network's start on held-out parameter sets
+    (Im-3m, Ia-3d, Fddd at the model's grid), residual 1e-8, FP64."""
+    from cfft_hip import warmstart as W
+    path = os.path.join(ROOT, "crystallographicfft.jl_amd", "models", "warmstart.npz")
+    if not os.path.exists(path):
+        return None
+    ws = W.WarmStart.load(path)
+    grid = int(ws.meta.get("grid", 64))
+    ps = W.draw(n_eval, seed=2)
+    groups = {sg: W.Group(sg, (grid,) * 3) for sg in sorted({p.sg for p in ps})}
+    rows = W.evaluate(ws, ps, groups, workers=workers)
+    out = W.summary(rows)
+    out["grid"] = grid
+    out["trained_on"] = ws.meta.get("n_train")
+    out["full_evaluation"] = ws.meta.get("eval")
+    return out
 
 
 def cpu_scft_reference(ops_s, wA, wB):
@@ -265,6 +360,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-scft", action="store_true")
     ap.add_argument("--scft-iters", type=int, default=600)
+    ap.add_argument("--batch", type=int, default=64, help="batched SCFT instances (0: skip)")
+    ap.add_argument("--warm-eval", type=int, default=12, help="held-out sets for the warm start")
+    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--wide", type=int, default=16, help="fields per launch of the batched measurement")
     args = ap.parse_args()
     if args.gpus > 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -276,6 +375,12 @@ def main():
     import cfft_hip as cf
 
     torch.cuda.set_device(0)
+    # one plain 64 MiB device copy: the reference point that scales the HBM
+    # byte counters of a profiler run of this script (traffic per launch)
+    src = torch.empty(8 << 20, dtype=torch.float64, device="cuda")
+    torch.empty_like(src).copy_(src)
+    torch.cuda.synchronize()
+    del src
     ops = cf.get_ops(SG, 3, N)
     shift, ops_s = cf.find_optimal_shift(ops, N)
     Lv = cf.auto_L(ops_s)
@@ -321,6 +426,13 @@ def main():
                          step_bytes=step_bytes(sw.plan, e))
         del sw
 
+    # ---- many fields per launch (bandwidth regime of the same kernels) ----
+    wide = None
+    if args.wide > 0:
+        wide = {}
+        for name, prec, e in (("fp64", cf.FP64, 8), ("mixed", cf.MIXED, 4)):
+            wide[name] = batched_fields(cf, torch, ops_s, wA, prec, e, args.wide, 50, cf.PATH_QR)
+
     # ---- SCFT: Im-3m 128^3 plain cold start, Ia-3d 128^3 mixed vs FP64 ----
     scft = None
     if not args.no_scft:
@@ -335,6 +447,9 @@ def main():
                 "abs_dF": abs(first["F"] - Fc),
                 "max_abs_dphi": float(max(np.max(np.abs(phi0[0] - pA)), np.max(np.abs(phi0[1] - pB))))}
         scft["ia3d_128_mixed"] = scft_ia3d(cf, 1e-8, 1e-6, args.scft_iters)
+        if args.batch > 0:
+            scft["batched"] = batched_instances(cf, args.batch, 4, args.workers)
+        scft["learned_warm_start"] = learned_warm_start(args.warm_eval, args.workers)
 
     r64, r32 = res["fp64"], res["mixed"]
     K = args.steps
@@ -387,6 +502,7 @@ def main():
         "launch_strategy": variants,
         "step_alg_bytes_fp64": r64["step_bytes"],
         "cpu_baseline": cpu,
+        "many_fields_per_launch": wide,
         "scft": scft,
     }
     print(json.dumps(line), flush=True)
