@@ -93,12 +93,16 @@ class Group:
         with self.lock:
             ca = stars.encode(self.plan, wA, n, self.order)
             cb = stars.encode(self.plan, wB, n, self.order)
+        # grids with fewer than n stars: the missing coefficients are zero
+        ca = np.pad(ca, (0, n - ca.size))
+        cb = np.pad(cb, (0, n - cb.size))
         return np.concatenate([ca.real, ca.imag, cb.real, cb.imag]).astype(np.float32)
 
     def decode(self, v, n=N_STARS):
         v = np.asarray(v, dtype=np.float64)
-        ca = v[0:n] + 1j * v[n:2 * n]
-        cb = v[2 * n:3 * n] + 1j * v[3 * n:4 * n]
+        m = min(n, self.plan.n_spec)
+        ca = v[0:m] + 1j * v[n:n + m]
+        cb = v[2 * n:2 * n + m] + 1j * v[3 * n:3 * n + m]
         with self.lock:
             return stars.decode(self.plan, ca, self.order), stars.decode(self.plan, cb, self.order)
 
