@@ -459,9 +459,10 @@ PipeP pipe_params(cfft_plan* p, const ESel& es) {
   for (int c = 0; c < p->n_ch; ++c) pp.offs[c] = p->offs[c][0] | (p->offs[c][1] << 1) | (p->offs[c][2] << 2);
   for (int i = 0; i < 8; ++i) pp.chmap[i] = p->chmap[i];
   pp.batch = p->batch;
-  int yb = std::min(8, p->H[1]);
-  while (p->H[1] % yb) --yb;
+  int yb = 8;  // largest power of two <= 8 dividing H1 (shift/mask tile indexing)
+  while (p->H[1] % yb) yb >>= 1;
   pp.YB = yb;
+  pp.ybs = yb == 8 ? 3 : (yb == 4 ? 2 : (yb == 2 ? 1 : 0));
   pp.n_spec = p->n_spec;
   pp.d = p->d;
   pp.P = p->P;

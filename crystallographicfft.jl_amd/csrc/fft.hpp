@@ -192,6 +192,50 @@ __device__ __forceinline__ void fft_ml(cx<T> (&v)[MLS<LEN>::R2], int r, int g, c
   }
 }
 
+// Multi-lane FFT of one transform that already lives in LDS (element p at
+// slot[p * es]): the exchange between the R1 lanes of the group goes through
+// the transform's own slots.  The R1 lanes are consecutive lanes of one
+// wavefront, so wavefront-scope fences order the exchange; no workgroup
+// barrier and no separate exchange buffer.  Same entry/exit layout as fft_ml.
+template <typename T, typename TW, int LEN, int DIR>
+__device__ __forceinline__ void fft_ml_lds(cx<T> (&v)[MLS<LEN>::R2], int r, cx<T>* slot, int es) {
+  constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2;
+  stockham<T, TW, R2, DIR, 1>(v);
+  if constexpr (R1 > 1) {
+#pragma unroll
+    for (int k = 1; k < R2; ++k) {
+      cx<TW> w = tw_c<LEN, TW>(0);
+#pragma unroll
+      for (int s = 1; s < R1; ++s)
+        if (r == s) w = tw_c<LEN, TW>((s * k) % LEN);
+      v[k] = twmul<T, TW, DIR>(v[k], w);
+    }
+    // every lane of the group has its inputs in registers: the slots are free
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int k = 0; k < R2; ++k) slot[(k * R1 + r) * es] = v[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int j = 0; j < R2 / R1; ++j) {
+      const int kp = r + R1 * j;
+      cx<T> u[R1];
+#pragma unroll
+      for (int s = 0; s < R1; ++s) u[s] = slot[(kp * R1 + s) * es];
+      dft_small<R1, DIR, T, TW>(u);
+#pragma unroll
+      for (int k2 = 0; k2 < R1; ++k2) v[j * R1 + k2] = u[k2];
+    }
+    // the group's reads of the exchange precede the caller's stores
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
 // Output position of register p after fft_ml.
 template <int LEN>
 __device__ __forceinline__ int ml_out_pos(int r, int p) {

@@ -28,7 +28,8 @@ constexpr int kNT = 256;  // threads per workgroup of the phase kernels
 struct PipeP {
   int M[3], H[3];
   int n_ch, offs[8], chmap[8];
-  int batch, YB;  // y rows per P1/P5 tile (divides H1)
+  int batch, YB;  // y rows per P1/P5 tile: a power of two dividing H1
+  int ybs;        // log2(YB)
   int n_spec, d, P, Pn[3];
   int R[8][9], t[8][3];
   // interaction factors: entry b uses E1[b] for steps < sw[b], else E2[b]
@@ -66,20 +67,24 @@ struct PipeG {
 // FFT of n_tr length-LEN transforms held in LDS: element j of transform t is
 // buf[t * ts + j * es].  Every thread of the workgroup must call.
 template <typename T, typename TW, int LEN, int DIR>
-__device__ __forceinline__ void lds_fft(cx<T>* buf, int n_tr, int ts, int es, cx<T>* xch) {
+__device__ __forceinline__ void lds_fft(cx<T>* buf, int n_tr, int ts, int es) {
   if constexpr (LEN > 1) {
-    constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2, G = PipeG<LEN>::G, GP = PipeG<LEN>::GP;
+    constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2, G = PipeG<LEN>::G;
     const int r = threadIdx.x % R1, g = threadIdx.x / R1;
     for (int t0 = 0; t0 < n_tr; t0 += G) {
       const int t = t0 + g;
+      // groups past the last transform work on a copy of transform 0's
+      // values in registers and store nothing: whole groups only, so the
+      // in-slot exchange of the real groups is not disturbed
       const bool ok = t < n_tr;
+      cx<T>* slot = buf + (ok ? t : 0) * ts;
       cx<T> v[R2];
 #pragma unroll
-      for (int m = 0; m < R2; ++m) v[m] = ok ? buf[t * ts + (r + R1 * m) * es] : cx<T>{T(0), T(0)};
-      fft_ml<T, TW, LEN, DIR, GP>(v, r, g, xch);
+      for (int m = 0; m < R2; ++m) v[m] = slot[(r + R1 * m) * es];
       if (ok) {
+        fft_ml_lds<T, TW, LEN, DIR>(v, r, slot, es);
 #pragma unroll
-        for (int q = 0; q < R2; ++q) buf[t * ts + ml_out_pos<LEN>(r, q) * es] = v[q];
+        for (int q = 0; q < R2; ++q) slot[ml_out_pos<LEN>(r, q) * es] = v[q];
       }
     }
   }
@@ -141,14 +146,14 @@ __device__ __forceinline__ void tile_to_lds(const PipeP& p, const cx<T>* __restr
   for (int it = 0; it < MAXIT; ++it) {
     const int i0 = threadIdx.x + it * kNT;
     const int i = i0 < n ? i0 : n - 1;  // unconditional load, clamped index
-    const int yy = i % YB, x = (i / YB) % L0, c = i / (YB * L0);
+    const int yy = i & (YB - 1), x = (i >> p.ybs) % L0, c = (i >> p.ybs) / L0;
     r[it] = ch[(((long)b * p.n_ch + c) * L0 + x) * HH + (long)z * p.H[1] + y0 + yy];
   }
 #pragma unroll
   for (int it = 0; it < MAXIT; ++it) {
     const int i = threadIdx.x + it * kNT;
     if (i < n) {
-      const int yy = i % YB, x = (i / YB) % L0, c = i / (YB * L0);
+      const int yy = i & (YB - 1), x = (i >> p.ybs) % L0, c = (i >> p.ybs) / L0;
       lds[(c * YB + yy) * (L0 + 1) + x] = r[it];
     }
   }
@@ -164,7 +169,7 @@ __device__ __forceinline__ void lds_to_tile(const PipeP& p, const cx<T>* lds, cx
   for (int it = 0; it < MAXIT; ++it) {
     const int i = threadIdx.x + it * kNT;
     if (i < n) {
-      const int yy = i % YB, x = (i / YB) % L0, c = i / (YB * L0);
+      const int yy = i & (YB - 1), x = (i >> p.ybs) % L0, c = (i >> p.ybs) / L0;
       ch[(((long)b * p.n_ch + c) * L0 + x) * HH + (long)z * p.H[1] + y0 + yy] = lds[(c * YB + yy) * (L0 + 1) + x];
     }
   }
@@ -204,8 +209,7 @@ __device__ __forceinline__ FoldTw<TW> load_fold_tw(const cx<TW>* __restrict__ tw
 // ---- P1: fold + E + FFT x --------------------------------------------------
 // tile item = (b, z, y-block); LDS A[c][yy][x] with row stride H0 + 1.
 template <typename T, typename TW, int L0>
-__device__ void ph_fold_fftx(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds,
-                             cx<T>* xch) {
+__device__ void ph_fold_fftx(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds) {
   const int H0 = p.H[0], H1 = p.H[1], H2 = p.H[2], YB = p.YB;
   const int nyb = H1 / YB;
   const int yb = (int)(item % nyb);
@@ -249,7 +253,7 @@ __device__ void ph_fold_fftx(const PipeP& p, const PipeBufs& B, long item, int s
     }
   }
   __syncthreads();
-  lds_fft<T, TW, L0, -1>(lds, p.n_ch * YB, rs, 1, xch);
+  lds_fft<T, TW, L0, -1>(lds, p.n_ch * YB, rs, 1);
   // store to ch[b][c][x][z][y] (y fastest within a tile row)
   lds_to_tile<T, L0>(p, lds, static_cast<cx<T>*>(B.ch), b, z, y0);
   __syncthreads();
@@ -257,13 +261,13 @@ __device__ void ph_fold_fftx(const PipeP& p, const PipeBufs& B, long item, int s
 
 // ---- P2: FFT y then FFT z of one (b, c, x) plane ----------------------------
 template <typename T, typename TW, int L1, int L2>
-__device__ void ph_fft_yz(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds, cx<T>* xch) {
+__device__ void ph_fft_yz(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds) {
   constexpr int rs = L1 + 1;
   cx<T>* plane = static_cast<cx<T>*>(B.ch) + item * ((long)L1 * L2);
   plane_to_lds<T, L1, L2>(plane, lds);
   __syncthreads();
-  lds_fft<T, TW, L1, -1>(lds, L2, rs, 1, xch);
-  lds_fft<T, TW, L2, -1>(lds, L1, 1, rs, xch);
+  lds_fft<T, TW, L1, -1>(lds, L2, rs, 1);
+  lds_fft<T, TW, L2, -1>(lds, L1, 1, rs);
   lds_to_plane<T, L1, L2>(lds, plane);
   __syncthreads();
 }
@@ -303,8 +307,7 @@ __device__ __forceinline__ void ph_recon_point(const PipeP& p, const PipeBufs& B
 // G0(q) = (1/d) sum_alpha F(q + M alpha) through the reverse-lookup table
 // (src/krfft.jl:1285-1311, src/centering_fold.jl:782-813).
 template <typename T, typename TW, int L1, int L2>
-__device__ void ph_irecon_ifft_zy(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds,
-                                  cx<T>* xch) {
+__device__ void ph_irecon_ifft_zy(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds) {
   constexpr int rs = L1 + 1, NP = L1 * L2, NIT = (NP + kNT - 1) / kNT;
   const long H3 = (long)p.H[0] * NP;
   const long n_alive = (long)p.n_ch * H3;
@@ -341,8 +344,8 @@ __device__ void ph_irecon_ifft_zy(const PipeP& p, const PipeBufs& B, long item, 
     lds[(i / L1) * rs + (i % L1)] = {(T)((TW)acc.x * p.scale_irecon), (T)((TW)acc.y * p.scale_irecon)};
   }
   __syncthreads();
-  lds_fft<T, TW, L2, +1>(lds, L1, 1, rs, xch);
-  lds_fft<T, TW, L1, +1>(lds, L2, rs, 1, xch);
+  lds_fft<T, TW, L2, +1>(lds, L1, 1, rs);
+  lds_fft<T, TW, L1, +1>(lds, L2, rs, 1);
   lds_to_plane<T, L1, L2>(lds, static_cast<cx<T>*>(B.ch) + item * (long)NP);
   __syncthreads();
 }
@@ -350,7 +353,7 @@ __device__ void ph_irecon_ifft_zy(const PipeP& p, const PipeBufs& B, long item, 
 // Orbit expansion G0(e) = exp(2 pi i j_e / P) G0'(rep_e), then IFFT z, IFFT y
 // of one (b, c, x) plane (path QR).
 template <typename T, typename TW, int L1, int L2>
-__device__ void ph_expand_ifft_zy(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds, cx<T>* xch) {
+__device__ void ph_expand_ifft_zy(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds) {
   constexpr int rs = L1 + 1, NP = L1 * L2, NIT = (NP + kNT - 1) / kNT;
   const long n_alive = (long)p.n_ch * p.H[0] * NP;
   const int b = (int)(item / ((long)p.n_ch * p.H[0]));
@@ -370,8 +373,8 @@ __device__ void ph_expand_ifft_zy(const PipeP& p, const PipeBufs& B, long item, 
       lds[(i / L1) * rs + (i % L1)] = twmul<T, TW, -1>(G[ent[it] & 0xFFFFFu], W[ent[it] >> 20]);
   }
   __syncthreads();
-  lds_fft<T, TW, L2, +1>(lds, L1, 1, rs, xch);
-  lds_fft<T, TW, L1, +1>(lds, L2, rs, 1, xch);
+  lds_fft<T, TW, L2, +1>(lds, L1, 1, rs);
+  lds_fft<T, TW, L1, +1>(lds, L2, rs, 1);
   lds_to_plane<T, L1, L2>(lds, static_cast<cx<T>*>(B.ch) + item * (long)NP);
   __syncthreads();
 }
@@ -379,12 +382,12 @@ __device__ void ph_expand_ifft_zy(const PipeP& p, const PipeBufs& B, long item, 
 // IFFT z, IFFT y of one plane of `src` written to `dst` (Q-fused path).
 template <typename T, typename TW, int L1, int L2>
 __device__ void ph_ifft_zy(const PipeP& p, const PipeBufs& B, const cx<T>* src, long item,
-                           cx<T>* lds, cx<T>* xch) {
+                           cx<T>* lds) {
   constexpr int rs = L1 + 1;
   plane_to_lds<T, L1, L2>(src + item * ((long)L1 * L2), lds);
   __syncthreads();
-  lds_fft<T, TW, L2, +1>(lds, L1, 1, rs, xch);
-  lds_fft<T, TW, L1, +1>(lds, L2, rs, 1, xch);
+  lds_fft<T, TW, L2, +1>(lds, L1, 1, rs);
+  lds_fft<T, TW, L1, +1>(lds, L2, rs, 1);
   lds_to_plane<T, L1, L2>(lds, static_cast<cx<T>*>(B.ch) + item * ((long)L1 * L2));
   __syncthreads();
 }
@@ -393,8 +396,7 @@ __device__ void ph_ifft_zy(const PipeP& p, const PipeBufs& B, const cx<T>* src, 
 // f0(x + H eps) = scale * sum_c (-1)^(off_c.eps) Re(g_c(x) conj(tw_c))
 // (src/centering_fold.jl:481-580); scale = 1/(8 H^3) (IFFT and inverse WHT).
 template <typename T, typename TW, int L0>
-__device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds,
-                                cx<T>* xch) {
+__device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds) {
   const int H0 = p.H[0], H1 = p.H[1], H2 = p.H[2], YB = p.YB;
   const int nyb = H1 / YB;
   const int yb = (int)(item % nyb);
@@ -420,7 +422,7 @@ __device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, in
   }
   tile_to_lds<T, L0>(p, static_cast<const cx<T>*>(B.ch), b, z, y0, lds);
   __syncthreads();
-  lds_fft<T, TW, L0, +1>(lds, p.n_ch * YB, rs, 1, xch);
+  lds_fft<T, TW, L0, +1>(lds, p.n_ch * YB, rs, 1);
 #pragma unroll
   for (int k = 0; k < NPT; ++k) {
     const int pt = threadIdx.x + k * kNT;
@@ -452,8 +454,7 @@ __device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, in
 // IFFT x + unfold + E_s writes q(s + 1) to f0_out; the same thread then folds
 // E_{s+1} q(s + 1) at the same tile points, so q(s + 1) is not read back.
 template <typename T, typename TW, int L0>
-__device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds,
-                               cx<T>* xch) {
+__device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds) {
   const int H0 = p.H[0], H1 = p.H[1], H2 = p.H[2], YB = p.YB;
   const int nyb = H1 / YB;
   const int yb = (int)(item % nyb);
@@ -486,7 +487,7 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
   }
   tile_to_lds<T, L0>(p, ch, b, z, y0, lds);
   __syncthreads();
-  lds_fft<T, TW, L0, +1>(lds, p.n_ch * YB, rs, 1, xch);
+  lds_fft<T, TW, L0, +1>(lds, p.n_ch * YB, rs, 1);
 #pragma unroll
   for (int k = 0; k < NPT; ++k) {
     const int pt = threadIdx.x + k * kNT;
@@ -525,7 +526,7 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
     }
   }
   __syncthreads();
-  lds_fft<T, TW, L0, -1>(lds, p.n_ch * YB, rs, 1, xch);
+  lds_fft<T, TW, L0, -1>(lds, p.n_ch * YB, rs, 1);
   lds_to_tile<T, L0>(p, lds, ch, b, z, y0);
   __syncthreads();
 }
@@ -534,16 +535,13 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
 template <typename T, typename TW, int L0>
 __global__ void __launch_bounds__(kNT) k_p1(PipeP p, PipeBufs B, int step) {
   __shared__ cx<T> lds[64 * (L0 + 1)];
-  __shared__ cx<T> xch[PipeG<L0>::XCH];
-  ph_fold_fftx<T, TW, L0>(p, B, blockIdx.x, step, lds, xch);
+  ph_fold_fftx<T, TW, L0>(p, B, blockIdx.x, step, lds);
 }
 
 template <typename T, typename TW, int L1, int L2>
 __global__ void __launch_bounds__(kNT) k_p2(PipeP p, PipeBufs B) {
-  constexpr int X = PipeG<L1>::XCH > PipeG<L2>::XCH ? PipeG<L1>::XCH : PipeG<L2>::XCH;
   __shared__ cx<T> lds[L2 * (L1 + 1)];
-  __shared__ cx<T> xch[X];
-  ph_fft_yz<T, TW, L1, L2>(p, B, blockIdx.x, lds, xch);
+  ph_fft_yz<T, TW, L1, L2>(p, B, blockIdx.x, lds);
 }
 
 template <typename T, typename TW>
@@ -554,33 +552,26 @@ __global__ void __launch_bounds__(kNT) k_p3(PipeP p, PipeBufs B) {
 
 template <typename T, typename TW, int L1, int L2>
 __global__ void __launch_bounds__(kNT) k_p4(PipeP p, PipeBufs B) {
-  constexpr int X = PipeG<L1>::XCH > PipeG<L2>::XCH ? PipeG<L1>::XCH : PipeG<L2>::XCH;
   __shared__ cx<T> lds[L2 * (L1 + 1)];
-  __shared__ cx<T> xch[X];
-  ph_irecon_ifft_zy<T, TW, L1, L2>(p, B, blockIdx.x, lds, xch);
+  ph_irecon_ifft_zy<T, TW, L1, L2>(p, B, blockIdx.x, lds);
 }
 
 template <typename T, typename TW, int L1, int L2>
 __global__ void __launch_bounds__(kNT) k_p4q(PipeP p, PipeBufs B, const cx<T>* src) {
-  constexpr int X = PipeG<L1>::XCH > PipeG<L2>::XCH ? PipeG<L1>::XCH : PipeG<L2>::XCH;
   __shared__ cx<T> lds[L2 * (L1 + 1)];
-  __shared__ cx<T> xch[X];
-  ph_ifft_zy<T, TW, L1, L2>(p, B, src, blockIdx.x, lds, xch);
+  ph_ifft_zy<T, TW, L1, L2>(p, B, src, blockIdx.x, lds);
 }
 
 template <typename T, typename TW, int L0>
 __global__ void __launch_bounds__(kNT) k_p5(PipeP p, PipeBufs B, int step) {
   __shared__ cx<T> lds[64 * (L0 + 1)];
-  __shared__ cx<T> xch[PipeG<L0>::XCH];
-  ph_ifftx_unfold<T, TW, L0>(p, B, blockIdx.x, step, lds, xch);
+  ph_ifftx_unfold<T, TW, L0>(p, B, blockIdx.x, step, lds);
 }
 
 template <typename T, typename TW, int L1, int L2>
 __global__ void __launch_bounds__(kNT) k_p4r(PipeP p, PipeBufs B) {
-  constexpr int X = PipeG<L1>::XCH > PipeG<L2>::XCH ? PipeG<L1>::XCH : PipeG<L2>::XCH;
   __shared__ cx<T> lds[L2 * (L1 + 1)];
-  __shared__ cx<T> xch[X];
-  ph_expand_ifft_zy<T, TW, L1, L2>(p, B, blockIdx.x, lds, xch);
+  ph_expand_ifft_zy<T, TW, L1, L2>(p, B, blockIdx.x, lds);
 }
 
 // Q multiply at the orbit representatives (path QR):
@@ -630,8 +621,7 @@ __global__ void __launch_bounds__(256) k_expand(const cx<T>* __restrict__ grep, 
 template <typename T, typename TW, int L0>
 __global__ void __launch_bounds__(kNT) k_p51(PipeP p, PipeBufs B, int step) {
   __shared__ cx<T> lds[64 * (L0 + 1)];
-  __shared__ cx<T> xch[PipeG<L0>::XCH];
-  ph_unfold_fold<T, TW, L0>(p, B, blockIdx.x, step, lds, xch);
+  ph_unfold_fold<T, TW, L0>(p, B, blockIdx.x, step, lds);
 }
 
 // ---- persistent sweep: nsteps Strang steps in one cooperative launch ---------
@@ -645,25 +635,24 @@ __global__ void __launch_bounds__(kNT) k_sweep(PipeP p, PipeBufs B, int step0, i
   constexpr int TILE = 64 * (L + 1);
   constexpr int PLANE = L * (L + 1);
   __shared__ cx<T> lds[TILE > PLANE ? TILE : PLANE];
-  __shared__ cx<T> xch[PipeG<L>::XCH];
   const long n_tiles = (long)p.batch * p.H[2] * (p.H[1] / p.YB);
   const long n_planes = (long)p.batch * p.n_ch * p.H[0];
   const long n_pts = (long)p.n_spec * p.batch;
   const long n_oct = (n_pts * 8 + kNT - 1) / kNT;  // workgroup-chunks of P3
   for (int s = 0; s < nsteps; ++s) {
     const int step = step0 + s;
-    for (long it = blockIdx.x; it < n_tiles; it += gridDim.x) ph_fold_fftx<T, TW, L>(p, B, it, step, lds, xch);
+    for (long it = blockIdx.x; it < n_tiles; it += gridDim.x) ph_fold_fftx<T, TW, L>(p, B, it, step, lds);
     grid.sync();
-    for (long it = blockIdx.x; it < n_planes; it += gridDim.x) ph_fft_yz<T, TW, L, L>(p, B, it, lds, xch);
+    for (long it = blockIdx.x; it < n_planes; it += gridDim.x) ph_fft_yz<T, TW, L, L>(p, B, it, lds);
     grid.sync();
     for (long it = blockIdx.x; it < n_oct; it += gridDim.x) {
       const long pt = (it * kNT + threadIdx.x) >> 3;
       ph_recon_point<T, TW>(p, B, pt, pt < n_pts);
     }
     grid.sync();
-    for (long it = blockIdx.x; it < n_planes; it += gridDim.x) ph_irecon_ifft_zy<T, TW, L, L>(p, B, it, lds, xch);
+    for (long it = blockIdx.x; it < n_planes; it += gridDim.x) ph_irecon_ifft_zy<T, TW, L, L>(p, B, it, lds);
     grid.sync();
-    for (long it = blockIdx.x; it < n_tiles; it += gridDim.x) ph_ifftx_unfold<T, TW, L>(p, B, it, step, lds, xch);
+    for (long it = blockIdx.x; it < n_tiles; it += gridDim.x) ph_ifftx_unfold<T, TW, L>(p, B, it, step, lds);
     grid.sync();
   }
 }
