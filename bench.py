@@ -485,7 +485,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded BCC first-star w fields + symmetrised noise)",
         "config": {"workload": "Im-3m #229 128^3 cell 4.0 AB diblock f=0.3 chiN=20 Ns=100, "
-                               "q and q+ batched (centred KRFFT M7 path)",
+                               "q and q+ batched; FP64 launch strategy: " + variants["fp64"]["used"],
                    "grid": list(N), "space_group": SG, "batch_fields": 2, "contour_steps": NS},
         "roofline": {"bound": "hbm", "kernel": r64["dom"], "achieved": r64["achieved"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS,

@@ -59,9 +59,7 @@ struct PipeBufs {
 template <int LEN>
 struct PipeG {
   static constexpr int R1 = MLS<LEN>::R1;
-  static constexpr int G = kNT / R1;
-  static constexpr int GP = G + 1;
-  static constexpr int XCH = R1 > 1 ? MLS<LEN>::R2 * R1 * GP : 1;
+  static constexpr int G = kNT / R1;  // transforms per pass of a workgroup
 };
 
 // FFT of n_tr length-LEN transforms held in LDS: element j of transform t is
