@@ -239,6 +239,7 @@ int cfft_pack_asu(int32_t dim, const int32_t* idx, const int32_t* depth, int64_t
 
 int cfft_block_plan_create(int32_t dim, const int32_t* blocks, int64_t n_blocks, cfft_bplan** out) {
   return guarded([&] {
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
     check_dim(dim);
     require(out && n_blocks >= 0 && (n_blocks == 0 || blocks), CFFT_ERR_INVALID, "null argument");
     *out = nullptr;
@@ -309,6 +310,7 @@ int cfft_block_inverse(cfft_bplan* p, const double* in, double* out) {
 int cfft_block_plan_destroy(cfft_bplan* p) {
   return guarded([&] {
     if (!p) return;
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     delete p;
   });
@@ -316,6 +318,7 @@ int cfft_block_plan_destroy(cfft_bplan* p) {
 
 int cfft_prefold_plan_create(const int32_t* N, const int32_t* ops, int32_t nops, cfft_pplan** out) {
   return guarded([&] {
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
     require(N && ops && out && nops > 0, CFFT_ERR_INVALID, "null argument");
     *out = nullptr;
     std::unique_ptr<cfft_pplan> p(new cfft_pplan());
@@ -421,6 +424,7 @@ int cfft_prefold_forward(cfft_pplan* p, const double* u, double* F) {
 int cfft_prefold_plan_destroy(cfft_pplan* p) {
   return guarded([&] {
     if (!p) return;
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     delete p;
   });

@@ -23,6 +23,11 @@ using namespace cfft;
 
 thread_local std::string cfft::g_last_error;
 
+std::recursive_mutex& cfft::setup_mutex() {
+  static std::recursive_mutex m;
+  return m;
+}
+
 struct cfft_plan {
   int N[3], M[3], L[3], H[3];
   int d = 1, n_spec = 0, cent = 0, n_ch = 0, path = 0, prec = 0, batch = 1, P = 1;
@@ -854,6 +859,7 @@ void run_steps_graph(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0
       hip_check(hipGraphLaunch(c.exec, s), "hipGraphLaunch");
       return;
     }
+  std::lock_guard<std::recursive_mutex> guard(setup_mutex());
   if (!p->cap_stream) hip_check(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking), "hipStreamCreate");
   hipGraph_t g = nullptr;
   hip_check(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -1037,6 +1043,7 @@ int cfft_plan_create(const int32_t* N, const int32_t* ops, int32_t nops, double 
   return guarded([&] {
     require(N && ops && lattice && out && nops > 0, CFFT_ERR_INVALID, "null argument");
     *out = nullptr;
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
     std::unique_ptr<cfft_plan> p(new cfft_plan());
     int n[3] = {N[0], N[1], N[2]};
     build_plan(p.get(), n, unpack_ops(ops, nops), ds, lattice, path, precision, batch);
@@ -1081,6 +1088,7 @@ int cfft_update_kernel(cfft_plan* p, double ds, const double* lattice) {
     if (lattice) std::memcpy(p->lattice, lattice, sizeof(double) * 9);
     // work of this plan may be in flight on any stream; the new values go into
     // the same allocation, so captured chains keep pointing at valid memory
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     require(p->d_K.bytes == p->K.size() * sizeof(double), CFFT_ERR_INVALID, "kernel table size changed");
     if (!p->K.empty())
@@ -1091,6 +1099,7 @@ int cfft_update_kernel(cfft_plan* p, double ds, const double* lattice) {
 int cfft_plan_destroy(cfft_plan* p) {
   return guarded([&] {
     if (!p) return;
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     delete p;
   });

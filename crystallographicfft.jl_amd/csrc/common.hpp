@@ -12,9 +12,18 @@
 #include "../../include/cfft_hip.h"
 #include "plan.hpp"
 
+#include <mutex>
+
 namespace cfft {
 
 extern thread_local std::string g_last_error;
+
+// Serialises hipGraph capture (stream capture in one thread) with the
+// library's set-up calls that use the null stream, allocate or free memory
+// (plan and context creation, table updates, destruction, every device
+// allocation) in other threads: such calls during another thread's capture
+// invalidate that capture.  Recursive: set-up calls allocate while holding it.
+std::recursive_mutex& setup_mutex();
 
 
 inline void hip_check(hipError_t e, const char* what) {
@@ -67,13 +76,16 @@ struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
   void alloc(size_t n) {
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
     release();
     if (n == 0) return;
     hip_check(hipMalloc(&p, n), "hipMalloc");
     bytes = n;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (!p) return;
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
+    (void)hipFree(p);
     p = nullptr;
     bytes = 0;
   }

@@ -292,6 +292,7 @@ int cfft_mlp_create(int32_t n_layers, const int32_t* dims, uint64_t seed, cfft_m
   return guarded([&] {
     require(n_layers >= 1 && dims && out, CFFT_ERR_INVALID, "bad argument");
     *out = nullptr;
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
     std::unique_ptr<cfft_mlp> p(new cfft_mlp());
     p->L = n_layers;
     p->dims.assign(dims, dims + n_layers + 1);
@@ -436,6 +437,7 @@ int cfft_mlp_train(cfft_mlp* p, const float* X, const float* T, int64_t n, int32
 int cfft_mlp_destroy(cfft_mlp* p) {
   return guarded([&] {
     if (!p) return;
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     delete p;
   });

@@ -356,6 +356,7 @@ int cfft_scft_create(cfft_plan* plan, double f, double chiN, int32_t ns, int32_t
     require(ns >= 2 && f > 0.0 && f < 1.0, CFFT_ERR_INVALID, "need ns >= 2 and 0 < f < 1");
     require(m_hist >= 0 && m_hist < kMaxHist, CFFT_ERR_INVALID, "Anderson history must be < 32");
     *out = nullptr;
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
     int32_t info[18];
     int rc = cfft_plan_info(plan, info);
     if (rc != CFFT_OK) throw Error{rc, g_last_error};
@@ -386,8 +387,8 @@ int cfft_scft_create(cfft_plan* plan, double f, double chiN, int32_t ns, int32_t
     c->slots_dev.alloc(4 * kMaxHist);
     c->G.assign((m_hist + 1) * (m_hist + 1), 0.0);
     const std::vector<double> wa = simpson(c->nA), wb = simpson(ns - c->nA);
-    c->wa.alloc(8 * wa.size());
-    c->wb.alloc(8 * wb.size());
+    c->wa.alloc(8 * (size_t)(ns + 1));
+    c->wb.alloc(8 * (size_t)(ns + 1));
     hip_check(hipMemcpy(c->wa.p, wa.data(), 8 * wa.size(), hipMemcpyHostToDevice), "upload");
     hip_check(hipMemcpy(c->wb.p, wb.data(), 8 * wb.size(), hipMemcpyHostToDevice), "upload");
     hip_check(hipMemset(c->w.p, 0, 8 * n2), "memset");
@@ -404,11 +405,11 @@ int cfft_scft_set_params(cfft_scft* c, double f, double chiN) {
     c->f = f;
     c->chiN = chiN;
     c->nA = nA;
+    // the weight buffers hold ns + 1 entries each (sized at creation)
     const std::vector<double> wa = simpson(c->nA), wb = simpson(c->ns - c->nA);
-    c->wa.alloc(8 * wa.size());
-    c->wb.alloc(8 * wb.size());
-    hip_check(hipMemcpy(c->wa.p, wa.data(), 8 * wa.size(), hipMemcpyHostToDevice), "upload");
-    hip_check(hipMemcpy(c->wb.p, wb.data(), 8 * wb.size(), hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpyAsync(c->wa.p, wa.data(), 8 * wa.size(), hipMemcpyHostToDevice, c->stream), "upload");
+    hip_check(hipMemcpyAsync(c->wb.p, wb.data(), 8 * wb.size(), hipMemcpyHostToDevice, c->stream), "upload");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
   });
 }
 
@@ -481,6 +482,7 @@ int cfft_scft_propagators(cfft_scft* c, void** hist_out) {
 int cfft_scft_destroy(cfft_scft* c) {
   return guarded([&] {
     if (!c) return;
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     delete c;
   });
