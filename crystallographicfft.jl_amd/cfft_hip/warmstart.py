@@ -16,7 +16,7 @@ import argparse
 import json
 import threading
 import time
-from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import ThreadPoolExecutor, as_completed
 from dataclasses import dataclass
 
 import numpy as np
@@ -110,6 +110,21 @@ class Group:
 _tls = threading.local()
 
 
+def _run_all(fn, items, workers, stage, every=100):
+    """fn over items on a thread pool, in order; prints a progress line each
+    time another `every` items have finished."""
+    out = [None] * len(items)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        futs = {ex.submit(fn, it): i for i, it in enumerate(items)}
+        for k, fut in enumerate(as_completed(futs), 1):
+            out[futs[fut]] = fut.result()
+            if k % every == 0 or k == len(items):
+                print(json.dumps({"stage": stage, "done": k, "of": len(items),
+                                  "wall_s": round(time.perf_counter() - t0, 2)}), flush=True)
+    return out
+
+
 def _solver(g: Group, p: Params, precision):
     """One solver (plan + context) per thread and group, re-pointed at each
     instance's cell, f and chiN (plan creation is not repeated per instance)."""
@@ -148,8 +163,7 @@ def generate(ps, N, workers=8, tol=1e-8, max_iter=600):
         ok = out["res"] < tol and np.isfinite(out["F"])
         return g.encode(wA, wB), ok, it, el
 
-    with ThreadPoolExecutor(max_workers=workers) as ex:
-        res = list(ex.map(one, ps))
+    res = _run_all(one, ps, workers, "generate")
     T = np.stack([r[0] for r in res])
     ok = np.array([r[1] for r in res])
     its = np.array([r[2] for r in res])
@@ -190,15 +204,21 @@ class WarmStart:
 def evaluate(ws: WarmStart, ps, groups, tol=1e-8, max_iter=600, workers=8):
     """Cold start vs network start on every instance; a network start that does
     not reach `tol` is finished from the cold start (fallback), and its
-    iterations and time are both counted."""
-    X = features(ps)
-    V = ws.predict(X)
+    iterations and time are both counted.  The network start's time includes
+    its share of the inference and the decoding of its star coefficients."""
+    t0 = time.perf_counter()
+    V = ws.predict(features(ps))
+    t_pred = (time.perf_counter() - t0) / max(len(ps), 1)  # inference, per instance
 
     def one(i):
         p = ps[i]
         g = groups[p.sg]
         ic, tc, oc, _ = solve(p, g, None, tol, max_iter)
-        iw, tw, ow, _ = solve(p, g, g.decode(V[i]), tol, max_iter // 2)
+        t1 = time.perf_counter()
+        guess = g.decode(V[i])
+        t_dec = time.perf_counter() - t1
+        iw, tw, ow, _ = solve(p, g, guess, tol, max_iter // 2)
+        tw += t_pred + t_dec
         fell_back = not (ow["res"] < tol)
         if fell_back:
             iw += ic
@@ -206,8 +226,7 @@ def evaluate(ws: WarmStart, ps, groups, tol=1e-8, max_iter=600, workers=8):
         return dict(cold_iters=ic, cold_s=tc, warm_iters=iw, warm_s=tw, fallback=fell_back,
                     dF=abs(ow["F"] - oc["F"]) if not fell_back else 0.0, cold_ok=oc["res"] < tol)
 
-    with ThreadPoolExecutor(max_workers=workers) as ex:
-        return list(ex.map(one, range(len(ps))))
+    return _run_all(one, list(range(len(ps))), workers, "evaluate", every=25)
 
 
 def summary(rows):
@@ -222,6 +241,9 @@ def summary(rows):
         "time_ratio": float(wsec.sum() / cs.sum()), "iter_ratio": float(wi.sum() / ci.sum()),
         "fallbacks": int(sum(r["fallback"] for r in rows)),
         "max_abs_dF": float(max(r["dF"] for r in rows)),
+        # warm starts that converged to another stationary point than the cold start
+        "other_solution": int(sum(r["dF"] > 1e-6 for r in rows)),
+        "cold_not_converged": int(sum(not r["cold_ok"] for r in rows)),
     }
 
 
