@@ -33,6 +33,21 @@ def first_star_bcc(N, shift_grid, cell):
             np.cos(tp * (Y - Z)) + np.cos(tp * (Z + X)) + np.cos(tp * (Z - X)))
 
 
+def double_gyroid_211(N, shift_grid, cell):
+    """sin 2x cos y sin z + sin 2y cos z sin x + sin 2z cos x sin y (x in units
+    of 2 pi / a): the {211} star, the first harmonic of the double gyroid that
+    Ia-3d does not extinguish (the {110} terms of the single-gyroid nodal
+    function sin x cos y + ... are extinct in Ia-3d and symmetrise to zero).
+    It is the {211} part of the square of that nodal function, so it is large
+    on both networks."""
+    x = [(np.arange(n) + b) / n for n, b in zip(N, shift_grid)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    tp = 2.0 * np.pi
+    return (np.sin(2 * tp * X) * np.cos(tp * Y) * np.sin(tp * Z) +
+            np.sin(2 * tp * Y) * np.cos(tp * Z) * np.sin(tp * X) +
+            np.sin(2 * tp * Z) * np.cos(tp * X) * np.sin(tp * Y))
+
+
 def gyroid_first_harmonic(N, shift_grid, cell):
     """sin x cos y + sin y cos z + sin z cos x, x in units of 2 pi / a."""
     x = [(np.arange(n) + b) / n for n, b in zip(N, shift_grid)]

@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import os
 import threading
 import time
 from concurrent.futures import ThreadPoolExecutor, as_completed
@@ -25,7 +26,7 @@ from . import _lib as L
 from . import stars
 from .mlp import MLP
 from .plans import DevicePlan, auto_L, find_optimal_shift
-from .scft import SCFTSolver, first_star_bcc, gyroid_first_harmonic, symmetrize_subgrid
+from .scft import SCFTSolver, double_gyroid_211, first_star_bcc, symmetrize_subgrid
 from .symmetry_ops import get_ops
 
 GROUPS = (229, 230, 70)  # Im-3m, Ia-3d, Fddd
@@ -73,12 +74,13 @@ class Group:
 
     def cold_fields(self, seed, amp=3.0, noise=0.05):
         """Seeded cold start: the group's lowest harmonic (BCC {110} star for
-        Im-3m, gyroid for Ia-3d, the {111} star for Fddd) +- amp, plus noise."""
+        Im-3m, the double-gyroid {211} star for Ia-3d with w_A low on the
+        networks, the {111} star for Fddd) +- amp, plus noise."""
         b = [s * n for s, n in zip(self.shift, self.N)]
         if self.sg == 229:
             base = first_star_bcc(self.N, b, 1.0)
         elif self.sg == 230:
-            base = gyroid_first_harmonic(self.N, b, 1.0)
+            base = -double_gyroid_211(self.N, b, 1.0)
         else:
             x = [(np.arange(n) + s) / n for n, s in zip(self.N, b)]
             X, Y, Z = np.meshgrid(*x, indexing="ij")
@@ -179,10 +181,19 @@ class WarmStart:
         self.mu = np.zeros(dims[-1], dtype=np.float32)
         self.sd = np.ones(dims[-1], dtype=np.float32)
 
-    def fit(self, X, T, epochs=300, batch=64, lr=1e-3, seed=0):
+    def fit(self, X, T, epochs=300, batch=64, lr=1e-3, seed=0, chunk=25, lr_min=0.01):
+        """Adam on the normalised targets; the learning rate follows a cosine
+        from lr down to lr * lr_min, constant over chunks of `chunk` epochs."""
         self.mu = T.mean(axis=0).astype(np.float32)
         self.sd = (T.std(axis=0) + 1e-6).astype(np.float32)
-        return self.net.train(X, (T - self.mu) / self.sd, epochs=epochs, batch=batch, lr=lr, seed=seed)
+        Tn = (T - self.mu) / self.sd
+        hist = []
+        n_chunks = max(1, -(-epochs // chunk))
+        for k in range(n_chunks):
+            e = min(chunk, epochs - k * chunk)
+            lr_k = lr * (lr_min + (1.0 - lr_min) * 0.5 * (1.0 + np.cos(np.pi * k / n_chunks)))
+            hist.append(self.net.train(X, Tn, epochs=e, batch=batch, lr=float(lr_k), seed=seed + k))
+        return np.concatenate(hist)
 
     def predict(self, X):
         return self.net.forward(X) * self.sd + self.mu
@@ -213,7 +224,9 @@ def evaluate(ws: WarmStart, ps, groups, tol=1e-8, max_iter=600, workers=8):
     def one(i):
         p = ps[i]
         g = groups[p.sg]
-        ic, tc, oc, _ = solve(p, g, None, tol, max_iter)
+        ic, tc, oc, (ca, cb) = solve(p, g, None, tol, max_iter)
+        target = g.encode(ca, cb)
+        coef_err = float(np.linalg.norm(V[i] - target) / max(np.linalg.norm(target), 1e-300))
         t1 = time.perf_counter()
         guess = g.decode(V[i])
         t_dec = time.perf_counter() - t1
@@ -223,7 +236,8 @@ def evaluate(ws: WarmStart, ps, groups, tol=1e-8, max_iter=600, workers=8):
         if fell_back:
             iw += ic
             tw += tc
-        return dict(cold_iters=ic, cold_s=tc, warm_iters=iw, warm_s=tw, fallback=fell_back,
+        return dict(sg=p.sg, coef_err=coef_err, cold_iters=ic, cold_s=tc, warm_iters=iw, warm_s=tw,
+                    fallback=fell_back,
                     dF=abs(ow["F"] - oc["F"]) if not fell_back else 0.0, cold_ok=oc["res"] < tol)
 
     return _run_all(one, list(range(len(ps))), workers, "evaluate", every=25)
@@ -244,6 +258,12 @@ def summary(rows):
         # warm starts that converged to another stationary point than the cold start
         "other_solution": int(sum(r["dF"] > 1e-6 for r in rows)),
         "cold_not_converged": int(sum(not r["cold_ok"] for r in rows)),
+        "coef_relerr_median": float(np.median([r["coef_err"] for r in rows])),
+        "per_group": {str(sg): {"n": len(g), "cold_iters_mean": float(np.mean([r["cold_iters"] for r in g])),
+                                "warm_iters_mean": float(np.mean([r["warm_iters"] for r in g])),
+                                "time_ratio": float(sum(r["warm_s"] for r in g) / sum(r["cold_s"] for r in g))}
+                      for sg in sorted({r["sg"] for r in rows})
+                      for g in [[r for r in rows if r["sg"] == sg]]},
     }
 
 
@@ -254,17 +274,28 @@ def main():
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--out", default="warmstart.npz")
+    ap.add_argument("--data", default=None,
+                    help="training set (.npz): loaded when present, else generated and saved there")
     args = ap.parse_args()
     N = (args.grid,) * 3
     t0 = time.perf_counter()
     ps = draw(args.n_train, seed=1)
-    X, T, ok, its, els, groups = generate(ps, N, workers=args.workers)
+    if args.data and os.path.exists(args.data):
+        d = np.load(args.data, allow_pickle=False)
+        X, T, ok, its = d["X"], d["T"], d["ok"], d["iters"]
+        groups = {sg: Group(sg, N) for sg in GROUPS}
+    else:
+        X, T, ok, its, els, groups = generate(ps, N, workers=args.workers)
+        if args.data:
+            np.savez(args.data, X=X, T=T, ok=ok, iters=its, wall_s=els)
     t1 = time.perf_counter()
     print(json.dumps({"stage": "generate", "n": len(ps), "converged": int(ok.sum()),
                       "iters_mean": float(its.mean()), "wall_s": t1 - t0}), flush=True)
     ws = WarmStart()
-    hist = ws.fit(X[ok], T[ok], epochs=args.epochs)
+    hist = ws.fit(X[ok], T[ok], epochs=args.epochs, batch=args.batch, lr=args.lr)
     t2 = time.perf_counter()
     print(json.dumps({"stage": "train", "epochs": args.epochs, "loss_first": float(hist[0]),
                       "loss_last": float(hist[-1]), "wall_s": t2 - t1}), flush=True)

@@ -402,19 +402,22 @@ int cfft_mlp_train(cfft_mlp* p, const float* X, const float* T, int64_t n, int32
     std::vector<int> perm(n);
     for (long i = 0; i < n; ++i) perm[i] = (int)i;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    DevBuf perm_dev;
+    perm_dev.alloc(4L * n);
     for (int ep = 0; ep < epochs; ++ep) {
+      // the previous epoch's gathers have read the device copy before it is replaced
+      hip_check(hipStreamSynchronize(p->stream), "sync");
       std::shuffle(perm.begin(), perm.end(), rng);
+      hip_check(hipMemcpyAsync(perm_dev.p, perm.data(), 4L * n, hipMemcpyHostToDevice, p->stream), "H2D");
       double lsum = 0.0;
       long seen = 0;
       for (long s0 = 0; s0 < n; s0 += batch) {
         const int rows = (int)std::min<long>(batch, n - s0);
-        hip_check(hipMemcpyAsync(p->idx.p, perm.data() + s0, 4L * rows, hipMemcpyHostToDevice, p->stream), "H2D");
+        const int* idx = static_cast<const int*>(perm_dev.p) + s0;
         hipLaunchKernelGGL(k_gather_rows, dim3(nblk((long)rows * din, 256)), dim3(256), 0, p->stream,
-                           static_cast<const float*>(p->X.p), static_cast<const int*>(p->idx.p),
-                           static_cast<float*>(p->xb.p), rows, din);
+                           static_cast<const float*>(p->X.p), idx, static_cast<float*>(p->xb.p), rows, din);
         hipLaunchKernelGGL(k_gather_rows, dim3(nblk((long)rows * dout, 256)), dim3(256), 0, p->stream,
-                           static_cast<const float*>(p->T.p), static_cast<const int*>(p->idx.p),
-                           static_cast<float*>(p->tb.p), rows, dout);
+                           static_cast<const float*>(p->T.p), idx, static_cast<float*>(p->tb.p), rows, dout);
         hip_check(hipGetLastError(), "k_gather_rows");
         forward(p, static_cast<const float*>(p->xb.p), rows);
         lsum += loss_grad(p, static_cast<const float*>(p->tb.p), rows) * rows;
