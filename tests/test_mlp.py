@@ -92,3 +92,23 @@ def test_device_training_fits_smooth_map():
     assert hist[-1] < hist[0] / 20
     err = np.mean((net.forward(X[1000:]) - Y[1000:]) ** 2)
     assert err < 3 * hist[-1] + 1e-3
+
+
+def test_committed_warm_start_model_file():
+    """The trained warm-start network ships as plain arrays (no pickle): a
+    4-layer MLP 6 -> 512 -> 512 -> 512 -> 256 trained on 2000 solutions at 64^3,
+    whose recorded held-out evaluation met the 0.5x wall-time target."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(__file__), "..", "crystallographicfft.jl_amd", "models",
+                        "warmstart.npz")
+    d = np.load(path, allow_pickle=False)
+    dims = [int(v) for v in d["dims"]]
+    assert dims == [6, 512, 512, 512, 256]
+    n_params = sum(a * b + b for a, b in zip(dims[:-1], dims[1:]))
+    assert d["params"].size == n_params and d["params"].dtype == np.float32
+    assert np.all(np.isfinite(d["params"])) and d["sd"].min() > 0
+    meta = json.loads(str(d["meta"]))
+    assert meta["grid"] == 64 and meta["n_train"] == 2000 and meta["n_stars"] == 64
+    ev = meta["eval"]
+    assert ev["n"] == 200 and ev["fallbacks"] == 0 and ev["time_ratio"] <= 0.5
