@@ -119,6 +119,33 @@ def profile_stages(sw, steps, torch):
     return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(L.STAGES) if cnt[i]}
 
 
+def profiled_traffic(stage_name, prec_name):
+    """HBM bytes per launch of a chain kernel at this workload (2 fields per
+    launch), from the FETCH_SIZE / WRITE_SIZE profiler runs recorded under
+    profiles/ (each counter scaled by what it reads for a plain 64 MiB copy);
+    None when no such record exists."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_im3m128_qr_chain.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh)["kernels"].get(f"{stage_name}/{prec_name}")
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if rec is None else float(rec["bytes"])
+
+
+def live_launch_ms(plan, stage_name, q, E_ptrs, stream, reps=200):
+    """Average duration of one launch of a chain kernel, timed live with two
+    hipEvents on the stream it runs on: `reps` identical launches back to back
+    (cfft_time_stage_dev) on q, which is overwritten."""
+    import ctypes as C
+    import cfft_hip._lib as L
+    stage = L.STAGES.index(stage_name)
+    ms = C.c_double()
+    L.check(L.lib().cfft_time_stage_dev(plan.handle, stage, C.c_void_p(q), E_ptrs, int(reps),
+                                        C.c_void_p(stream), C.byref(ms)))
+    return ms.value
+
+
 def cpu_threads():
     """Host threads for the multi-core CPU leg: the affinity set, at most 16."""
     try:
@@ -264,7 +291,9 @@ def batched_fields(cf, torch, ops_s, wA, prec, e, batch, steps, path):
     L.check(lib.cfft_profile(plan.handle, 0))
     stages = {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(L.STAGES) if cnt[i]}
     dom = max(stages, key=lambda k: stages[k][0])
-    avg = stages[dom][0] / stages[dom][1]
+    esz = 8 if prec == cf.FP64 else 4
+    E_ptrs = (C.c_void_p * batch)(*[E.data_ptr() + b * M3 * esz for b in range(batch)])
+    avg = live_launch_ms(plan, dom, q.data_ptr(), E_ptrs, s)
     nb = stage_bytes(plan, e)[dom]
     return {"batch": batch, "value": batch * steps / el, "unit": "diffusion steps/s",
             "ms_per_step": el / steps * 1e3, "dominant_kernel": dom, "avg_launch_ms": avg,
@@ -418,7 +447,9 @@ def main():
         el = time_sweeps(sw, args.steps, args.warmup, torch)
         stages = profile_stages(sw, args.steps, torch)
         dom = max(stages, key=lambda k: stages[k][0])
-        avg_ms = stages[dom][0] / stages[dom][1]
+        scratch = sw.q.clone()
+        avg_ms = live_launch_ms(sw.plan, dom, scratch.data_ptr(), sw._E1, sw.stream())
+        del scratch
         nb = launch_bytes(sw.plan, e, dom, min(args.steps, NS))
         achieved = nb / (avg_ms * 1e-3) / 1e9
         res[name] = dict(el=el, Qq=Qq, Qd=Qd, q=qfields, stages=stages, dom=dom, avg_ms=avg_ms,
@@ -489,7 +520,9 @@ def main():
                    "grid": list(N), "space_group": SG, "batch_fields": 2, "contour_steps": NS},
         "roofline": {"bound": "hbm", "kernel": r64["dom"], "achieved": r64["achieved"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": r64["sbytes"],
+                     "traffic": (profiled_traffic(r64["dom"], "fp64")
+                                 if variants["fp64"]["used"] == "qr_chain" else None),
+                     "bytes_per_launch": r64["sbytes"],
                      "avg_launch_ms": r64["avg_ms"]},
         "mixed": {"value": mixed_value, "dtype": "f32 data/FFT/propagator, f64 tables+reductions",
                   "speedup_vs_fp64": mixed_value / value, "abs_dQ_vs_fp64": dq,
@@ -498,7 +531,9 @@ def main():
                                "frac": r32["achieved"] / HBM_PEAK_GBS,
                                "avg_launch_ms": r32["avg_ms"]}},
         "Q": {"fp64_q": r64["Qq"], "fp64_qdagger": r64["Qd"], "mixed_q": r32["Qq"]},
-        "stage_ms_per_launch": {k: v[0] / v[1] for k, v in r64["stages"].items()},
+        # one event pair around every launch of a non-graph sweep (includes
+        # the events' own overhead); picks the dominant kernel
+        "stage_ms_per_launch_events": {k: v[0] / v[1] for k, v in r64["stages"].items()},
         "launch_strategy": variants,
         "step_alg_bytes_fp64": r64["step_bytes"],
         "cpu_baseline": cpu,

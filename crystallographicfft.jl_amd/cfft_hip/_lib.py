@@ -39,7 +39,7 @@ EXPORTED = [
     "cfft_forward_g0", "cfft_diffuse", "cfft_step", "cfft_sub_to_full", "cfft_full_to_sub",
     "cfft_forward_dev", "cfft_backward_dev", "cfft_diffuse_dev", "cfft_step_dev",
     "cfft_exp_dev", "cfft_sync", "cfft_step_ptrs_dev", "cfft_profile", "cfft_profile_read",
-    "cfft_sweep_dev", "cfft_plan_set_option",
+    "cfft_time_stage_dev", "cfft_sweep_dev", "cfft_plan_set_option",
     "cfft_calc_asu", "cfft_pack_asu", "cfft_block_plan_create", "cfft_block_plan_info",
     "cfft_block_forward", "cfft_block_inverse", "cfft_block_plan_destroy",
     "cfft_prefold_plan_create", "cfft_prefold_plan_info", "cfft_prefold_forward",
@@ -115,6 +115,7 @@ def _declare(lib):
         "cfft_step_ptrs_dev": (C.c_int, [P, P, C.POINTER(P), I32, P]),
         "cfft_profile": (C.c_int, [P, I32]),
         "cfft_profile_read": (C.c_int, [P, PD, PI64]),
+        "cfft_time_stage_dev": (C.c_int, [P, I32, P, C.POINTER(P), I32, P, PD]),
         "cfft_sweep_dev": (C.c_int, [P, P, C.POINTER(P), C.POINTER(P), PI32, I32, I32, P]),
         "cfft_plan_set_option": (C.c_int, [P, I32, I64]),
         "cfft_calc_asu": (C.c_int, [I32, PI32, PI32, I32, PI32, PI32, PI32, I64, PI64]),

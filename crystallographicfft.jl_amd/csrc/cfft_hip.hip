@@ -909,6 +909,48 @@ void run_steps_hist(cfft_plan* p, T* hist, const ESel& es, int step0, int nsteps
   dispatch_steps<T>(p, hist, p->M3 * p->batch, pp, step0, nsteps, s);
 }
 
+// Average duration of one launch of a chain kernel, measured live: one
+// untimed launch, then `reps` identical launches back to back on `s` between
+// two hipEvents (the same launches a kernel trace lists one by one).
+template <typename T>
+double time_stage(cfft_plan* p, T* q, const ESel& es, int stage, int reps, hipStream_t s) {
+  PipeP pp = pipe_params(p, es);
+  PipeBufs B = pipe_bufs(p, nullptr, q, p->d_F.p, false);
+  auto launch = [&] {
+    if (stage == CFFT_STAGE_UNFOLD_FOLD)
+      launch_p51<T>(p, pp, B, 0, s);
+    else
+      launch_p2<T>(p, pp, B, s);
+  };
+  launch();
+  hipEvent_t a = nullptr, b = nullptr;
+  hip_check(hipEventCreate(&a), "hipEventCreate");
+  hipError_t e = hipEventCreate(&b);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(a);
+    hip_check(e, "hipEventCreate");
+  }
+  float ms = 0.f;
+  e = hipEventRecord(a, s);
+  if (e == hipSuccess) {
+    try {
+      for (int i = 0; i < reps; ++i) launch();
+    } catch (...) {
+      (void)hipStreamSynchronize(s);
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+      throw;
+    }
+    e = hipEventRecord(b, s);
+  }
+  if (e == hipSuccess) e = hipEventSynchronize(b);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  hip_check(e, "stage timing");
+  return (double)ms / reps;
+}
+
 template <typename T>
 void step_host(cfft_plan* p, double* q, const double* w, double dt, int nsteps) {
   to_dev<T>(p, p->d_f0, q, p->M3 * p->batch);
@@ -1279,6 +1321,26 @@ int cfft_sweep_dev(cfft_plan* p, void* q, const void* const* E1, const void* con
       run_steps<double>(p, static_cast<double*>(q), es, step0, nsteps, s);
     else
       run_steps<float>(p, static_cast<float*>(q), es, step0, nsteps, s);
+  });
+}
+
+int cfft_time_stage_dev(cfft_plan* p, int32_t stage, void* q, const void* const* E, int32_t reps,
+                        void* stream, double* ms_per_launch) {
+  return guarded([&] {
+    check_plan(p);
+    require(q && E && ms_per_launch && reps >= 1, CFFT_ERR_INVALID, "bad argument");
+    require(stage == CFFT_STAGE_UNFOLD_FOLD || stage == CFFT_STAGE_FFT_Y, CFFT_ERR_INVALID,
+            "stage must be CFFT_STAGE_UNFOLD_FOLD or CFFT_STAGE_FFT_Y");
+    require(!p->prof, CFFT_ERR_INVALID, "per-launch event profiling is enabled on this plan");
+    require(!is_q_path(p) || is_qr(p), CFFT_ERR_UNSUPPORTED, "plan path has no fused chain kernels");
+    hipStream_t s = pick(p, stream);
+    ESel es;
+    es.mode = 2;
+    es.E1 = E;
+    if (p->prec == CFFT_FP64)
+      *ms_per_launch = time_stage<double>(p, static_cast<double*>(q), es, stage, reps, s);
+    else
+      *ms_per_launch = time_stage<float>(p, static_cast<float*>(q), es, stage, reps, s);
   });
 }
 

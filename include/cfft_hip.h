@@ -204,6 +204,14 @@ int cfft_profile(cfft_plan* plan, int32_t enable);
 /* Sum of the recorded kernel durations (ms_out[CFFT_N_STAGES]) and launch
  * counts per stage since the last read; clears the record. */
 int cfft_profile_read(cfft_plan* plan, double* ms_out, int64_t* count_out);
+/* Average duration of one launch of a chain kernel (no counterpart in the
+ * reference): stage CFFT_STAGE_UNFOLD_FOLD (P5 fused with P1) or
+ * CFFT_STAGE_FFT_Y (P2), launched once untimed and then reps times back to
+ * back on stream between two hipEvents, on the device field q (batch * M^3
+ * entries, overwritten) with interaction factors E[b] (device pointers).
+ * Plans on the fused-chain paths only (M2, M7, QR); profiling must be off. */
+int cfft_time_stage_dev(cfft_plan* plan, int32_t stage, void* q, const void* const* E, int32_t reps,
+                        void* stream, double* ms_per_launch);
 
 /* Propagator sweep keeping every contour point (SCFT densities; no
  * counterpart in the reference).  hist holds nsteps + 1 slices of batch

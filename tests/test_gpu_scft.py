@@ -52,6 +52,45 @@ def test_sweep_fp64_vs_oracle(sg, N, persistent, fuse, graph):
     assert abs(Q1 - float(np.mean(rq))) / abs(Q1) < 1e-12
 
 
+@pytest.mark.parametrize("path", [cf.PATH_AUTO, cf.PATH_QR])
+def test_live_stage_timing(path):
+    """cfft_time_stage_dev: positive per-launch times for P5+P1 and P2,
+    other stages rejected, and the plan still sweeps to the oracle afterwards
+    (the timed launches only touch the scratch field and the channel array)."""
+    import ctypes as C
+    import torch
+    from cfft_hip.scft import PropagatorSweep
+    N, sg = (32, 32, 32), 229
+    ops, ops_s, shift, R, t = group_setup(sg, N)
+    Lv = O.auto_L(t)
+    wA = 2.0 * O.extract_subgrid(O.make_symmetric(R, t, N, seed=3), Lv)
+    wB = -1.5 * O.extract_subgrid(O.make_symmetric(R, t, N, seed=4), Lv)
+    sw = PropagatorSweep(N, ops_s, 4.0 * np.eye(3), 0.3, 20, cf.FP64, path=path)
+    sw.set_fields(wA, wB)
+    lib = cf._lib.lib()
+    scratch = sw.q.clone()
+    for name in ("p51_unfold_fold", "p2_fft_yz"):
+        ms = C.c_double()
+        cf._lib.check(lib.cfft_time_stage_dev(sw.plan.handle, cf._lib.STAGES.index(name),
+                                              C.c_void_p(scratch.data_ptr()), sw._E1, 5,
+                                              C.c_void_p(sw.stream()), C.byref(ms)))
+        assert 0.0 < ms.value < 10.0
+    with pytest.raises(cf.CFFTError):
+        cf._lib.check(lib.cfft_time_stage_dev(sw.plan.handle, cf._lib.STAGES.index("p3_recon"),
+                                              C.c_void_p(scratch.data_ptr()), sw._E1, 5,
+                                              C.c_void_p(sw.stream()), C.byref(ms)))
+    sw.reset()
+    sw.run()
+    torch.cuda.synchronize()
+    q, _ = sw.fields()
+    spec = O.calc_spectral_asu(R, t, N)
+    table = O.m2_backward_table(spec, R, t, N)
+    lat = 4.0 * np.eye(3)
+    diffuse = lambda x: O.m2_scft(x, spec, R, t, N, 1.0 / 20, lat, table)
+    ref = O.propagate(np.ones_like(wA), lambda j: wA if j < 6 else wB, 20, 1.0 / 20, diffuse)[-1]
+    assert np.max(np.abs(q - ref)) / np.max(np.abs(ref)) < 1e-12
+
+
 def test_sweep_mixed_vs_fp64():
     q, qd, rq, rqd, (Q1, Q2) = sweep_case(229, (32, 32, 32), 0.3, 20, cf.MIXED)
     assert np.max(np.abs(q - rq)) / np.max(np.abs(rq)) < 2e-5
