@@ -45,42 +45,75 @@ def draw_instances(n, seed, sizes=(32, 48, 64, 96), groups=(47, 70, 223, 229, 23
     return out
 
 
+# candidate first harmonics, by |h|^2: the start uses the first whose
+# symmetrised cosine (or sine) does not vanish under the group
+_STARS = ((1, 0, 0), (1, 1, 0), (1, 1, 1), (2, 0, 0), (2, 1, 0), (2, 1, 1), (2, 2, 0), (2, 2, 1), (3, 1, 0))
+
+
+def lowest_star(N, ops_s, shift, Lv):
+    """Symmetrised cos 2 pi h.x (or sin) for the smallest |h| whose star is not
+    extinct in the group (x in cell units at (i + b)/N), normalised to unit RMS,
+    on the subgrid, and h.  {100} is extinct in Pm-3n, Fddd, Im-3m and Ia-3d, so
+    a fixed cos x + cos y + cos z start would symmetrise to zero there."""
+    x = [(np.arange(n) + s * n) / n for n, s in zip(N, shift)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    for h in _STARS:
+        arg = 2 * np.pi * (h[0] * X + h[1] * Y + h[2] * Z)
+        for fn in (np.cos, np.sin):
+            u = symmetrize_subgrid(fn(arg), ops_s, Lv)
+            rms = float(np.sqrt(np.mean(u * u)))
+            if rms > 1e-3:
+                return u / rms, h
+    raise ValueError("no harmonic up to |h|^2 = 10 survives the group average")
+
+
 def start_fields(inst: Instance, ops_s, shift, amp=2.0, noise=0.1):
-    """Symmetrised cosine-plus-noise start: w_A = +amp c + n_A, w_B = -amp c + n_B
-    with c = cos 2 pi x + cos 2 pi y + cos 2 pi z (x in cell units), on the subgrid."""
+    """Seeded cosine-plus-noise start: w_A = +amp c + n_A, w_B = -amp c + n_B,
+    c = the group's lowest non-extinct star (unit RMS), n_A, n_B symmetrised
+    seeded N(0, noise) fields; on the subgrid."""
     N = inst.N
     Lv = auto_L(ops_s)
     rng = np.random.default_rng(inst.seed)
-    x = [(np.arange(n) + s * n) / n for n, s in zip(N, shift)]
-    X, Y, Z = np.meshgrid(*x, indexing="ij")
-    c = np.cos(2 * np.pi * X) + np.cos(2 * np.pi * Y) + np.cos(2 * np.pi * Z)
-    wA = symmetrize_subgrid(amp * c + noise * rng.standard_normal(N), ops_s, Lv)
-    wB = symmetrize_subgrid(-amp * c + noise * rng.standard_normal(N), ops_s, Lv)
+    c, _ = lowest_star(N, ops_s, shift, Lv)
+    wA = amp * c + symmetrize_subgrid(noise * rng.standard_normal(N), ops_s, Lv)
+    wB = -amp * c + symmetrize_subgrid(noise * rng.standard_normal(N), ops_s, Lv)
     return wA, wB
 
 
-def _prepare(inst: Instance, precision, m_hist):
+def _prepare(inst: Instance, precision, m_hist, finish):
     ops = get_ops(inst.sg, 3, inst.N)
     shift, ops_s = find_optimal_shift(ops, inst.N)
-    solver = SCFTSolver(inst.N, ops_s, inst.cell * np.eye(3), inst.f, inst.chiN, inst.ns, precision,
-                        path=L.PATH_AUTO, m_hist=m_hist)
-    wA, wB = start_fields(inst, ops_s, shift)
-    return solver, (wA, wB)
+    mk = lambda prec: SCFTSolver(inst.N, ops_s, inst.cell * np.eye(3), inst.f, inst.chiN, inst.ns, prec,
+                                 path=L.PATH_AUTO, m_hist=m_hist)
+    solver = mk(precision)
+    fin = mk(L.FP64) if finish else None
+    return solver, fin, start_fields(inst, ops_s, shift)
 
 
-def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8, m_hist=10):
-    """Plan every instance, then solve all of them concurrently.  Returns
-    (results, plan_seconds, solve_seconds); results[i] = dict(iters, F, res, Q)."""
+def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8, m_hist=10,
+                finish_tol=None):
+    """Plan every instance, then solve all of them concurrently.  With
+    finish_tol, each instance continues in FP64 from its `precision` solution
+    until the residual is below finish_tol (its FP64 plan is made in the
+    planning phase).  Returns (results, plan_seconds, solve_seconds);
+    results[i] = dict(iters, F, res, Q[, iters_finish])."""
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=workers) as ex:
-        prepared = list(ex.map(lambda inst: _prepare(inst, precision, m_hist), instances))
+        prepared = list(ex.map(lambda inst: _prepare(inst, precision, m_hist, finish_tol is not None),
+                               instances))
     t1 = time.perf_counter()
 
     def run(item):
-        solver, (wA, wB) = item
+        solver, fin, (wA, wB) = item
         solver.set_fields(wA, wB)
         it, out = solver.solve(tol=tol, max_iter=max_iter)
-        return dict(iters=it, F=out["F"], res=out["res"], Q=out["Q"])
+        r = dict(iters=it, F=out["F"], res=out["res"], Q=out["Q"])
+        if fin is not None:
+            fin.set_fields(*solver.fields())
+            it2, o2 = fin.solve(tol=finish_tol, max_iter=max_iter)
+            r.update(iters_finish=it2, F=o2["F"], res=o2["res"], Q=o2["Q"], F_first=out["F"],
+                     res_first=out["res"])
+        return r
 
     with ThreadPoolExecutor(max_workers=workers) as ex:
         results = list(ex.map(run, prepared))

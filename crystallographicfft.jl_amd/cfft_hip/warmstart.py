@@ -15,6 +15,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import sys
 import threading
 import time
 from concurrent.futures import ThreadPoolExecutor, as_completed
@@ -113,8 +114,8 @@ _tls = threading.local()
 
 
 def _run_all(fn, items, workers, stage, every=100):
-    """fn over items on a thread pool, in order; prints a progress line each
-    time another `every` items have finished."""
+    """fn over items on a thread pool, in order; prints a progress line to
+    stderr each time another `every` items have finished."""
     out = [None] * len(items)
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=workers) as ex:
@@ -123,7 +124,7 @@ def _run_all(fn, items, workers, stage, every=100):
             out[futs[fut]] = fut.result()
             if k % every == 0 or k == len(items):
                 print(json.dumps({"stage": stage, "done": k, "of": len(items),
-                                  "wall_s": round(time.perf_counter() - t0, 2)}), flush=True)
+                                  "wall_s": round(time.perf_counter() - t0, 2)}), file=sys.stderr, flush=True)
     return out
 
 

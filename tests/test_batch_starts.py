@@ -1,0 +1,45 @@
+"""Start fields of the batched-instance workload: the lowest star that the
+group does not extinguish (host code; the symmetrisation is the gather form of
+test/test_helpers.jl:11-29)."""
+import numpy as np
+import pytest
+
+from cfft_hip import batch
+from cfft_hip.plans import auto_L, find_optimal_shift
+from cfft_hip.scft import symmetrize_subgrid
+from cfft_hip.symmetry_ops import get_ops
+
+# {100} survives only in Pmmm; {110} is allowed in Pm-3n (hhl: l even) and
+# Im-3m (h+k+l even); {111} is the first Fddd star (all odd); Ia-3d extinguishes
+# {110} (hhl: 2h+l = 4n) and {200} (00l: l = 4n), leaving {211}
+EXPECTED = {47: (1, 0, 0), 223: (1, 1, 0), 229: (1, 1, 0), 70: (1, 1, 1), 230: (2, 1, 1)}
+
+
+@pytest.mark.parametrize("sg", sorted(EXPECTED))
+def test_lowest_star_is_first_allowed_reflection(sg):
+    N = (16, 16, 16)
+    ops = get_ops(sg, 3, N)
+    shift, ops_s = find_optimal_shift(ops, N)
+    Lv = auto_L(ops_s)
+    u, h = batch.lowest_star(N, ops_s, shift, Lv)
+    assert tuple(h) == EXPECTED[sg]
+    assert abs(np.sqrt(np.mean(u * u)) - 1.0) < 1e-12
+    # every smaller candidate star symmetrises to zero (extinct), cos and sin
+    x = [(np.arange(n) + s * n) / n for n, s in zip(N, shift)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    for k in batch._STARS[:batch._STARS.index(tuple(h))]:
+        arg = 2 * np.pi * (k[0] * X + k[1] * Y + k[2] * Z)
+        for fn in (np.cos, np.sin):
+            assert np.max(np.abs(symmetrize_subgrid(fn(arg), ops_s, Lv))) < 1e-12
+
+
+def test_start_fields_seeded_and_antisymmetric_in_the_harmonic():
+    inst = batch.Instance((16, 16, 16), 229, 4.0, 0.3, 15.0, 20, 7)
+    ops = get_ops(inst.sg, 3, inst.N)
+    shift, ops_s = find_optimal_shift(ops, inst.N)
+    a1, b1 = batch.start_fields(inst, ops_s, shift, amp=2.0, noise=0.0)
+    assert np.allclose(a1, -b1, atol=1e-14)
+    a2, b2 = batch.start_fields(inst, ops_s, shift)
+    a3, b3 = batch.start_fields(inst, ops_s, shift)
+    assert np.array_equal(a2, a3) and np.array_equal(b2, b3)
+    assert 0.0 < np.max(np.abs(a2 - a1)) < 1.0
