@@ -65,6 +65,8 @@ def stage_bytes(plan, e):
         "qmul": B * ((min(chan, plan.n_rep * plan.d * 2 * e) + rep) if qr else 2 * chan),
         # channels, E -> f0 (q(s+1)) -> channels
         "p51_unfold_fold": B * (2 * chan + 2 * M3 * e),
+        # (QS) channels -> FFT y, z, Q multiply, IFFT z, y -> channels
+        "p2q_plane_pair": B * 2 * chan,
     }
 
 
@@ -103,6 +105,21 @@ def time_sweeps(sw, steps, warmup, torch):
     return time.perf_counter() - t0
 
 
+def median_sweep_ms(sw, torch, trials=30, warmup=5):
+    """The reference's timing method (median of 30 trials after 5 warm-ups)
+    applied to one full contour sweep of Ns steps: median ms per step."""
+    times = []
+    for i in range(warmup + trials):
+        sw.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sw.run()
+        torch.cuda.synchronize()
+        if i >= warmup:
+            times.append((time.perf_counter() - t0) / NS * 1e3)
+    return float(np.median(times))
+
+
 def profile_stages(sw, steps, torch):
     import cfft_hip._lib as L
     import ctypes as C
@@ -119,12 +136,15 @@ def profile_stages(sw, steps, torch):
     return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(L.STAGES) if cnt[i]}
 
 
-def profiled_traffic(stage_name, prec_name):
+STRATEGY_PATH = {"m7_chain": 0, "m7_persistent": 0, "qr_chain": 5, "qs_chain": 6}
+
+
+def profiled_traffic(strategy, stage_name, prec_name):
     """HBM bytes per launch of a chain kernel at this workload (2 fields per
     launch), from the FETCH_SIZE / WRITE_SIZE profiler runs recorded under
-    profiles/ (each counter scaled by what it reads for a plain 64 MiB copy);
-    None when no such record exists."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_im3m128_qr_chain.json")
+    profiles/ for that launch strategy (each counter scaled by what it reads
+    for a plain 64 MiB copy); None when no such record exists."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_im3m128_{strategy}.json")
     try:
         with open(path) as fh:
             rec = json.load(fh)["kernels"].get(f"{stage_name}/{prec_name}")
@@ -174,15 +194,19 @@ def cpu_baseline(ops_s, wA, Lv, budget_s):
     for nt in (1, cpu_threads()):
         cs.set_threads(nt)
         q = np.ones(cs.M, order="F")
-        q = cs.step(q, E)
+        for _ in range(5):  # warm-ups, as in the reference's benches
+            q = cs.step(q, E)
         n = 0
+        dts = []
         t0 = time.perf_counter()
         while True:
+            t1 = time.perf_counter()
             q = cs.step(q, E)
+            dts.append(time.perf_counter() - t1)
             n += 1
             if time.perf_counter() - t0 > budget_s:
                 break
-        out[nt] = (n / (time.perf_counter() - t0), n, setup)
+        out[nt] = (n / (time.perf_counter() - t0), n, setup, 1.0 / float(np.median(dts)))
     return out
 
 
@@ -446,7 +470,7 @@ def main():
         trial = {}
         sweeps = {}
         for label, path, persist in (("m7_chain", cf.PATH_AUTO, 0), ("m7_persistent", cf.PATH_AUTO, 1),
-                                     ("qr_chain", cf.PATH_QR, 0)):
+                                     ("qr_chain", cf.PATH_QR, 0), ("qs_chain", cf.PATH_QS, 0)):
             sw = cf.scft.PropagatorSweep(N, ops_s, lat, F_A, NS, prec, path=path)
             sw.set_fields(wA, wB)
             sw.plan.set_option(cf._lib.OPT_PERSISTENT, persist)
@@ -466,6 +490,7 @@ def main():
         Qq, Qd = sw.Q()
         qfields = sw.fields()
         el = time_sweeps(sw, args.steps, args.warmup, torch)
+        med = median_sweep_ms(sw, torch)
         stages = profile_stages(sw, args.steps, torch)
         dom = max(stages, key=lambda k: stages[k][0])
         scratch = sw.q.clone()
@@ -473,7 +498,7 @@ def main():
         del scratch
         nb = launch_bytes(sw.plan, e, dom, min(args.steps, NS))
         achieved = nb / (avg_ms * 1e-3) / 1e9
-        res[name] = dict(el=el, Qq=Qq, Qd=Qd, q=qfields, stages=stages, dom=dom, avg_ms=avg_ms,
+        res[name] = dict(el=el, med=med, Qq=Qq, Qd=Qd, q=qfields, stages=stages, dom=dom, avg_ms=avg_ms,
                          achieved=achieved, plan=sw.plan, sbytes=nb,
                          step_bytes=step_bytes(sw.plan, e))
         del sw
@@ -483,7 +508,8 @@ def main():
     if args.wide > 0:
         wide = {}
         for name, prec, e in (("fp64", cf.FP64, 8), ("mixed", cf.MIXED, 4)):
-            wide[name] = batched_fields(cf, torch, ops_s, wA, prec, e, args.wide, 50, cf.PATH_QR)
+            wide[name] = batched_fields(cf, torch, ops_s, wA, prec, e, args.wide, 50,
+                                        STRATEGY_PATH[variants[name]["used"]])
 
     # ---- SCFT: Im-3m 128^3 plain cold start, Ia-3d 128^3 mixed vs FP64 ----
     scft = None
@@ -517,7 +543,7 @@ def main():
         nt = max(rates)
         r1, rn = rates[1], rates[nt]
         cpu = {"value": rn[0], "unit": "diffusion steps/s", "cores": nt, "kind": "port",
-               "value_1thread": r1[0],
+               "value_1thread": r1[0], "median_step_rate": rn[3], "median_step_rate_1thread": r1[3],
                "sample": f"C restatement of the oracle's Strang step (oracle/cstep.c, FP64, "
                          f"OpenMP): {rn[1]} steps at {nt} threads and {r1[1]} steps at 1 thread "
                          f"of q on the A block, ~{args.cpu_seconds:.0f}s each; table setup "
@@ -541,8 +567,7 @@ def main():
                    "grid": list(N), "space_group": SG, "batch_fields": 2, "contour_steps": NS},
         "roofline": {"bound": "hbm", "kernel": r64["dom"], "achieved": r64["achieved"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS,
-                     "traffic": (profiled_traffic(r64["dom"], "fp64")
-                                 if variants["fp64"]["used"] == "qr_chain" else None),
+                     "traffic": profiled_traffic(variants["fp64"]["used"], r64["dom"], "fp64"),
                      "bytes_per_launch": r64["sbytes"],
                      "avg_launch_ms": r64["avg_ms"]},
         "mixed": {"value": mixed_value, "dtype": "f32 data/FFT/propagator, f64 tables+reductions",
@@ -552,6 +577,9 @@ def main():
                                "frac": r32["achieved"] / HBM_PEAK_GBS,
                                "avg_launch_ms": r32["avg_ms"]}},
         "Q": {"fp64_q": r64["Qq"], "fp64_qdagger": r64["Qd"], "mixed_q": r32["Qq"]},
+        # the reference's bench method on full sweeps: median of 30 after 5 warm-ups
+        "median_of_30": {"fp64_ms_per_step": r64["med"], "fp64_steps_per_s": 2e3 / r64["med"],
+                         "mixed_ms_per_step": r32["med"], "mixed_steps_per_s": 2e3 / r32["med"]},
         # one event pair around every launch of a non-graph sweep (includes
         # the events' own overhead); picks the dominant kernel
         "stage_ms_per_launch_events": {k: v[0] / v[1] for k, v in r64["stages"].items()},

@@ -5,7 +5,7 @@ and calls the C ABI of ``libcfft_hip.so`` (include/cfft_hip.h).  Julia's
 mutating ``f!`` functions are exposed as ``f`` and mutate their arguments.
 """
 from ._lib import (CFFTError, CFFTLibraryMissing, FP64, MIXED, PATH_AUTO, PATH_M2, PATH_M2Q,
-                   PATH_M7, PATH_M7Q, PATH_QR, lib)
+                   PATH_M7, PATH_M7Q, PATH_QR, PATH_QS, lib)
 from .symmetry_ops import (CentA, CentC, CentF, CentI, CentP, CenteringType, SymOp, apply_op,
                            check_shift_invariance, convert_op, detect_centering_type, dual_ops,
                            get_ops, ops_arrays, pack_ops, strip_centering)

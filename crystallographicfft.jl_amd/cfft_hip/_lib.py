@@ -28,7 +28,7 @@ CFFT_ERR_BOUNDS = 10
 CFFT_ERR_UNSUPPORTED = 11
 CFFT_ERR_HIP = 12
 
-PATH_AUTO, PATH_M2, PATH_M7, PATH_M2Q, PATH_M7Q, PATH_QR = 0, 1, 2, 3, 4, 5
+PATH_AUTO, PATH_M2, PATH_M7, PATH_M2Q, PATH_M7Q, PATH_QR, PATH_QS = 0, 1, 2, 3, 4, 5, 6
 FP64, MIXED = 0, 1
 
 EXPORTED = [
@@ -57,7 +57,7 @@ OPT_FUSE = 2
 OPT_GRAPH = 3
 
 STAGES = ["p1_fold_fftx", "fft_x", "p2_fft_yz", "fft_z", "p3_recon", "p4_irecon_ifft_zy", "ifft_x",
-          "ifft_y", "ifft_z", "p5_ifftx_unfold", "qmul", "sweep", "p51_unfold_fold"]
+          "ifft_y", "ifft_z", "p5_ifftx_unfold", "qmul", "sweep", "p51_unfold_fold", "p2q_plane_pair"]
 
 
 class CFFTError(RuntimeError):

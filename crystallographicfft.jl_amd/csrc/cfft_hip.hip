@@ -40,6 +40,8 @@ struct cfft_plan {
   int chmap[8];
   long H3 = 0, M3 = 0, n_alive = 0, N3 = 0;
   long n_rep = 0;  // orbit representatives of the alive M-grid (path QR)
+  int qs_items = 0;  // plane-pair items per batch entry (path QS)
+  int qs_sep = 0;    // path QS: translations separable in the sign pattern
   size_t esz = 8;  // element size of the plan's real type
   hipStream_t stream = nullptr;
   // device tables
@@ -47,6 +49,7 @@ struct cfft_plan {
   // device work buffers
   DevBuf d_f0, d_ch, d_ch2, d_F, d_E, d_u;
   DevBuf d_qmap, d_grep;  // path QR: element -> (rep, phase); rep values
+  DevBuf d_qsep;          // path QS: axis factors F[d][s][q]
   std::vector<int32_t> fmap_host;
   // options (cfft_plan_set_option)
   int persistent = 0;  // contour sweeps as one cooperative launch
@@ -79,9 +82,14 @@ struct cfft_plan {
 namespace {
 
 bool is_qr(const cfft_plan* p) { return p->path == CFFT_PATH_QR; }
+bool is_qs(const cfft_plan* p) { return p->path == CFFT_PATH_QS; }
 bool is_q_path(const cfft_plan* p) {
-  return p->path == CFFT_PATH_M2Q || p->path == CFFT_PATH_M7Q || p->path == CFFT_PATH_QR;
+  return p->path == CFFT_PATH_M2Q || p->path == CFFT_PATH_M7Q || p->path == CFFT_PATH_QR ||
+         p->path == CFFT_PATH_QS;
 }
+// paths whose steps are the fused launch chain (P51 between steps)
+bool chain_path(const cfft_plan* p) { return !is_q_path(p) || is_qr(p) || is_qs(p); }
+
 
 template <typename T>
 void upload_complex(DevBuf& buf, const std::vector<double>& re, const std::vector<double>& im) {
@@ -101,6 +109,75 @@ void upload_real(DevBuf& buf, const std::vector<double>& v) {
 void upload_bytes(DevBuf& buf, const void* src, size_t n) {
   buf.alloc(n);
   if (n) hip_check(hipMemcpy(buf.p, src, n, hipMemcpyHostToDevice), "upload");
+}
+
+// Path QS axis factors F[d][s][q], q < M_d, s = sign pattern of the
+// representative op (bit i: R_ii = -1):
+//   F_d[s][q] = sum_{a in {0,1}} K_d(q + M_d a) exp(-2 pi i (q + M_d a) t_{s,d} / N_d),
+// K_d(h) = exp(-(2 pi hc / A_dd)^2 ds), hc the centred frequency; 1/8 is folded
+// into F_0.  Their product over d is Q_s(q) of the closed form used by the
+// other Q paths (first row of B^-1 diag(K) B, src/q_fused_krfft.jl:182-413)
+// when the cell is orthogonal and every R_s is diagonal.
+void build_qs_table(cfft_plan* p) {
+  const double* A = p->lattice;  // column-major 3 x 3
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      require(i == j || A[i + 3 * j] == 0.0, CFFT_ERR_UNSUPPORTED, "path QS needs an orthogonal cell");
+  int tsel[8][3];
+  bool seen[8] = {false, false, false, false, false, false, false, false};
+  for (int b = 0; b < p->d; ++b) {
+    const Op& g = p->ops[p->rep[b]];
+    int sf = 0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        const int v = g.R[i][j];
+        require(i == j ? (v == 1 || v == -1) : v == 0, CFFT_ERR_UNSUPPORTED,
+                "path QS needs diagonal representative ops");
+      }
+    for (int i = 0; i < 3; ++i) sf |= (g.R[i][i] < 0 ? 1 : 0) << i;
+    require(!seen[sf], CFFT_ERR_UNSUPPORTED, "path QS needs 8 distinct sign patterns");
+    seen[sf] = true;
+    for (int i = 0; i < 3; ++i) tsel[sf][i] = g.t[i];
+  }
+  // translation component d a function of sign bit d alone (mirrors with the
+  // magic shift: t_d = -1 on flipped axes): the octet sum factors per axis
+  p->qs_sep = 1;
+  for (int sf = 0; sf < 8; ++sf)
+    for (int d = 0; d < 3; ++d)
+      if ((((long)tsel[sf][d] - tsel[sf & (1 << d)][d]) % p->N[d]) != 0) p->qs_sep = 0;
+  const int Ms = p->M[0];
+  std::vector<double> re((size_t)3 * 8 * Ms), im(re.size());
+  for (int d = 0; d < 3; ++d) {
+    const double bd = 2.0 * M_PI / A[d + 3 * d];
+    for (int sf = 0; sf < 8; ++sf)
+      for (int q = 0; q < p->M[d]; ++q) {
+        double sr = 0.0, si = 0.0;
+        for (int a = 0; a < 2; ++a) {
+          const long h = q + (long)p->M[d] * a;
+          const double kv = bd * k_centered((int)h, p->N[d]);
+          const double kd = std::exp(-kv * kv * p->ds);
+          const long long j = (long long)h * tsel[sf][d] * (p->P / p->N[d]);
+          double cr, ci;
+          cis2pi(-j, p->P, cr, ci);
+          sr += kd * cr;
+          si += kd * ci;
+        }
+        const double sc = d == 0 ? 0.125 : 1.0;
+        re[((size_t)d * 8 + sf) * Ms + q] = sr * sc;
+        im[((size_t)d * 8 + sf) * Ms + q] = si * sc;
+      }
+  }
+  if (!p->d_qsep.p) {
+    upload_complex<double>(p->d_qsep, re, im);
+  } else {
+    // same allocation: captured chains keep pointing at it
+    std::vector<cx<double>> h(re.size());
+    for (size_t i = 0; i < re.size(); ++i) h[i] = {re[i], im[i]};
+    hip_check(hipMemcpy(p->d_qsep.p, h.data(), p->d_qsep.bytes, hipMemcpyHostToDevice), "upload QS factors");
+  }
+  p->qs_items = 0;
+  // orbits of x -> xbar: xbar = (H - x) mod H (offset 0) or H - 1 - x (offset 1)
+  for (int c = 0; c < p->n_ch; ++c) p->qs_items += p->offs[c][0] ? (p->H[0] + 1) / 2 : p->H[0] / 2 + 1;
 }
 
 // Phase table W[j] = exp(2 pi i j/P) and fold twiddle tables (the FFT
@@ -130,7 +207,7 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
                 const double* lattice, int path, int prec, int batch) {
   require(batch >= 1, CFFT_ERR_INVALID, "batch must be >= 1");
   require(prec == CFFT_FP64 || prec == CFFT_MIXED, CFFT_ERR_INVALID, "unknown precision");
-  require(path >= CFFT_PATH_AUTO && path <= CFFT_PATH_QR, CFFT_ERR_UNKNOWN_METHOD,
+  require(path >= CFFT_PATH_AUTO && path <= CFFT_PATH_QS, CFFT_ERR_UNKNOWN_METHOD,
           "Unknown method: path " + std::to_string(path));
   for (int d = 0; d < 3; ++d) require(N[d] >= 1, CFFT_ERR_INVALID, "grid size must be positive");
   p->ops = ops;
@@ -158,6 +235,7 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
             "M7 requires a centred lattice (I/F/C/A); use the M2 path for P-centring.");
     require(allL2, CFFT_ERR_CENTERING, "M7 requires L = (2,2,2)");
   }
+  if (path == CFFT_PATH_QS) require(allL2 && p->d == 8, CFFT_ERR_UNSUPPORTED, "path QS needs L = (2,2,2)");
   p->path = path;
   for (int d = 0; d < 3; ++d) {
     require(p->M[d] % 2 == 0, CFFT_ERR_ODD_M,
@@ -168,7 +246,7 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
             "FFT length " + std::to_string(p->H[d]) + " not supported (need 2^a 3^b <= 64)");
   }
   const bool centred_fold = path == CFFT_PATH_M7 || path == CFFT_PATH_M7Q ||
-                            (path == CFFT_PATH_QR && p->cent != CENT_P && allL2);
+                            ((path == CFFT_PATH_QR || path == CFFT_PATH_QS) && p->cent != CENT_P && allL2);
   p->offs = centred_fold ? alive_offsets(p->cent) : alive_offsets(CENT_P);
   p->n_ch = (int)p->offs.size();
   for (int i = 0; i < 8; ++i) p->chmap[i] = -1;
@@ -215,7 +293,10 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
     for (int a1 = 0; a1 < p->L[1]; ++a1)
       for (int a0 = 0; a0 < p->L[0]; ++a0) alphas.push_back({a0, a1, a2});
 
-  if (!is_q_path(p)) {
+  if (is_qs(p)) {
+    require(p->H[0] == p->H[1] && p->H[1] == p->H[2], CFFT_ERR_UNSUPPORTED, "path QS needs a cubic grid");
+    build_qs_table(p);
+  } else if (!is_q_path(p)) {
     ReverseLookup rl = reverse_lookup(p->spec, ops);
     std::vector<uint32_t> tab((size_t)8 * p->n_alive, 0xFFFFFu);
     for (long e = 0; e < p->n_alive; ++e) {
@@ -495,6 +576,8 @@ PipeP pipe_params(cfft_plan* p, const ESel& es) {
   pp.scale_unfold = 1.0 / (8.0 * (double)p->H3);
   pp.scale_irecon = 1.0 / p->d;
   pp.n_rep = p->n_rep;
+  pp.qs_items = p->qs_items;
+  pp.qs_sep = p->qs_sep;
   return pp;
 }
 
@@ -511,6 +594,7 @@ PipeBufs pipe_bufs(cfft_plan* p, const void* f0_in, void* f0_out, void* F, bool 
   B.btab = static_cast<const uint32_t*>(p->d_btab.p);
   B.grep = p->d_grep.p;
   B.qmap = static_cast<const uint32_t*>(p->d_qmap.p);
+  B.qsep = p->d_qsep.p;
   return B;
 }
 
@@ -626,6 +710,18 @@ void launch_qr(cfft_plan* p, const PipeP& pp, const PipeBufs& B, hipStream_t s) 
   launch_check<T>("k_p4r");
 }
 
+// Path QS middle of a step: one kernel per plane pair.
+template <typename T>
+void launch_p2q(cfft_plan* p, const PipeP& pp, const PipeBufs& B, hipStream_t s) {
+  StageTimer st(p, s, CFFT_STAGE_PLANE_Q);
+  const unsigned g = (unsigned)((long)p->qs_items * p->batch);
+#define CFFT_P2Q(L) \
+  case L: hipLaunchKernelGGL((k_p2q<T, double, L>), dim3(g), dim3(kNT), 0, s, pp, B); break;
+  CFFT_LEN_SWITCH(p->H[1], CFFT_P2Q)
+#undef CFFT_P2Q
+  launch_check<T>("k_p2q");
+}
+
 template <typename T>
 void launch_p51(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipStream_t s) {
   StageTimer st(p, s, CFFT_STAGE_UNFOLD_FOLD);
@@ -663,6 +759,13 @@ void run_backward(cfft_plan* p, const PipeP& pp, const cx<T>* F, T* f0, int step
 // of `pp` on both sides: out <- E D (E in).  `in` may equal `out`.
 template <typename T>
 void run_diffuse(cfft_plan* p, const T* in, T* out_f0, const PipeP& pp, int step, hipStream_t s) {
+  if (is_qs(p)) {
+    PipeBufs B = pipe_bufs(p, in, out_f0, nullptr, false);
+    launch_p1<T>(p, pp, B, step, s);
+    launch_p2q<T>(p, pp, B, s);
+    launch_p5<T>(p, pp, B, step, s);
+    return;
+  }
   if (is_qr(p)) {
     run_forward_channels<T>(p, in, pp, step, s);
     PipeBufs B = pipe_bufs(p, nullptr, out_f0, nullptr, false);
@@ -810,6 +913,21 @@ void exp_dev(cfft_plan* p, const T* w, T* E, double factor, long n, hipStream_t 
 template <typename T>
 void run_chain(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int nsteps, hipStream_t s) {
   cx<T>* F = static_cast<cx<T>*>(p->d_F.p);
+  if (is_qs(p)) {
+    // per step: the plane-pair kernel, then P5 of the step fused with P1 of the next
+    launch_p1<T>(p, pp, pipe_bufs(p, q, nullptr, nullptr, false), step0, s);
+    for (int i = 0; i < nsteps; ++i) {
+      PipeBufs B = pipe_bufs(p, q + (i + 1) * stride, q + (i + 1) * stride, F, false);
+      launch_p2q<T>(p, pp, B, s);
+      if (i + 1 < nsteps && p->fuse) {
+        launch_p51<T>(p, pp, B, step0 + i, s);
+      } else {
+        launch_p5<T>(p, pp, B, step0 + i, s);
+        if (i + 1 < nsteps) launch_p1<T>(p, pp, B, step0 + i + 1, s);
+      }
+    }
+    return;
+  }
   run_forward_channels<T>(p, q, pp, step0, s);
   for (int i = 0; i < nsteps; ++i) {
     PipeBufs B = pipe_bufs(p, nullptr, q + (i + 1) * stride, F, false);
@@ -833,7 +951,7 @@ void run_chain(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int 
 // the Q paths).
 template <typename T>
 void launch_steps(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int nsteps, hipStream_t s) {
-  if (!is_q_path(p) || is_qr(p)) {
+  if (chain_path(p)) {
     run_chain<T>(p, q, stride, pp, step0, nsteps, s);
     return;
   }
@@ -919,6 +1037,8 @@ double time_stage(cfft_plan* p, T* q, const ESel& es, int stage, int reps, hipSt
   auto launch = [&] {
     if (stage == CFFT_STAGE_UNFOLD_FOLD)
       launch_p51<T>(p, pp, B, 0, s);
+    else if (stage == CFFT_STAGE_PLANE_Q)
+      launch_p2q<T>(p, pp, B, s);
     else
       launch_p2<T>(p, pp, B, s);
   };
@@ -1122,8 +1242,8 @@ int cfft_plan_tables(const cfft_plan* p, int32_t* idx_out, int32_t* mult_out, do
 int cfft_update_kernel(cfft_plan* p, double ds, const double* lattice) {
   return guarded([&] {
     check_plan(p);
-    require(!is_q_path(p), CFFT_ERR_INVALID,
-            "update_kernel applies to the fwd+K+bwd paths; replan the Q-fused paths");
+    require(!is_q_path(p) || is_qs(p), CFFT_ERR_INVALID,
+            "update_kernel applies to the fwd+K+bwd paths and path QS; replan the other Q-fused paths");
     const double* lat = lattice ? lattice : p->lattice;
     p->K = kernel_spec(p->spec, ds, lat);
     p->ds = ds;
@@ -1135,6 +1255,7 @@ int cfft_update_kernel(cfft_plan* p, double ds, const double* lattice) {
     require(p->d_K.bytes == p->K.size() * sizeof(double), CFFT_ERR_INVALID, "kernel table size changed");
     if (!p->K.empty())
       hip_check(hipMemcpy(p->d_K.p, p->K.data(), p->d_K.bytes, hipMemcpyHostToDevice), "upload K");
+    if (is_qs(p)) build_qs_table(p);
   });
 }
 
@@ -1329,10 +1450,12 @@ int cfft_time_stage_dev(cfft_plan* p, int32_t stage, void* q, const void* const*
   return guarded([&] {
     check_plan(p);
     require(q && E && ms_per_launch && reps >= 1, CFFT_ERR_INVALID, "bad argument");
-    require(stage == CFFT_STAGE_UNFOLD_FOLD || stage == CFFT_STAGE_FFT_Y, CFFT_ERR_INVALID,
-            "stage must be CFFT_STAGE_UNFOLD_FOLD or CFFT_STAGE_FFT_Y");
+    require(stage == CFFT_STAGE_UNFOLD_FOLD || stage == CFFT_STAGE_FFT_Y ||
+                (stage == CFFT_STAGE_PLANE_Q && is_qs(p)),
+            CFFT_ERR_INVALID, "stage must be CFFT_STAGE_UNFOLD_FOLD, CFFT_STAGE_FFT_Y or (path QS) CFFT_STAGE_PLANE_Q");
     require(!p->prof, CFFT_ERR_INVALID, "per-launch event profiling is enabled on this plan");
-    require(!is_q_path(p) || is_qr(p), CFFT_ERR_UNSUPPORTED, "plan path has no fused chain kernels");
+    require(chain_path(p), CFFT_ERR_UNSUPPORTED, "plan path has no fused chain kernels");
+    require(cubic(p), CFFT_ERR_UNSUPPORTED, "stage timing needs a cubic grid");
     hipStream_t s = pick(p, stream);
     ESel es;
     es.mode = 2;

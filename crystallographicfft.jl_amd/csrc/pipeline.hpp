@@ -40,6 +40,8 @@ struct PipeP {
   double scale_unfold;  // 1 / (8 H0 H1 H2)
   double scale_irecon;  // 1 / d
   long n_rep;           // orbit representatives (path QR)
+  int qs_items;         // plane-pair work items per batch entry (path QS)
+  int qs_sep;           // path QS: t_{s,d} depends on bit d of s only
 };
 
 struct PipeBufs {
@@ -54,6 +56,7 @@ struct PipeBufs {
   const uint32_t* btab;  // inverse reconstruction table [e][8]
   const void* grep;      // path QR: Q-multiplied values at the orbit representatives
   const uint32_t* qmap;  // path QR: element -> rep (bits 0-19) | phase exponent << 20
+  const void* qsep;      // path QS: axis factors F[d][s][q] (cx<double>, 3 x 8 x M0)
 };
 
 template <int LEN>
@@ -62,20 +65,23 @@ struct PipeG {
   static constexpr int G = kNT / R1;  // transforms per pass of a workgroup
 };
 
-// FFT of n_tr length-LEN transforms held in LDS: element j of transform t is
-// buf[t * ts + j * es].  Every thread of the workgroup must call.
+// FFT of n1 * n2 length-LEN transforms held in LDS: element j of transform
+// t = t1 + n1 t2 is buf[t1 * ts1 + t2 * ts2 + j * es].  Every thread of the
+// workgroup must call.
 template <typename T, typename TW, int LEN, int DIR>
-__device__ __forceinline__ void lds_fft(cx<T>* buf, int n_tr, int ts, int es) {
+__device__ __forceinline__ void lds_fft2(cx<T>* buf, int n1, int ts1, int n2, int ts2, int es) {
   if constexpr (LEN > 1) {
     constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2, G = PipeG<LEN>::G;
     const int r = threadIdx.x % R1, g = threadIdx.x / R1;
+    const int n_tr = n1 * n2;
     for (int t0 = 0; t0 < n_tr; t0 += G) {
       const int t = t0 + g;
       // groups past the last transform work on a copy of transform 0's
       // values in registers and store nothing: whole groups only, so the
       // in-slot exchange of the real groups is not disturbed
       const bool ok = t < n_tr;
-      cx<T>* slot = buf + (ok ? t : 0) * ts;
+      const int t2 = ok ? t / n1 : 0;
+      cx<T>* slot = buf + (ok ? (t - t2 * n1) * ts1 + t2 * ts2 : 0);
       cx<T> v[R2];
 #pragma unroll
       for (int m = 0; m < R2; ++m) v[m] = slot[(r + R1 * m) * es];
@@ -87,6 +93,13 @@ __device__ __forceinline__ void lds_fft(cx<T>* buf, int n_tr, int ts, int es) {
     }
   }
   __syncthreads();
+}
+
+// FFT of n_tr length-LEN transforms held in LDS: element j of transform t is
+// buf[t * ts + j * es].
+template <typename T, typename TW, int LEN, int DIR>
+__device__ __forceinline__ void lds_fft(cx<T>* buf, int n_tr, int ts, int es) {
+  lds_fft2<T, TW, LEN, DIR>(buf, n_tr, ts, 1, 0, es);
 }
 
 template <typename T>
@@ -377,6 +390,118 @@ __device__ void ph_expand_ifft_zy(const PipeP& p, const PipeBufs& B, long item, 
   __syncthreads();
 }
 
+// ---- P2Q: FFT y, z + Q multiply + IFFT z, y on plane pairs (path QS) -------
+// Work item (b, c, x0): the planes x0 and x1, the image of x0 under
+// q0 -> -q0 mod M0 (one plane when x1 = x0).  The d = 8 representative ops are
+// diagonal, R_s = diag(+-1) with flip pattern s (bit i: axis i negated), so the
+// Q-fused multiply G0'(q) = sum_s Q_s(q) G0(R_s q) (src/q_fused_krfft.jl:
+// 518-667) reads only the 8 sign images of q: same channel, planes {x0, x1},
+// rows {z, zbar}, columns {y, ybar} -- the octet of q.  For an orthogonal cell
+// Q_s(q) = F0[s][q0] F1[s][q1] F2[s][q2] with the axis factors
+// F_d[s][q] = sum_{a in {0,1}} K_d(q + M_d a) exp(-2 pi i (q + M_d a) t_{s,d} / N_d)
+// (1/8 folded into F0).  Octets partition the item's elements and each thread
+// owns whole octets, so the multiply runs in place between the forward and
+// the inverse transforms.
+template <typename T, typename TW, int L>
+__device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds, cx<TW>* fq) {
+  constexpr int rs = L + 1, NP = L * L, SL = L * rs, NF = 2 + 2 * L;
+  constexpr int NFT = 8 * NF, FIT = (NFT + kNT - 1) / kNT;
+  const int b = (int)(item / p.qs_items);
+  int r = (int)(item - (long)b * p.qs_items);
+  int c = 0;
+  for (; c < p.n_ch - 1; ++c) {
+    const int nc = (p.offs[c] & 1) ? (L + 1) / 2 : L / 2 + 1;  // orbits of x -> xbar
+    if (r < nc) break;
+    r -= nc;
+  }
+  const int o0 = p.offs[c] & 1, o1 = (p.offs[c] >> 1) & 1, o2 = (p.offs[c] >> 2) & 1;
+  const int x0 = r, x1 = o0 ? L - 1 - r : (L - r) % L;
+  const int ns = x1 == x0 ? 1 : 2;
+  cx<T>* chan = static_cast<cx<T>*>(B.ch) + ((long)b * p.n_ch + c) * ((long)L * NP);
+  // this item's axis factors: fq[s * NF + j], j = 0, 1: x0, x1; 2 + y; 2 + L + z
+  const cx<TW>* F = static_cast<const cx<TW>*>(B.qsep);
+  const int Ms = p.M[0];
+  cx<TW> fv[FIT];
+#pragma unroll
+  for (int k = 0; k < FIT; ++k) {
+    const int i0 = threadIdx.x + k * kNT;
+    const int i = i0 < NFT ? i0 : NFT - 1;  // unconditional load, clamped index
+    const int sf = i / NF, j = i - sf * NF;
+    const int d = j < 2 ? 0 : (j < 2 + L ? 1 : 2);
+    const int q = j < 2 ? 2 * (j ? x1 : x0) + o0 : (j < 2 + L ? 2 * (j - 2) + o1 : 2 * (j - 2 - L) + o2);
+    fv[k] = F[(d * 8 + sf) * Ms + q];
+  }
+  plane_to_lds<T, L, L>(chan + (long)x0 * NP, lds);
+  if (ns == 2) plane_to_lds<T, L, L>(chan + (long)x1 * NP, lds + SL);
+#pragma unroll
+  for (int k = 0; k < FIT; ++k) {
+    const int i = threadIdx.x + k * kNT;
+    if (i < NFT) fq[i] = fv[k];
+  }
+  __syncthreads();
+  lds_fft<T, TW, L, -1>(lds, ns * L, rs, 1);         // FFT y: rows of both planes
+  lds_fft2<T, TW, L, -1>(lds, L, 1, ns, SL, rs);      // FFT z: columns of both planes
+  // octet representatives y <= ybar, z <= zbar
+  const int ny = o1 ? (L + 1) / 2 : L / 2 + 1, nz = o2 ? (L + 1) / 2 : L / 2 + 1;
+  for (int oc = threadIdx.x; oc < ny * nz; oc += kNT) {
+    const int y = oc % ny, z = oc / ny;
+    const int yb = o1 ? L - 1 - y : (L - y) % L, zb = o2 ? L - 1 - z : (L - z) % L;
+    int pos[8];
+    cx<T> v[8];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      pos[o] = ((o & 1) && ns == 2 ? SL : 0) + ((o & 4) ? zb : z) * rs + ((o & 2) ? yb : y);
+      v[o] = lds[pos[o]];
+    }
+    cx<TW> acc[8];
+    if (p.qs_sep) {
+      // t_{s,d} depends on bit d of s only: Q_s(q) = prod_d g_d(s_d, q_d) with
+      // g_d(s_d, .) = F_d[s_d << d][.], and the octet sum factors into three
+      // two-point stages (z, y, x)
+      cx<TW> u[8];
+#pragma unroll
+      for (int o = 0; o < 8; ++o) u[o] = {(TW)v[o].x, (TW)v[o].y};
+      const cx<TW> gz[2][2] = {{fq[2 + L + z], fq[4 * NF + 2 + L + z]}, {fq[2 + L + zb], fq[4 * NF + 2 + L + zb]}};
+      const cx<TW> gy[2][2] = {{fq[2 + y], fq[2 * NF + 2 + y]}, {fq[2 + yb], fq[2 * NF + 2 + yb]}};
+      const cx<TW> gx[2][2] = {{fq[0], fq[NF]}, {fq[1], fq[NF + 1]}};
+      cx<TW> w[8];
+#pragma unroll
+      for (int o = 0; o < 8; ++o) w[o] = cadd(cmul(gz[o >> 2][0], u[o]), cmul(gz[o >> 2][1], u[o ^ 4]));
+#pragma unroll
+      for (int o = 0; o < 8; ++o) u[o] = cadd(cmul(gy[(o >> 1) & 1][0], w[o]), cmul(gy[(o >> 1) & 1][1], w[o ^ 2]));
+#pragma unroll
+      for (int o = 0; o < 8; ++o) acc[o] = cadd(cmul(gx[o & 1][0], u[o]), cmul(gx[o & 1][1], u[o ^ 1]));
+    } else {
+#pragma unroll
+      for (int o = 0; o < 8; ++o) acc[o] = {TW(0), TW(0)};
+#pragma unroll
+      for (int sf = 0; sf < 8; ++sf) {
+        const cx<TW>* f = fq + sf * NF;
+        const cx<TW> fx0 = f[0], fx1 = f[1];
+        const cx<TW> fy0 = f[2 + y], fy1 = f[2 + yb];
+        const cx<TW> fz0 = f[2 + L + z], fz1 = f[2 + L + zb];
+        const cx<TW> fxy[4] = {cmul(fx0, fy0), cmul(fx1, fy0), cmul(fx0, fy1), cmul(fx1, fy1)};
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+          const cx<TW> qc = cmul(fxy[o & 3], (o & 4) ? fz1 : fz0);
+          const cx<T> g = v[o ^ sf];
+          acc[o].x += qc.x * (TW)g.x - qc.y * (TW)g.y;
+          acc[o].y += qc.x * (TW)g.y + qc.y * (TW)g.x;
+        }
+      }
+    }
+    // coinciding positions (self-images) receive identical values
+#pragma unroll
+    for (int o = 0; o < 8; ++o) lds[pos[o]] = {(T)acc[o].x, (T)acc[o].y};
+  }
+  __syncthreads();
+  lds_fft2<T, TW, L, +1>(lds, L, 1, ns, SL, rs);  // IFFT z
+  lds_fft<T, TW, L, +1>(lds, ns * L, rs, 1);      // IFFT y
+  lds_to_plane<T, L, L>(lds, chan + (long)x0 * NP);
+  if (ns == 2) lds_to_plane<T, L, L>(lds + SL, chan + (long)x1 * NP);
+  __syncthreads();
+}
+
 // IFFT z, IFFT y of one plane of `src` written to `dst` (Q-fused path).
 template <typename T, typename TW, int L1, int L2>
 __device__ void ph_ifft_zy(const PipeP& p, const PipeBufs& B, const cx<T>* src, long item,
@@ -614,6 +739,13 @@ __global__ void __launch_bounds__(256) k_expand(const cx<T>* __restrict__ grep, 
   const int b = (int)(tid / n_alive);
   const uint32_t ent = qmap[tid - (long)b * n_alive];
   ch[tid] = twmul<T, double, -1>(grep[(long)b * n_rep + (ent & 0xFFFFFu)], W[ent >> 20]);
+}
+
+template <typename T, typename TW, int L>
+__global__ void __launch_bounds__(kNT) k_p2q(PipeP p, PipeBufs B) {
+  __shared__ cx<T> lds[2 * L * (L + 1)];
+  __shared__ cx<TW> fq[8 * (2 + 2 * L)];
+  ph_plane_pair_q<T, TW, L>(p, B, blockIdx.x, lds, fq);
 }
 
 template <typename T, typename TW, int L0>

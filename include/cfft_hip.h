@@ -59,6 +59,14 @@ extern "C" {
                           * under the subgrid-preserving ops, then the orbit
                           * expansion of src/centering_fold.jl:630-731; diffusion
                           * and steps only (no F_spec output)                   */
+#define CFFT_PATH_QS 6   /* fold + one kernel per plane pair: FFT y, z, the
+                          * Q-fused multiply in separable form over the sign
+                          * images of each point, IFFT z, y (the separable
+                          * multiply of src/q_fused_krfft.jl:606-667 on the
+                          * channel array); needs cubic H, L = (2,2,2), the 8
+                          * representative ops diagonal with distinct sign
+                          * patterns and an orthogonal cell; diffusion, steps
+                          * and the forward transform (no F_spec backward)     */
 
 #define CFFT_FP64 0  /* all arithmetic FP64                                   */
 #define CFFT_MIXED 1 /* FP32 data/FFT/propagator, FP64 tables and reductions  */
@@ -196,7 +204,8 @@ int cfft_exp_dev(cfft_plan* plan, const void* w, void* E, double factor, int64_t
 #define CFFT_STAGE_QMUL 10
 #define CFFT_STAGE_SWEEP 11 /* persistent sweep (all phases, all steps) */
 #define CFFT_STAGE_UNFOLD_FOLD 12 /* P5 of step s fused with P1 of s + 1 */
-#define CFFT_N_STAGES 13
+#define CFFT_STAGE_PLANE_Q 13     /* path QS: FFT y, z + Q + IFFT z, y      */
+#define CFFT_N_STAGES 14
 /* No counterpart in the reference (its benches time stages with @elapsed,
  * test/profile_centering_fold.jl:48-107).  enable != 0 brackets every
  * subsequent kernel launch of the plan with a pair of hipEvents. */
@@ -205,11 +214,12 @@ int cfft_profile(cfft_plan* plan, int32_t enable);
  * counts per stage since the last read; clears the record. */
 int cfft_profile_read(cfft_plan* plan, double* ms_out, int64_t* count_out);
 /* Average duration of one launch of a chain kernel (no counterpart in the
- * reference): stage CFFT_STAGE_UNFOLD_FOLD (P5 fused with P1) or
- * CFFT_STAGE_FFT_Y (P2), launched once untimed and then reps times back to
+ * reference): stage CFFT_STAGE_UNFOLD_FOLD (P5 fused with P1),
+ * CFFT_STAGE_FFT_Y (P2) or, on path QS, CFFT_STAGE_PLANE_Q (the plane-pair
+ * kernel), launched once untimed and then reps times back to
  * back on stream between two hipEvents, on the device field q (batch * M^3
  * entries, overwritten) with interaction factors E[b] (device pointers).
- * Plans on the fused-chain paths only (M2, M7, QR); profiling must be off. */
+ * Plans on the fused-chain paths only (M2, M7, QR, QS); profiling must be off. */
 int cfft_time_stage_dev(cfft_plan* plan, int32_t stage, void* q, const void* const* E, int32_t reps,
                         void* stream, double* ms_per_launch);
 

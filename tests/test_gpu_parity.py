@@ -113,6 +113,57 @@ def test_orbit_reduced_q_fp64(sg, N):
         plan.backward(F)
 
 
+@pytest.mark.parametrize("sg,N", [(229, (16, 16, 16)), (229, (32, 32, 32)), (230, (24, 24, 24)),
+                                  (70, (16, 16, 16)), (47, (16, 16, 16)), (221, (24, 24, 24)),
+                                  (225, (32, 32, 32)), (223, (24, 24, 24)), (227, (32, 32, 32)),
+                                  (47, (12, 12, 12)), (229, (12, 12, 12)), (230, (48, 48, 48))])
+def test_plane_pair_q_fp64(sg, N):
+    """Path QS (FFT y, z + separable Q multiply over the sign images + IFFT z, y
+    in one kernel per plane pair) = full-grid FFT.K.IFFT and = the oracle's
+    Q-fused restatement; Strang steps = the oracle's; mixed precision within
+    2e-5; a kernel update equals a fresh plan.  H = 3 exercises the odd-length
+    orbit counts."""
+    print(f"QS sg={sg} N={N}", flush=True)
+    ops_s, R, t, u, Lv = field(sg, N, seed=5)
+    f0 = O.extract_subgrid(u, Lv)
+    plan = cf.DevicePlan(N, ops_s, DS, LAT4, cf.PATH_QS)
+    d = plan.diffuse(f0)
+    assert rel(d, O.fullgrid_reference(u, N, DS, LAT4, Lv)) < TOL64
+    qp = O.m2q_plan(R, t, N, DS, LAT4)
+    orb = O.subgrid_orbits(R, t, N, Lv)
+    assert rel(d, O.qr_scft(f0, qp, orb)) < TOL64
+    w = 2.0 * O.extract_subgrid(O.make_symmetric(R, t, N, seed=9), Lv)
+    q = f0 + 1.0
+    out = plan.step(q, w, DS, nsteps=5)
+    ref = q.copy()
+    for _ in range(5):
+        ref = O.strang_step(ref, w, DS, lambda x: O.qr_scft(x, qp, orb))
+    assert rel(out, ref) < TOL64
+    p32 = cf.DevicePlan(N, ops_s, DS, LAT4, cf.PATH_QS, cf.MIXED)
+    assert rel(p32.diffuse(f0), d) < TOL32
+    spec = O.calc_spectral_asu(R, t, N)
+    F = plan.forward(f0)
+    assert rel(F, np.fft.fftn(u)[tuple(spec.idx[:, k] for k in range(3))]) < TOL64
+    with pytest.raises(cf.CFFTError):
+        plan.backward(F)
+    lat2 = 3.7 * np.eye(3)
+    plan.update_kernel(0.02, lat2)
+    fresh = cf.DevicePlan(N, ops_s, 0.02, lat2, cf.PATH_QS)
+    assert np.array_equal(plan.diffuse(f0), fresh.diffuse(f0))
+    assert rel(fresh.diffuse(f0), O.fullgrid_reference(u, N, 0.02, lat2, Lv)) < TOL64
+
+
+@pytest.mark.parametrize("sg,N,lat", [(10, (16, 16, 16), LAT4), (71, (24, 32, 48), LAT4),
+                                      (229, (32, 32, 32), np.array([[4.0, 0.5, 0.0], [0.0, 4.0, 0.0],
+                                                                    [0.0, 0.0, 4.0]]))])
+def test_plane_pair_q_rejects_unsupported(sg, N, lat):
+    """L != (2,2,2), a non-cubic grid and a non-orthogonal cell are refused at
+    plan time with an error, not run."""
+    ops_s, R, t, u, Lv = field(sg, N, seed=5)
+    with pytest.raises(cf.CFFTError):
+        cf.DevicePlan(N, ops_s, DS, lat, cf.PATH_QS)
+
+
 @pytest.mark.parametrize("sg,N", [(229, (32, 32, 32)), (230, (32, 32, 32)), (70, (32, 32, 32)),
                                   (47, (32, 32, 32)), (223, (48, 48, 48)), (229, (48, 48, 48)),
                                   (47, (24, 24, 24)), (230, (96, 96, 96))])

@@ -10,7 +10,7 @@ from conftest import group_setup
 pytestmark = pytest.mark.gpu
 
 
-def sweep_case(sg, N, f, ns, prec, seed=3, persistent=0, fuse=1, graph=1):
+def sweep_case(sg, N, f, ns, prec, seed=3, persistent=0, fuse=1, graph=1, path=cf.PATH_AUTO):
     import torch
     from cfft_hip.scft import PropagatorSweep
     ops, ops_s, shift, R, t = group_setup(sg, N)
@@ -18,7 +18,7 @@ def sweep_case(sg, N, f, ns, prec, seed=3, persistent=0, fuse=1, graph=1):
     lat = 4.0 * np.eye(3)
     wA = 2.0 * O.extract_subgrid(O.make_symmetric(R, t, N, seed=seed), Lv)
     wB = -1.5 * O.extract_subgrid(O.make_symmetric(R, t, N, seed=seed + 1), Lv)
-    sw = PropagatorSweep(N, ops_s, lat, f, ns, prec)
+    sw = PropagatorSweep(N, ops_s, lat, f, ns, prec, path=path)
     sw.plan.set_option(cf._lib.OPT_PERSISTENT, persistent)
     sw.plan.set_option(cf._lib.OPT_FUSE, fuse)
     sw.plan.set_option(cf._lib.OPT_GRAPH, graph)
@@ -52,7 +52,7 @@ def test_sweep_fp64_vs_oracle(sg, N, persistent, fuse, graph):
     assert abs(Q1 - float(np.mean(rq))) / abs(Q1) < 1e-12
 
 
-@pytest.mark.parametrize("path", [cf.PATH_AUTO, cf.PATH_QR])
+@pytest.mark.parametrize("path", [cf.PATH_AUTO, cf.PATH_QR, cf.PATH_QS])
 def test_live_stage_timing(path):
     """cfft_time_stage_dev: positive per-launch times for P5+P1 and P2,
     other stages rejected, and the plan still sweeps to the oracle afterwards
@@ -69,7 +69,7 @@ def test_live_stage_timing(path):
     sw.set_fields(wA, wB)
     lib = cf._lib.lib()
     scratch = sw.q.clone()
-    for name in ("p51_unfold_fold", "p2_fft_yz"):
+    for name in ("p51_unfold_fold", "p2q_plane_pair" if path == cf.PATH_QS else "p2_fft_yz"):
         ms = C.c_double()
         cf._lib.check(lib.cfft_time_stage_dev(sw.plan.handle, cf._lib.STAGES.index(name),
                                               C.c_void_p(scratch.data_ptr()), sw._E1, 5,
@@ -89,6 +89,19 @@ def test_live_stage_timing(path):
     diffuse = lambda x: O.m2_scft(x, spec, R, t, N, 1.0 / 20, lat, table)
     ref = O.propagate(np.ones_like(wA), lambda j: wA if j < 6 else wB, 20, 1.0 / 20, diffuse)[-1]
     assert np.max(np.abs(q - ref)) / np.max(np.abs(ref)) < 1e-12
+
+
+@pytest.mark.parametrize("fuse,graph", [(1, 1), (0, 0), (1, 0)])
+@pytest.mark.parametrize("sg,N", [(229, (32, 32, 32)), (230, (32, 32, 32)), (47, (32, 32, 32)),
+                                  (70, (32, 32, 32)), (223, (24, 24, 24))])
+def test_sweep_plane_pair_q_vs_oracle(sg, N, fuse, graph):
+    """Contour sweeps on path QS (plane-pair kernel + fused unfold/fold chain,
+    eager or replayed from a hipGraph) vs the oracle's Strang propagation."""
+    print(f"QS sweep sg={sg} N={N} fuse={fuse} graph={graph}", flush=True)
+    q, qd, rq, rqd, (Q1, Q2) = sweep_case(sg, N, 0.3, 20, cf.FP64, fuse=fuse, graph=graph, path=cf.PATH_QS)
+    assert np.max(np.abs(q - rq)) / np.max(np.abs(rq)) < 1e-12
+    assert np.max(np.abs(qd - rqd)) / np.max(np.abs(rqd)) < 1e-12
+    assert abs(Q1 - Q2) / abs(Q1) < 1e-12
 
 
 def test_sweep_mixed_vs_fp64():
