@@ -84,7 +84,7 @@ def _prepare(inst: Instance, precision, m_hist, finish):
     ops = get_ops(inst.sg, 3, inst.N)
     shift, ops_s = find_optimal_shift(ops, inst.N)
     mk = lambda prec: SCFTSolver(inst.N, ops_s, inst.cell * np.eye(3), inst.f, inst.chiN, inst.ns, prec,
-                                 path=L.PATH_AUTO, m_hist=m_hist)
+                                 path=None, m_hist=m_hist)
     solver = mk(precision)
     fin = mk(L.FP64) if finish else None
     return solver, fin, start_fields(inst, ops_s, shift)

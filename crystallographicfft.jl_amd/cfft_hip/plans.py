@@ -203,6 +203,28 @@ def build_fill_map(ops_shifted, Lv, N) -> np.ndarray:
 _I = np.eye(3)
 
 
+def qs_eligible(N, ops_shifted, lattice) -> bool:
+    """Whether path QS (plane-pair kernel) can plan this case: cubic grid,
+    L = (2,2,2), an orthogonal cell and 8 diagonal representative ops with
+    distinct sign patterns (host-side check, no GPU)."""
+    N = tuple(int(n) for n in N)
+    if len(N) != 3 or len(set(N)) != 1:
+        return False
+    lat = np.asarray(lattice, dtype=np.float64).reshape(3, 3)
+    if np.count_nonzero(lat - np.diag(np.diag(lat))):
+        return False
+    Lv = auto_L(ops_shifted)
+    if tuple(int(v) for v in Lv) != (2, 2, 2) or N[0] % 4:
+        return False
+    patterns = set()
+    for r in select_rep_ops(ops_shifted, Lv, N):
+        R = np.asarray(ops_shifted[int(r)].Rm)
+        if np.count_nonzero(R - np.diag(np.diag(R))) or not np.all(np.abs(np.diag(R)) == 1):
+            return False
+        patterns.add(tuple(int(v) for v in np.diag(R)))
+    return len(patterns) == 8
+
+
 class DevicePlan:
     """One C plan (device tables + buffers) for a 3-D grid, a shifted op list,
     a pipeline path, a precision and a batch size."""

@@ -21,7 +21,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib as L
-from .plans import DevicePlan
+from .plans import DevicePlan, qs_eligible
 
 
 def first_star_bcc(N, shift_grid, cell):
@@ -144,11 +144,20 @@ class SCFTSolver:
 
     KEYS = ("Q", "Qd", "F", "res", "phi_total", "phiA_mean", "anderson", "hist")
 
-    def __init__(self, N, ops_shifted, lattice, f, chiN, ns, precision=L.FP64, path=L.PATH_AUTO,
+    def __init__(self, N, ops_shifted, lattice, f, chiN, ns, precision=L.FP64, path=None,
                  m_hist=10):
         self.ns = int(ns)
         self.f = float(f)
         self.chiN = float(chiN)
+        if path is None:
+            # FP64: the plane-pair chain where it applies.  Mixed precision keeps
+            # the default path: there every step passes through the spectral
+            # ASU, which projects FP32 rounding back onto the symmetric fields,
+            # while on the plane-pair chain the asymmetric part of the rounding
+            # accumulates over a sweep (residual floor ~4e-6 instead of ~7e-7 at
+            # Ia-3d 128^3, Ns 200)
+            fp64 = precision == L.FP64
+            path = L.PATH_QS if fp64 and qs_eligible(N, ops_shifted, lattice) else L.PATH_AUTO
         self.plan = DevicePlan(N, ops_shifted, 1.0 / self.ns, lattice, path, precision, batch=2)
         self.M = self.plan.M
         self.M3 = int(np.prod(self.M))

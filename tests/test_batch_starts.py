@@ -43,3 +43,19 @@ def test_start_fields_seeded_and_antisymmetric_in_the_harmonic():
     a3, b3 = batch.start_fields(inst, ops_s, shift)
     assert np.array_equal(a2, a3) and np.array_equal(b2, b3)
     assert 0.0 < np.max(np.abs(a2 - a1)) < 1.0
+
+
+@pytest.mark.parametrize("sg,N,lat,expect", [
+    (229, (32, 32, 32), 4.0 * np.eye(3), True), (230, (32, 32, 32), 3.8 * np.eye(3), True),
+    (70, (32, 32, 32), np.diag([4.0, 5.0, 6.0]), True), (47, (24, 24, 24), 4.0 * np.eye(3), True),
+    (223, (48, 48, 48), 4.0 * np.eye(3), True),
+    (10, (16, 16, 16), 4.0 * np.eye(3), False),            # L = (2,2,1)
+    (71, (24, 32, 48), 4.0 * np.eye(3), False),            # not cubic
+    (229, (32, 32, 32), np.array([[4.0, 0.5, 0], [0, 4.0, 0], [0, 0, 4.0]]), False)])  # oblique
+def test_plane_pair_path_eligibility(sg, N, lat, expect):
+    """Host check used to pick path QS: cubic grid, L = (2,2,2), orthogonal
+    cell, diagonal representative ops with 8 distinct sign patterns."""
+    from cfft_hip.plans import qs_eligible
+    ops = get_ops(sg, 3, N)
+    shift, ops_s = find_optimal_shift(ops, N)
+    assert qs_eligible(N, ops_s, lat) is expect
