@@ -25,5 +25,16 @@ from .plans import (ASUPoint, CenteredKRFFTBackwardPlan, CenteredKRFFTPlan, Cent
 from .asu import (ASUBlock, CFFTPlan, CrystallographicASU, calc_asu, ldiv, mul, pack_asu,
                   plan_cfft)
 from . import scft  # noqa: F401
+from .families import (FractalCFFTPlan, G0ASUBackwardPlan, G0ASUPlan, GeneralG0ASUPlan,
+                       RecursiveKRFFTPlan, SelectiveG0Plan, SparseKRFFTPlan, StagedPm3mPlan,
+                       compute_spectral_asu, execute_fractal_krfft, execute_fractal_krfft_v2,
+                       execute_fractal_krfft_v3, execute_g0asu_ikrfft, execute_g0asu_krfft,
+                       execute_general_g0asu_krfft, execute_recursive_krfft,
+                       execute_selective_krfft, execute_sparse_krfft, execute_staged,
+                       execute_staged_full, has_cubic_p3c_symmetry, lookup_full_spectrum,
+                       plan_fractal_krfft, plan_fractal_krfft_v2, plan_fractal_krfft_v3,
+                       plan_krfft_g0asu, plan_krfft_g0asu_backward, plan_krfft_g0asu_general,
+                       plan_krfft_recursive, plan_krfft_selective, plan_krfft_sparse,
+                       plan_staged_pm3m, plan_staged_pm3m_full)
 from .diffusion_solver import (DiffusionSolver, QFusedDiffusionSolver, apply_diffusion_operator,
                                interaction_step, plan_diffusion, step_diffusion)

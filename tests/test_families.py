@@ -1,0 +1,112 @@
+"""The reference's other KRFFT families (recursive P3c, sparse, selective,
+G0-ASU forward/backward, general G0-ASU, fractal, staged Pm-3m) on the device
+pipeline: the plan builders' preconditions on the CPU (test/test_g0_asu.jl,
+test/test_g0_asu_backward.jl, test/test_staged_pm3m.jl check the same
+assertions), and on the GPU the outputs against the full-grid FFT of the
+symmetrised field at the spectral-ASU points (the check of
+test/test_selective_g0.jl, test/test_general_g0_asu.jl, test/test_fractal_krfft.jl)."""
+import numpy as np
+import pytest
+
+import cfft_hip as cf
+import oracle as O
+from conftest import group_setup
+
+
+def _spec(sg, N):
+    ops, ops_s, shift, R, t = group_setup(sg, N)
+    return ops_s, R, t, cf.calc_spectral_asu(ops_s, 3, N)
+
+
+@pytest.mark.parametrize("sg,expect", [(221, True), (229, True), (230, True), (223, True),
+                                       (47, False), (70, False), (139, False)])
+def test_has_cubic_p3c_symmetry(sg, expect):
+    """src/recursive_blocks.jl:974-982: only cubic groups carry the
+    body-diagonal 3-fold axis."""
+    ops_s, R, t, spec = _spec(sg, (16, 16, 16))
+    assert cf.has_cubic_p3c_symmetry(ops_s) is expect
+
+
+def test_p3c_plans_require_even_subgrid():
+    """N = 18: N is even but M = 9 is odd (the P3c assertion M2 * 2 == M)."""
+    ops_s, R, t, spec = _spec(229, (18, 18, 18))
+    for planner in (cf.plan_krfft_recursive, cf.plan_krfft_selective, cf.plan_krfft_sparse,
+                    cf.plan_krfft_g0asu, cf.plan_krfft_g0asu_backward):
+        with pytest.raises(cf.CFFTError, match="divisible by 2 for P3c"):
+            planner(spec, ops_s)
+
+
+def test_g0asu_backward_requires_cubic_symmetry():
+    ops_s, R, t, spec = _spec(47, (16, 16, 16))
+    with pytest.raises(cf.CFFTError, match="requires cubic symmetry"):
+        cf.plan_krfft_g0asu_backward(spec, ops_s)
+
+
+def test_staged_plan_preconditions():
+    """src/staged_krfft.jl:116-117."""
+    with pytest.raises(cf.CFFTError, match="even"):
+        cf.plan_staged_pm3m((18, 18, 17))
+    with pytest.raises(cf.CFFTError, match="cubic"):
+        cf.plan_staged_pm3m((16, 16, 32))
+
+
+FORWARD = [(cf.plan_krfft_recursive, cf.execute_recursive_krfft),
+           (cf.plan_krfft_sparse, cf.execute_sparse_krfft),
+           (cf.plan_krfft_selective, cf.execute_selective_krfft),
+           (cf.plan_krfft_g0asu, cf.execute_g0asu_krfft),
+           (cf.plan_krfft_g0asu_general, cf.execute_general_g0asu_krfft)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sg,N", [(229, (32, 32, 32)), (221, (16, 16, 16)), (47, (16, 16, 16)),
+                                  (230, (24, 24, 24)), (70, (16, 16, 16))])
+def test_forward_families_match_full_fft(sg, N):
+    ops_s, R, t, spec = _spec(sg, N)
+    u = O.make_symmetric(R, t, N, seed=21)
+    ref = np.fft.fftn(u)[tuple(spec.idx[:, d] for d in range(3))]
+    scale = np.max(np.abs(ref))
+    for planner, execute in FORWARD:
+        plan = planner(spec, ops_s)
+        out = execute(plan, spec, u)
+        assert out is plan.output_buffer
+        assert np.max(np.abs(out - ref)) / scale < 1e-12, planner.__name__
+    fp = cf.plan_fractal_krfft(spec, ops_s)
+    assert np.max(np.abs(cf.execute_fractal_krfft(fp, u) - ref)) / scale < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sg,N", [(229, (32, 32, 32)), (221, (16, 16, 16)), (230, (24, 24, 24))])
+def test_g0asu_backward_round_trip(sg, N):
+    """F_spec -> u on the full grid reproduces the symmetric field
+    (test/test_g0_asu_backward.jl)."""
+    ops_s, R, t, spec = _spec(sg, N)
+    u = O.make_symmetric(R, t, N, seed=22)
+    F = np.fft.fftn(u)[tuple(spec.idx[:, d] for d in range(3))]
+    bp = cf.plan_krfft_g0asu_backward(spec, ops_s)
+    out = np.zeros(N)
+    res = cf.execute_g0asu_ikrfft(bp, spec, F, out)
+    assert res is out
+    assert np.max(np.abs(out - u)) / np.max(np.abs(u)) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [16, 32])
+def test_staged_pm3m(n):
+    """F000 = FFT of u(2x); the full spectrum at random frequencies and at the
+    spectral ASU from F000 (test/test_staged_pm3m.jl)."""
+    N = (n, n, n)
+    ops_s, R, t, spec = _spec(221, N)
+    u = O.make_symmetric(R, t, N, seed=23)
+    plan = cf.plan_staged_pm3m(N)
+    F000 = cf.execute_staged(plan, u)
+    ref000 = np.fft.fftn(u[::2, ::2, ::2])
+    assert np.max(np.abs(F000 - ref000)) / np.max(np.abs(ref000)) < 1e-12
+    full = np.fft.fftn(u)
+    rng = np.random.default_rng(5)
+    h = rng.integers(0, n, size=(50, 3))
+    got = cf.lookup_full_spectrum(plan, F000, h[:, 0], h[:, 1], h[:, 2])
+    assert np.max(np.abs(got - full[h[:, 0], h[:, 1], h[:, 2]])) / np.max(np.abs(full)) < 1e-12
+    res = np.zeros(len(spec), dtype=np.complex128)
+    cf.compute_spectral_asu(res, plan, F000, spec)
+    ref = full[tuple(spec.idx[:, d] for d in range(3))]
+    assert np.max(np.abs(res - ref)) / np.max(np.abs(ref)) < 1e-12
