@@ -12,7 +12,7 @@ using CrystallographicFFT.ASU: find_optimal_shift
 
 const libcfft = get(ENV, "CFFT_HIP_LIB", joinpath(@__DIR__, "..", "libcfft_hip.so"))
 
-const PATH_AUTO, PATH_M2, PATH_M7, PATH_M2Q, PATH_M7Q = Int32.(0:4)
+const PATH_AUTO, PATH_M2, PATH_M7, PATH_M2Q, PATH_M7Q, PATH_QR, PATH_QS = Int32.(0:6)
 const FP64, MIXED = Int32(0), Int32(1)
 
 struct CFFTError <: Exception
