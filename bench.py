@@ -136,7 +136,7 @@ def profile_stages(sw, steps, torch):
     return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(L.STAGES) if cnt[i]}
 
 
-STRATEGY_PATH = {"m7_chain": 0, "m7_persistent": 0, "qr_chain": 5, "qs_chain": 6}
+STRATEGY_PATH = {"m7_chain": 0, "m7_persistent": 0, "qr_chain": 5, "qs_chain": 6, "qs_chain_eager": 6}
 
 
 def profiled_traffic(strategy, stage_name, prec_name):
@@ -483,11 +483,13 @@ def main():
         # orbit-reduced Q chain; the fastest one is used for the timed region
         trial = {}
         sweeps = {}
-        for label, path, persist in (("m7_chain", cf.PATH_AUTO, 0), ("m7_persistent", cf.PATH_AUTO, 1),
-                                     ("qr_chain", cf.PATH_QR, 0), ("qs_chain", cf.PATH_QS, 0)):
+        for label, path, persist, graph in (("m7_chain", cf.PATH_AUTO, 0, 1), ("m7_persistent", cf.PATH_AUTO, 1, 1),
+                                            ("qr_chain", cf.PATH_QR, 0, 1), ("qs_chain", cf.PATH_QS, 0, 1),
+                                            ("qs_chain_eager", cf.PATH_QS, 0, 0)):
             sw = cf.scft.PropagatorSweep(N, ops_s, lat, F_A, NS, prec, path=path)
             sw.set_fields(wA, wB)
             sw.plan.set_option(cf._lib.OPT_PERSISTENT, persist)
+            sw.plan.set_option(cf._lib.OPT_GRAPH, graph)
             trial[label] = time_sweeps(sw, min(args.steps, NS), 1, torch)
             sweeps[label] = sw
         best = min(trial, key=trial.get)
