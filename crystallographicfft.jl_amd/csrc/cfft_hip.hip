@@ -716,7 +716,7 @@ void launch_p2q(cfft_plan* p, const PipeP& pp, const PipeBufs& B, hipStream_t s)
   StageTimer st(p, s, CFFT_STAGE_PLANE_Q);
   const unsigned g = (unsigned)((long)p->qs_items * p->batch);
 #define CFFT_P2Q(L) \
-  case L: hipLaunchKernelGGL((k_p2q<T, double, L>), dim3(g), dim3(kNT), 0, s, pp, B); break;
+  case L: hipLaunchKernelGGL((k_p2q<T, double, L>), dim3(g), dim3(kQNT), 0, s, pp, B); break;
   CFFT_LEN_SWITCH(p->H[1], CFFT_P2Q)
 #undef CFFT_P2Q
   launch_check<T>("k_p2q");

@@ -59,19 +59,19 @@ struct PipeBufs {
   const void* qsep;      // path QS: axis factors F[d][s][q] (cx<double>, 3 x 8 x M0)
 };
 
-template <int LEN>
+template <int LEN, int NT = kNT>
 struct PipeG {
   static constexpr int R1 = MLS<LEN>::R1;
-  static constexpr int G = kNT / R1;  // transforms per pass of a workgroup
+  static constexpr int G = NT / R1;  // transforms per pass of a workgroup of NT threads
 };
 
 // FFT of n1 * n2 length-LEN transforms held in LDS: element j of transform
 // t = t1 + n1 t2 is buf[t1 * ts1 + t2 * ts2 + j * es].  Every thread of the
 // workgroup must call.
-template <typename T, typename TW, int LEN, int DIR>
+template <typename T, typename TW, int LEN, int DIR, int NT = kNT>
 __device__ __forceinline__ void lds_fft2(cx<T>* buf, int n1, int ts1, int n2, int ts2, int es) {
   if constexpr (LEN > 1) {
-    constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2, G = PipeG<LEN>::G;
+    constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2, G = PipeG<LEN, NT>::G;
     const int r = threadIdx.x % R1, g = threadIdx.x / R1;
     const int n_tr = n1 * n2;
     for (int t0 = 0; t0 < n_tr; t0 += G) {
@@ -97,9 +97,9 @@ __device__ __forceinline__ void lds_fft2(cx<T>* buf, int n1, int ts1, int n2, in
 
 // FFT of n_tr length-LEN transforms held in LDS: element j of transform t is
 // buf[t * ts + j * es].
-template <typename T, typename TW, int LEN, int DIR>
+template <typename T, typename TW, int LEN, int DIR, int NT = kNT>
 __device__ __forceinline__ void lds_fft(cx<T>* buf, int n_tr, int ts, int es) {
-  lds_fft2<T, TW, LEN, DIR>(buf, n_tr, ts, 1, 0, es);
+  lds_fft2<T, TW, LEN, DIR, NT>(buf, n_tr, ts, 1, 0, es);
 }
 
 template <typename T>
@@ -116,31 +116,31 @@ __device__ __forceinline__ long corner_index(const PipeP& p, int x, int y, int z
 // Plane of L1 x L2 entries (y fastest, contiguous) into LDS rows of stride
 // L1 + 1.  The trip count is a compile-time constant, so every global load is
 // issued before the first LDS write.
-template <typename T, int L1, int L2>
+template <typename T, int L1, int L2, int NT = kNT>
 __device__ __forceinline__ void plane_to_lds(const cx<T>* __restrict__ src, cx<T>* lds) {
-  constexpr int NP = L1 * L2, NIT = (NP + kNT - 1) / kNT;
+  constexpr int NP = L1 * L2, NIT = (NP + NT - 1) / NT;
   cx<T> r[NIT];
   // loads are unconditional (a clamped index on the ragged last pass) so that
   // none of them waits behind a branch
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
-    const int i = threadIdx.x + it * kNT;
-    r[it] = src[(NP % kNT == 0 || i < NP) ? i : NP - 1];
+    const int i = threadIdx.x + it * NT;
+    r[it] = src[(NP % NT == 0 || i < NP) ? i : NP - 1];
   }
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
-    const int i = threadIdx.x + it * kNT;
-    if (NP % kNT == 0 || i < NP) lds[(i / L1) * (L1 + 1) + i % L1] = r[it];
+    const int i = threadIdx.x + it * NT;
+    if (NP % NT == 0 || i < NP) lds[(i / L1) * (L1 + 1) + i % L1] = r[it];
   }
 }
 
-template <typename T, int L1, int L2>
+template <typename T, int L1, int L2, int NT = kNT>
 __device__ __forceinline__ void lds_to_plane(const cx<T>* lds, cx<T>* __restrict__ dst) {
-  constexpr int NP = L1 * L2, NIT = (NP + kNT - 1) / kNT;
+  constexpr int NP = L1 * L2, NIT = (NP + NT - 1) / NT;
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
-    const int i = threadIdx.x + it * kNT;
-    if (NP % kNT == 0 || i < NP) dst[i] = lds[(i / L1) * (L1 + 1) + i % L1];
+    const int i = threadIdx.x + it * NT;
+    if (NP % NT == 0 || i < NP) dst[i] = lds[(i / L1) * (L1 + 1) + i % L1];
   }
 }
 
@@ -217,6 +217,43 @@ __device__ __forceinline__ FoldTw<TW> load_fold_tw(const cx<TW>* __restrict__ tw
   return {twM[x], twM[p.M[0] + y], twM[p.M[0] + p.M[1] + z]};
 }
 
+// Channel values of one tile point (LDS entry base + c * cstride of channel c)
+// into its 8 corner parities: v[e] = Re(g_c conj(tw_e)) for the channel c alive
+// at parity e (chmap), 0 at extinct parities.  The parity loop is unrolled, so
+// the twiddle products are static; the 8 LDS reads are unconditional (extinct
+// parities read channel 0 and discard it) so that all of them issue at once.
+template <typename T, typename TW>
+__device__ __forceinline__ void gather_parities(const PipeP& p, const cx<T>* lds, int base, int cstride,
+                                                const FoldTw<TW>& ft, T (&v)[8]) {
+  cx<T> g[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int c = p.chmap[e];
+    g[e] = lds[base + (c >= 0 ? c : 0) * cstride];
+  }
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const cx<TW> tw = fold_tw(e, ft);
+    const T val = (T)((TW)g[e].x * tw.x + (TW)g[e].y * tw.y);
+    v[e] = p.chmap[e] >= 0 ? val : T(0);
+  }
+}
+
+// The alive parities of v into the channel entries of one tile point:
+// g_c = v[e] tw_e for the channel c alive at parity e.
+template <typename T, typename TW>
+__device__ __forceinline__ void scatter_parities(const PipeP& p, cx<T>* lds, int base, int cstride,
+                                                 const FoldTw<TW>& ft, const T (&v)[8]) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int c = p.chmap[e];
+    if (c >= 0) {
+      const cx<TW> tw = fold_tw(e, ft);
+      lds[base + c * cstride] = {(T)((TW)v[e] * tw.x), (T)((TW)v[e] * tw.y)};
+    }
+  }
+}
+
 // ---- P1: fold + E + FFT x --------------------------------------------------
 // tile item = (b, z, y-block); LDS A[c][yy][x] with row stride H0 + 1.
 template <typename T, typename TW, int L0>
@@ -253,15 +290,7 @@ __device__ void ph_fold_fftx(const PipeP& p, const PipeBufs& B, long item, int s
     if (pt >= H0 * YB) continue;
     const int x = pt % H0, yy = pt / H0;
     wht8(v[k]);
-    for (int c = 0; c < p.n_ch; ++c) {
-      const int o = p.offs[c];
-      T val = T(0);
-#pragma unroll
-      for (int e = 0; e < 8; ++e)
-        if (e == o) val = v[k][e];
-      const cx<TW> tw = fold_tw(o, ft[k]);
-      lds[(c * YB + yy) * rs + x] = {(T)((TW)val * tw.x), (T)((TW)val * tw.y)};
-    }
+    scatter_parities<T, TW>(p, lds, yy * rs + x, YB * rs, ft[k], v[k]);
   }
   __syncthreads();
   lds_fft<T, TW, L0, -1>(lds, p.n_ch * YB, rs, 1);
@@ -402,10 +431,10 @@ __device__ void ph_expand_ifft_zy(const PipeP& p, const PipeBufs& B, long item, 
 // (1/8 folded into F0).  Octets partition the item's elements and each thread
 // owns whole octets, so the multiply runs in place between the forward and
 // the inverse transforms.
-template <typename T, typename TW, int L>
+template <typename T, typename TW, int L, int NT>
 __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds, cx<TW>* fq) {
   constexpr int rs = L + 1, NP = L * L, SL = L * rs, NF = 2 + 2 * L;
-  constexpr int NFT = 8 * NF, FIT = (NFT + kNT - 1) / kNT;
+  constexpr int NFT = 8 * NF, FIT = (NFT + NT - 1) / NT;
   const int b = (int)(item / p.qs_items);
   int r = (int)(item - (long)b * p.qs_items);
   int c = 0;
@@ -424,26 +453,26 @@ __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx
   cx<TW> fv[FIT];
 #pragma unroll
   for (int k = 0; k < FIT; ++k) {
-    const int i0 = threadIdx.x + k * kNT;
+    const int i0 = threadIdx.x + k * NT;
     const int i = i0 < NFT ? i0 : NFT - 1;  // unconditional load, clamped index
     const int sf = i / NF, j = i - sf * NF;
     const int d = j < 2 ? 0 : (j < 2 + L ? 1 : 2);
     const int q = j < 2 ? 2 * (j ? x1 : x0) + o0 : (j < 2 + L ? 2 * (j - 2) + o1 : 2 * (j - 2 - L) + o2);
     fv[k] = F[(d * 8 + sf) * Ms + q];
   }
-  plane_to_lds<T, L, L>(chan + (long)x0 * NP, lds);
-  if (ns == 2) plane_to_lds<T, L, L>(chan + (long)x1 * NP, lds + SL);
+  plane_to_lds<T, L, L, NT>(chan + (long)x0 * NP, lds);
+  if (ns == 2) plane_to_lds<T, L, L, NT>(chan + (long)x1 * NP, lds + SL);
 #pragma unroll
   for (int k = 0; k < FIT; ++k) {
-    const int i = threadIdx.x + k * kNT;
+    const int i = threadIdx.x + k * NT;
     if (i < NFT) fq[i] = fv[k];
   }
   __syncthreads();
-  lds_fft<T, TW, L, -1>(lds, ns * L, rs, 1);         // FFT y: rows of both planes
-  lds_fft2<T, TW, L, -1>(lds, L, 1, ns, SL, rs);      // FFT z: columns of both planes
+  lds_fft<T, TW, L, -1, NT>(lds, ns * L, rs, 1);         // FFT y: rows of both planes
+  lds_fft2<T, TW, L, -1, NT>(lds, L, 1, ns, SL, rs);      // FFT z: columns of both planes
   // octet representatives y <= ybar, z <= zbar
   const int ny = o1 ? (L + 1) / 2 : L / 2 + 1, nz = o2 ? (L + 1) / 2 : L / 2 + 1;
-  for (int oc = threadIdx.x; oc < ny * nz; oc += kNT) {
+  for (int oc = threadIdx.x; oc < ny * nz; oc += NT) {
     const int y = oc % ny, z = oc / ny;
     const int yb = o1 ? L - 1 - y : (L - y) % L, zb = o2 ? L - 1 - z : (L - z) % L;
     int pos[8];
@@ -495,10 +524,10 @@ __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx
     for (int o = 0; o < 8; ++o) lds[pos[o]] = {(T)acc[o].x, (T)acc[o].y};
   }
   __syncthreads();
-  lds_fft2<T, TW, L, +1>(lds, L, 1, ns, SL, rs);  // IFFT z
-  lds_fft<T, TW, L, +1>(lds, ns * L, rs, 1);      // IFFT y
-  lds_to_plane<T, L, L>(lds, chan + (long)x0 * NP);
-  if (ns == 2) lds_to_plane<T, L, L>(lds + SL, chan + (long)x1 * NP);
+  lds_fft2<T, TW, L, +1, NT>(lds, L, 1, ns, SL, rs);  // IFFT z
+  lds_fft<T, TW, L, +1, NT>(lds, ns * L, rs, 1);      // IFFT y
+  lds_to_plane<T, L, L, NT>(lds, chan + (long)x0 * NP);
+  if (ns == 2) lds_to_plane<T, L, L, NT>(lds + SL, chan + (long)x1 * NP);
   __syncthreads();
 }
 
@@ -552,17 +581,7 @@ __device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, in
     if (pt >= H0 * YB) continue;
     const int x = pt % H0, yy = pt / H0, y = y0 + yy;
     T v[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = T(0);
-    for (int c = 0; c < p.n_ch; ++c) {
-      const int o = p.offs[c];
-      const cx<TW> tw = fold_tw(o, ft[k]);
-      const cx<T> g = lds[(c * YB + yy) * rs + x];
-      const T val = (T)((TW)g.x * tw.x + (TW)g.y * tw.y);  // Re(g conj(tw))
-#pragma unroll
-      for (int e = 0; e < 8; ++e)
-        if (e == o) v[e] = val;
-    }
+    gather_parities<T, TW>(p, lds, yy * rs + x, YB * rs, ft[k], v);  // Re(g conj(tw))
     wht8(v);
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
@@ -617,17 +636,7 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
     if (pt >= H0 * YB) continue;
     const int x = pt % H0, yy = pt / H0, y = y0 + yy;
     T v[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = T(0);
-    for (int c = 0; c < p.n_ch; ++c) {
-      const int o = p.offs[c];
-      const cx<TW> tw = fold_tw(o, ft[k]);
-      const cx<T> g = lds[(c * YB + yy) * rs + x];
-      const T val = (T)((TW)g.x * tw.x + (TW)g.y * tw.y);
-#pragma unroll
-      for (int e = 0; e < 8; ++e)
-        if (e == o) v[e] = val;
-    }
+    gather_parities<T, TW>(p, lds, yy * rs + x, YB * rs, ft[k], v);
     wht8(v);
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
@@ -638,15 +647,7 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
     }
     wht8(v);
     // the entries of this thread's point only: no barrier needed before reuse
-    for (int c = 0; c < p.n_ch; ++c) {
-      const int o = p.offs[c];
-      T val = T(0);
-#pragma unroll
-      for (int e = 0; e < 8; ++e)
-        if (e == o) val = v[e];
-      const cx<TW> tw = fold_tw(o, ft[k]);
-      lds[(c * YB + yy) * rs + x] = {(T)((TW)val * tw.x), (T)((TW)val * tw.y)};
-    }
+    scatter_parities<T, TW>(p, lds, yy * rs + x, YB * rs, ft[k], v);
   }
   __syncthreads();
   lds_fft<T, TW, L0, -1>(lds, p.n_ch * YB, rs, 1);
@@ -741,11 +742,15 @@ __global__ void __launch_bounds__(256) k_expand(const cx<T>* __restrict__ grep, 
   ch[tid] = twmul<T, double, -1>(grep[(long)b * n_rep + (ent & 0xFFFFFu)], W[ent >> 20]);
 }
 
+// Threads per workgroup of the plane-pair kernel (512 gains 3 % at 2 fields
+// per launch and loses a third at 16: one workgroup per CU instead of two).
+constexpr int kQNT = 256;
+
 template <typename T, typename TW, int L>
-__global__ void __launch_bounds__(kNT) k_p2q(PipeP p, PipeBufs B) {
+__global__ void __launch_bounds__(kQNT) k_p2q(PipeP p, PipeBufs B) {
   __shared__ cx<T> lds[2 * L * (L + 1)];
   __shared__ cx<TW> fq[8 * (2 + 2 * L)];
-  ph_plane_pair_q<T, TW, L>(p, B, blockIdx.x, lds, fq);
+  ph_plane_pair_q<T, TW, L, kQNT>(p, B, blockIdx.x, lds, fq);
 }
 
 template <typename T, typename TW, int L0>
