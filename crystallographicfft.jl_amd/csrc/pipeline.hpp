@@ -420,6 +420,34 @@ __device__ void ph_expand_ifft_zy(const PipeP& p, const PipeBufs& B, long item, 
 }
 
 // ---- P2Q: FFT y, z + Q multiply + IFFT z, y on plane pairs (path QS) -------
+// Lines of work unit u along an axis of length L whose points q = 2k + off map
+// to kbar = (L - k) mod L (off = 0) or L - 1 - k (off = 1): a pair (a, b) of
+// distinct lines (pair = true) or self-images (pair = false).  For even L and
+// off = 0 the two self-paired lines 0 and L/2 share unit 0; for odd L the one
+// self-paired line is its own unit (a = b).  Units 0 .. (L + 1) / 2 - 1 cover
+// every line once.
+template <int L>
+__device__ __forceinline__ void unit_lines(int u, int off, int& a, int& b, bool& pair) {
+  if (off == 0) {
+    if (u == 0) {
+      a = 0;
+      b = (L % 2 == 0) ? L / 2 : 0;
+      pair = false;
+    } else {
+      a = u;
+      b = L - u;
+      pair = true;
+    }
+  } else if (L % 2 == 1 && u == (L - 1) / 2) {
+    a = b = u;
+    pair = false;
+  } else {
+    a = u;
+    b = L - 1 - u;
+    pair = true;
+  }
+}
+
 // Work item (b, c, x0): the planes x0 and x1, the image of x0 under
 // q0 -> -q0 mod M0 (one plane when x1 = x0).  The d = 8 representative ops are
 // diagonal, R_s = diag(+-1) with flip pattern s (bit i: axis i negated), so the
@@ -470,37 +498,57 @@ __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx
   __syncthreads();
   lds_fft<T, TW, L, -1, NT>(lds, ns * L, rs, 1);         // FFT y: rows of both planes
   lds_fft2<T, TW, L, -1, NT>(lds, L, 1, ns, SL, rs);      // FFT z: columns of both planes
-  // octet representatives y <= ybar, z <= zbar
-  const int ny = o1 ? (L + 1) / 2 : L / 2 + 1, nz = o2 ? (L + 1) / 2 : L / 2 + 1;
-  for (int oc = threadIdx.x; oc < ny * nz; oc += NT) {
-    const int y = oc % ny, z = oc / ny;
-    const int yb = o1 ? L - 1 - y : (L - y) % L, zb = o2 ? L - 1 - z : (L - z) % L;
-    int pos[8];
-    cx<T> v[8];
-#pragma unroll
-    for (int o = 0; o < 8; ++o) {
-      pos[o] = ((o & 1) && ns == 2 ? SL : 0) + ((o & 4) ? zb : z) * rs + ((o & 2) ? yb : y);
-      v[o] = lds[pos[o]];
-    }
-    cx<TW> acc[8];
-    if (p.qs_sep) {
-      // t_{s,d} depends on bit d of s only: Q_s(q) = prod_d g_d(s_d, q_d) with
-      // g_d(s_d, .) = F_d[s_d << d][.], and the octet sum factors into three
-      // two-point stages (z, y, x)
+  if (p.qs_sep) {
+    // t_{s,d} depends on bit d of s only: Q_s(q) = prod_d g_d(s_d, q_d) with
+    // g_d(s_d, .) = F_d[s_d << d][.], and the octet sum factors into three
+    // two-point stages (z, y, x).  Work units pair rows {z, zbar} and columns
+    // {y, ybar}; with offset 0 the two self-paired lines 0 and L/2 share one
+    // unit (each is its own partner), so an item has ((L + 1) / 2)^2 units
+    constexpr int NU = (L + 1) / 2;
+    for (int un = threadIdx.x; un < NU * NU; un += NT) {
+      int ya, yb, za, zb;
+      bool yp, zp;
+      unit_lines<L>(un % NU, o1, ya, yb, yp);
+      unit_lines<L>(un / NU, o2, za, zb, zp);
+      int pos[8];
       cx<TW> u[8];
 #pragma unroll
-      for (int o = 0; o < 8; ++o) u[o] = {(TW)v[o].x, (TW)v[o].y};
-      const cx<TW> gz[2][2] = {{fq[2 + L + z], fq[4 * NF + 2 + L + z]}, {fq[2 + L + zb], fq[4 * NF + 2 + L + zb]}};
-      const cx<TW> gy[2][2] = {{fq[2 + y], fq[2 * NF + 2 + y]}, {fq[2 + yb], fq[2 * NF + 2 + yb]}};
+      for (int o = 0; o < 8; ++o) {
+        pos[o] = ((o & 1) && ns == 2 ? SL : 0) + ((o & 4) ? zb : za) * rs + ((o & 2) ? yb : ya);
+        const cx<T> g = lds[pos[o]];
+        u[o] = {(TW)g.x, (TW)g.y};
+      }
+      const cx<TW> gz[2][2] = {{fq[2 + L + za], fq[4 * NF + 2 + L + za]}, {fq[2 + L + zb], fq[4 * NF + 2 + L + zb]}};
+      const cx<TW> gy[2][2] = {{fq[2 + ya], fq[2 * NF + 2 + ya]}, {fq[2 + yb], fq[2 * NF + 2 + yb]}};
       const cx<TW> gx[2][2] = {{fq[0], fq[NF]}, {fq[1], fq[NF + 1]}};
       cx<TW> w[8];
 #pragma unroll
-      for (int o = 0; o < 8; ++o) w[o] = cadd(cmul(gz[o >> 2][0], u[o]), cmul(gz[o >> 2][1], u[o ^ 4]));
+      for (int o = 0; o < 8; ++o)
+        w[o] = cadd(cmul(gz[o >> 2][0], u[o]), cmul(gz[o >> 2][1], zp ? u[o ^ 4] : u[o]));
 #pragma unroll
-      for (int o = 0; o < 8; ++o) u[o] = cadd(cmul(gy[(o >> 1) & 1][0], w[o]), cmul(gy[(o >> 1) & 1][1], w[o ^ 2]));
+      for (int o = 0; o < 8; ++o)
+        u[o] = cadd(cmul(gy[(o >> 1) & 1][0], w[o]), cmul(gy[(o >> 1) & 1][1], yp ? w[o ^ 2] : w[o]));
+      // a single plane (x0 = x1) loaded its entries twice: u[o ^ 1] is then u[o]'s twin
 #pragma unroll
-      for (int o = 0; o < 8; ++o) acc[o] = cadd(cmul(gx[o & 1][0], u[o]), cmul(gx[o & 1][1], u[o ^ 1]));
-    } else {
+      for (int o = 0; o < 8; ++o) {
+        const cx<TW> a = cadd(cmul(gx[o & 1][0], u[o]), cmul(gx[o & 1][1], u[o ^ 1]));
+        lds[pos[o]] = {(T)a.x, (T)a.y};
+      }
+    }
+  } else {
+    // octet representatives y <= ybar, z <= zbar
+    const int ny = o1 ? (L + 1) / 2 : L / 2 + 1, nz = o2 ? (L + 1) / 2 : L / 2 + 1;
+    for (int oc = threadIdx.x; oc < ny * nz; oc += NT) {
+      const int y = oc % ny, z = oc / ny;
+      const int yb = o1 ? L - 1 - y : (L - y) % L, zb = o2 ? L - 1 - z : (L - z) % L;
+      int pos[8];
+      cx<T> v[8];
+#pragma unroll
+      for (int o = 0; o < 8; ++o) {
+        pos[o] = ((o & 1) && ns == 2 ? SL : 0) + ((o & 4) ? zb : z) * rs + ((o & 2) ? yb : y);
+        v[o] = lds[pos[o]];
+      }
+      cx<TW> acc[8];
 #pragma unroll
       for (int o = 0; o < 8; ++o) acc[o] = {TW(0), TW(0)};
 #pragma unroll
@@ -518,10 +566,10 @@ __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx
           acc[o].y += qc.x * (TW)g.y + qc.y * (TW)g.x;
         }
       }
-    }
-    // coinciding positions (self-images) receive identical values
+      // coinciding positions (self-images) receive identical values
 #pragma unroll
-    for (int o = 0; o < 8; ++o) lds[pos[o]] = {(T)acc[o].x, (T)acc[o].y};
+      for (int o = 0; o < 8; ++o) lds[pos[o]] = {(T)acc[o].x, (T)acc[o].y};
+    }
   }
   __syncthreads();
   lds_fft2<T, TW, L, +1, NT>(lds, L, 1, ns, SL, rs);  // IFFT z
