@@ -426,6 +426,14 @@ __device__ void ph_expand_ifft_zy(const PipeP& p, const PipeBufs& B, long item, 
 // off = 0 the two self-paired lines 0 and L/2 share unit 0; for odd L the one
 // self-paired line is its own unit (a = b).  Units 0 .. (L + 1) / 2 - 1 cover
 // every line once.
+// c ? a : b on the components (a select of two array elements written as
+// `c ? arr[i] : arr[j]` can become a select of addresses, which moves the
+// array to scratch memory)
+template <typename U>
+__device__ __forceinline__ cx<U> csel(bool c, cx<U> a, cx<U> b) {
+  return {c ? a.x : b.x, c ? a.y : b.y};
+}
+
 template <int L>
 __device__ __forceinline__ void unit_lines(int u, int off, int& a, int& b, bool& pair) {
   if (off == 0) {
@@ -524,10 +532,10 @@ __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx
       cx<TW> w[8];
 #pragma unroll
       for (int o = 0; o < 8; ++o)
-        w[o] = cadd(cmul(gz[o >> 2][0], u[o]), cmul(gz[o >> 2][1], zp ? u[o ^ 4] : u[o]));
+        w[o] = cadd(cmul(gz[o >> 2][0], u[o]), cmul(gz[o >> 2][1], csel(zp, u[o ^ 4], u[o])));
 #pragma unroll
       for (int o = 0; o < 8; ++o)
-        u[o] = cadd(cmul(gy[(o >> 1) & 1][0], w[o]), cmul(gy[(o >> 1) & 1][1], yp ? w[o ^ 2] : w[o]));
+        u[o] = cadd(cmul(gy[(o >> 1) & 1][0], w[o]), cmul(gy[(o >> 1) & 1][1], csel(yp, w[o ^ 2], w[o])));
       // a single plane (x0 = x1) loaded its entries twice: u[o ^ 1] is then u[o]'s twin
 #pragma unroll
       for (int o = 0; o < 8; ++o) {
@@ -536,34 +544,58 @@ __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx
       }
     }
   } else {
-    // octet representatives y <= ybar, z <= zbar
-    const int ny = o1 ? (L + 1) / 2 : L / 2 + 1, nz = o2 ? (L + 1) / 2 : L / 2 + 1;
-    for (int oc = threadIdx.x; oc < ny * nz; oc += NT) {
-      const int y = oc % ny, z = oc / ny;
-      const int yb = o1 ? L - 1 - y : (L - y) % L, zb = o2 ? L - 1 - z : (L - z) % L;
+    // the same work units; a self-paired line is its own image under the flip
+    // of its axis, so the terms of sign patterns that differ only in that axis
+    // read the same entry
+    constexpr int NU = (L + 1) / 2;
+    for (int un = threadIdx.x; un < NU * NU; un += NT) {
+      int ya, yb, za, zb;
+      bool yp, zp;
+      unit_lines<L>(un % NU, o1, ya, yb, yp);
+      unit_lines<L>(un / NU, o2, za, zb, zp);
       int pos[8];
       cx<T> v[8];
 #pragma unroll
       for (int o = 0; o < 8; ++o) {
-        pos[o] = ((o & 1) && ns == 2 ? SL : 0) + ((o & 4) ? zb : z) * rs + ((o & 2) ? yb : y);
+        pos[o] = ((o & 1) && ns == 2 ? SL : 0) + ((o & 4) ? zb : za) * rs + ((o & 2) ? yb : ya);
         v[o] = lds[pos[o]];
       }
       cx<TW> acc[8];
 #pragma unroll
       for (int o = 0; o < 8; ++o) acc[o] = {TW(0), TW(0)};
+      // sign patterns in four classes by their y and z flips; within a class
+      // the entries src[k] = v[k with the flips of self-paired axes undone]
+      // are read at the static index o ^ sf
 #pragma unroll
-      for (int sf = 0; sf < 8; ++sf) {
-        const cx<TW>* f = fq + sf * NF;
-        const cx<TW> fx0 = f[0], fx1 = f[1];
-        const cx<TW> fy0 = f[2 + y], fy1 = f[2 + yb];
-        const cx<TW> fz0 = f[2 + L + z], fz1 = f[2 + L + zb];
-        const cx<TW> fxy[4] = {cmul(fx0, fy0), cmul(fx1, fy0), cmul(fx0, fy1), cmul(fx1, fy1)};
+      for (int cls = 0; cls < 4; ++cls) {
+        cx<T> src[8];
 #pragma unroll
-        for (int o = 0; o < 8; ++o) {
-          const cx<TW> qc = cmul(fxy[o & 3], (o & 4) ? fz1 : fz0);
-          const cx<T> g = v[o ^ sf];
-          acc[o].x += qc.x * (TW)g.x - qc.y * (TW)g.y;
-          acc[o].y += qc.x * (TW)g.y + qc.y * (TW)g.x;
+        for (int k = 0; k < 8; ++k) {
+          if (cls == 0) {
+            src[k] = v[k];
+          } else if (cls == 1) {
+            src[k] = csel(yp, v[k], v[k ^ 2]);
+          } else if (cls == 2) {
+            src[k] = csel(zp, v[k], v[k ^ 4]);
+          } else {
+            src[k] = csel(zp, csel(yp, v[k], v[k ^ 2]), csel(yp, v[k ^ 4], v[k ^ 6]));
+          }
+        }
+#pragma unroll
+        for (int xb = 0; xb < 2; ++xb) {
+          const int sf = 2 * cls + xb;
+          const cx<TW>* f = fq + sf * NF;
+          const cx<TW> fx0 = f[0], fx1 = f[1];
+          const cx<TW> fy0 = f[2 + ya], fy1 = f[2 + yb];
+          const cx<TW> fz0 = f[2 + L + za], fz1 = f[2 + L + zb];
+          const cx<TW> fxy[4] = {cmul(fx0, fy0), cmul(fx1, fy0), cmul(fx0, fy1), cmul(fx1, fy1)};
+#pragma unroll
+          for (int o = 0; o < 8; ++o) {
+            const cx<TW> qc = cmul(fxy[o & 3], (o & 4) ? fz1 : fz0);
+            const cx<T> g = src[o ^ sf];
+            acc[o].x += qc.x * (TW)g.x - qc.y * (TW)g.y;
+            acc[o].y += qc.x * (TW)g.y + qc.y * (TW)g.x;
+          }
         }
       }
       // coinciding positions (self-images) receive identical values

@@ -116,7 +116,8 @@ def test_orbit_reduced_q_fp64(sg, N):
 @pytest.mark.parametrize("sg,N", [(229, (16, 16, 16)), (229, (32, 32, 32)), (230, (24, 24, 24)),
                                   (70, (16, 16, 16)), (47, (16, 16, 16)), (221, (24, 24, 24)),
                                   (225, (32, 32, 32)), (223, (24, 24, 24)), (227, (32, 32, 32)),
-                                  (47, (12, 12, 12)), (229, (12, 12, 12)), (230, (48, 48, 48))])
+                                  (47, (12, 12, 12)), (229, (12, 12, 12)), (230, (48, 48, 48)),
+                                  (70, (32, 32, 32)), (227, (16, 16, 16))])
 def test_plane_pair_q_fp64(sg, N):
     """Path QS (FFT y, z + separable Q multiply over the sign images + IFFT z, y
     in one kernel per plane pair) = full-grid FFT.K.IFFT and = the oracle's
