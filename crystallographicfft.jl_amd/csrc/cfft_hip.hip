@@ -11,6 +11,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/cfft_hip.h"
@@ -89,6 +90,11 @@ bool is_q_path(const cfft_plan* p) {
 }
 // paths whose steps are the fused launch chain (P51 between steps)
 bool chain_path(const cfft_plan* p) { return !is_q_path(p) || is_qr(p) || is_qs(p); }
+
+// Mixed precision on path QS: the fold, unfold, transform and Q-multiply
+// arithmetic of its kernels in FP32 (tables stay FP64 in memory and are
+// rounded where they are used); the other paths form twiddle products in FP64.
+bool fp32_arith(const cfft_plan* p) { return p->prec == CFFT_MIXED && p->path == CFFT_PATH_QS; }
 
 
 template <typename T>
@@ -606,10 +612,22 @@ template <typename T>
 void launch_p1(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipStream_t s) {
   StageTimer st(p, s, CFFT_STAGE_FOLD);
   const unsigned g = (unsigned)n_tiles(p, pp);
+  bool f32 = false;  // FP32-arithmetic kernels are instantiated for FP32 data only
+  if constexpr (std::is_same<T, float>::value) {
+    if (fp32_arith(p)) {
+#define CFFT_P1(L) \
+  case L: hipLaunchKernelGGL((k_p1<T, float, L>), dim3(g), dim3(kNT), 0, s, pp, B, step); break;
+      CFFT_LEN_SWITCH(p->H[0], CFFT_P1)
+#undef CFFT_P1
+      f32 = true;
+    }
+  }
+  if (!f32) {
 #define CFFT_P1(L) \
   case L: hipLaunchKernelGGL((k_p1<T, double, L>), dim3(g), dim3(kNT), 0, s, pp, B, step); break;
-  CFFT_LEN_SWITCH(p->H[0], CFFT_P1)
+    CFFT_LEN_SWITCH(p->H[0], CFFT_P1)
 #undef CFFT_P1
+  }
   launch_check<T>("k_p1");
 }
 
@@ -669,10 +687,22 @@ template <typename T>
 void launch_p5(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipStream_t s) {
   StageTimer st(p, s, CFFT_STAGE_UNFOLD);
   const unsigned g = (unsigned)n_tiles(p, pp);
+  bool f32 = false;  // FP32-arithmetic kernels are instantiated for FP32 data only
+  if constexpr (std::is_same<T, float>::value) {
+    if (fp32_arith(p)) {
+#define CFFT_P5(L) \
+  case L: hipLaunchKernelGGL((k_p5<T, float, L>), dim3(g), dim3(kNT), 0, s, pp, B, step); break;
+      CFFT_LEN_SWITCH(p->H[0], CFFT_P5)
+#undef CFFT_P5
+      f32 = true;
+    }
+  }
+  if (!f32) {
 #define CFFT_P5(L) \
   case L: hipLaunchKernelGGL((k_p5<T, double, L>), dim3(g), dim3(kNT), 0, s, pp, B, step); break;
-  CFFT_LEN_SWITCH(p->H[0], CFFT_P5)
+    CFFT_LEN_SWITCH(p->H[0], CFFT_P5)
 #undef CFFT_P5
+  }
   launch_check<T>("k_p5");
 }
 
@@ -715,10 +745,22 @@ template <typename T>
 void launch_p2q(cfft_plan* p, const PipeP& pp, const PipeBufs& B, hipStream_t s) {
   StageTimer st(p, s, CFFT_STAGE_PLANE_Q);
   const unsigned g = (unsigned)((long)p->qs_items * p->batch);
+  bool f32 = false;  // FP32-arithmetic kernels are instantiated for FP32 data only
+  if constexpr (std::is_same<T, float>::value) {
+    if (fp32_arith(p)) {
+#define CFFT_P2Q(L) \
+  case L: hipLaunchKernelGGL((k_p2q<T, float, L>), dim3(g), dim3(kQNT), 0, s, pp, B); break;
+      CFFT_LEN_SWITCH(p->H[1], CFFT_P2Q)
+#undef CFFT_P2Q
+      f32 = true;
+    }
+  }
+  if (!f32) {
 #define CFFT_P2Q(L) \
   case L: hipLaunchKernelGGL((k_p2q<T, double, L>), dim3(g), dim3(kQNT), 0, s, pp, B); break;
-  CFFT_LEN_SWITCH(p->H[1], CFFT_P2Q)
+    CFFT_LEN_SWITCH(p->H[1], CFFT_P2Q)
 #undef CFFT_P2Q
+  }
   launch_check<T>("k_p2q");
 }
 
@@ -726,10 +768,22 @@ template <typename T>
 void launch_p51(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipStream_t s) {
   StageTimer st(p, s, CFFT_STAGE_UNFOLD_FOLD);
   const unsigned g = (unsigned)n_tiles(p, pp);
+  bool f32 = false;  // FP32-arithmetic kernels are instantiated for FP32 data only
+  if constexpr (std::is_same<T, float>::value) {
+    if (fp32_arith(p)) {
+#define CFFT_P51(L) \
+  case L: hipLaunchKernelGGL((k_p51<T, float, L>), dim3(g), dim3(kNT), 0, s, pp, B, step); break;
+      CFFT_LEN_SWITCH(p->H[0], CFFT_P51)
+#undef CFFT_P51
+      f32 = true;
+    }
+  }
+  if (!f32) {
 #define CFFT_P51(L) \
   case L: hipLaunchKernelGGL((k_p51<T, double, L>), dim3(g), dim3(kNT), 0, s, pp, B, step); break;
-  CFFT_LEN_SWITCH(p->H[0], CFFT_P51)
+    CFFT_LEN_SWITCH(p->H[0], CFFT_P51)
 #undef CFFT_P51
+  }
   launch_check<T>("k_p51");
 }
 

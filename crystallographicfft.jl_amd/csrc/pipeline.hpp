@@ -211,10 +211,19 @@ __device__ __forceinline__ cx<TW> fold_tw(int o, const FoldTw<TW>& f) {
   return t;
 }
 
+// FP64 table entry in the arithmetic precision TW
 template <typename TW>
-__device__ __forceinline__ FoldTw<TW> load_fold_tw(const cx<TW>* __restrict__ twM, int x, int y, int z,
+__device__ __forceinline__ cx<TW> tab(const cx<double>* __restrict__ t, long i) {
+  const cx<double> v = t[i];
+  return {(TW)v.x, (TW)v.y};
+}
+
+// The fold twiddle table is FP64 in both precisions; TW is the precision of
+// the products formed with it.
+template <typename TW>
+__device__ __forceinline__ FoldTw<TW> load_fold_tw(const cx<double>* __restrict__ twM, int x, int y, int z,
                                                    const PipeP& p) {
-  return {twM[x], twM[p.M[0] + y], twM[p.M[0] + p.M[1] + z]};
+  return {tab<TW>(twM, x), tab<TW>(twM, p.M[0] + y), tab<TW>(twM, p.M[0] + p.M[1] + z)};
 }
 
 // Channel values of one tile point (LDS entry base + c * cstride of channel c)
@@ -267,7 +276,7 @@ __device__ void ph_fold_fftx(const PipeP& p, const PipeBufs& B, long item, int s
   const long M3 = (long)p.M[0] * p.M[1] * p.M[2];
   const T* fb = static_cast<const T*>(B.f0_in) + b * M3;
   const T* Eb = pipe_E<T>(p, b, step, M3);
-  const cx<TW>* twM = static_cast<const cx<TW>*>(B.twM);
+  const cx<double>* twM = static_cast<const cx<double>*>(B.twM);
   const int rs = H0 + 1;
   constexpr int NPT = TilePts<L0>::NPT;
   T v[NPT][8];
@@ -277,7 +286,7 @@ __device__ void ph_fold_fftx(const PipeP& p, const PipeBufs& B, long item, int s
     const int pt0 = threadIdx.x + k * kNT;
     const int pt = pt0 < H0 * YB ? pt0 : H0 * YB - 1;  // unconditional loads
     const int x = pt % H0, y = y0 + pt / H0;
-    ft[k] = load_fold_tw(twM, x, y, z, p);
+    ft[k] = load_fold_tw<TW>(twM, x, y, z, p);
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const long idx = corner_index(p, x, y, z, e);
@@ -484,7 +493,7 @@ __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx
   const int ns = x1 == x0 ? 1 : 2;
   cx<T>* chan = static_cast<cx<T>*>(B.ch) + ((long)b * p.n_ch + c) * ((long)L * NP);
   // this item's axis factors: fq[s * NF + j], j = 0, 1: x0, x1; 2 + y; 2 + L + z
-  const cx<TW>* F = static_cast<const cx<TW>*>(B.qsep);
+  const cx<double>* F = static_cast<const cx<double>*>(B.qsep);  // FP64 in both precisions
   const int Ms = p.M[0];
   cx<TW> fv[FIT];
 #pragma unroll
@@ -494,7 +503,7 @@ __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx
     const int sf = i / NF, j = i - sf * NF;
     const int d = j < 2 ? 0 : (j < 2 + L ? 1 : 2);
     const int q = j < 2 ? 2 * (j ? x1 : x0) + o0 : (j < 2 + L ? 2 * (j - 2) + o1 : 2 * (j - 2 - L) + o2);
-    fv[k] = F[(d * 8 + sf) * Ms + q];
+    fv[k] = tab<TW>(F, (d * 8 + sf) * Ms + q);
   }
   plane_to_lds<T, L, L, NT>(chan + (long)x0 * NP, lds);
   if (ns == 2) plane_to_lds<T, L, L, NT>(chan + (long)x1 * NP, lds + SL);
@@ -639,7 +648,7 @@ __device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, in
   const long M3 = (long)p.M[0] * p.M[1] * p.M[2];
   T* fb = static_cast<T*>(B.f0_out) + b * M3;
   const T* Eb = pipe_E<T>(p, b, step, M3);
-  const cx<TW>* twM = static_cast<const cx<TW>*>(B.twM);
+  const cx<double>* twM = static_cast<const cx<double>*>(B.twM);
   constexpr int NPT = TilePts<L0>::NPT;
   T ev[NPT][8];
   FoldTw<TW> ft[NPT];
@@ -648,7 +657,7 @@ __device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, in
     const int pt0 = threadIdx.x + k * kNT;
     const int pt = pt0 < H0 * YB ? pt0 : H0 * YB - 1;  // unconditional loads
     const int x = pt % H0, y = y0 + pt / H0;
-    ft[k] = load_fold_tw(twM, x, y, z, p);
+    ft[k] = load_fold_tw<TW>(twM, x, y, z, p);
 #pragma unroll
     for (int e = 0; e < 8; ++e) ev[k][e] = Eb ? Eb[corner_index(p, x, y, z, e)] : T(1);
   }
@@ -689,7 +698,7 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
   T* fb = static_cast<T*>(B.f0_out) + b * M3;
   const T* Eo = pipe_E<T>(p, b, step, M3);
   const T* En = pipe_E<T>(p, b, step + 1, M3);
-  const cx<TW>* twM = static_cast<const cx<TW>*>(B.twM);
+  const cx<double>* twM = static_cast<const cx<double>*>(B.twM);
   constexpr int NPT = TilePts<L0>::NPT;
   T eo[NPT][8], en[NPT][8];
   FoldTw<TW> ft[NPT];
@@ -699,7 +708,7 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
     const int pt0 = threadIdx.x + k * kNT;
     const int pt = pt0 < H0 * YB ? pt0 : H0 * YB - 1;  // unconditional loads
     const int x = pt % H0, y = y0 + pt / H0;
-    ft[k] = load_fold_tw(twM, x, y, z, p);
+    ft[k] = load_fold_tw<TW>(twM, x, y, z, p);
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const long idx = corner_index(p, x, y, z, e);

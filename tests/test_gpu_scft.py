@@ -104,8 +104,11 @@ def test_sweep_plane_pair_q_vs_oracle(sg, N, fuse, graph):
     assert abs(Q1 - Q2) / abs(Q1) < 1e-12
 
 
-def test_sweep_mixed_vs_fp64():
-    q, qd, rq, rqd, (Q1, Q2) = sweep_case(229, (32, 32, 32), 0.3, 20, cf.MIXED)
+@pytest.mark.parametrize("path", [cf.PATH_AUTO, cf.PATH_QS])
+def test_sweep_mixed_vs_fp64(path):
+    """Mixed precision (FP32 data; on path QS also FP32 twiddle, fold and Q
+    arithmetic from the FP64 tables) against the FP64 oracle."""
+    q, qd, rq, rqd, (Q1, Q2) = sweep_case(229, (32, 32, 32), 0.3, 20, cf.MIXED, path=path)
     assert np.max(np.abs(q - rq)) / np.max(np.abs(rq)) < 2e-5
     assert abs(Q1 - float(np.mean(rq))) / abs(Q1) < 1e-6
 
