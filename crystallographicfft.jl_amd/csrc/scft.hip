@@ -83,13 +83,28 @@ __global__ void __launch_bounds__(kRB) k_density(const T* __restrict__ hist, lon
   block_sum<2>(acc, part);
 }
 
-// phi = c u (in place); xi = (wA + wB - chiN)/2; targets tA = chiN phiB + xi,
-// tB = chiN phiA + xi; d = t - w.  Partial sums: phiA phiB, wA phiA, wB phiB,
-// phiA + phiB, |d|^2, |w|^2, phiA.
-__global__ void __launch_bounds__(kRB) k_update(const double* __restrict__ w, double* __restrict__ phi,
-                                                double* __restrict__ d, double c, double chiN, long M3,
-                                                double* __restrict__ part) {
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+// Field update and the newest row of the Anderson Gram matrix in one pass.
+// Q = qtot[0] / M3 (the total of q(ns)); phi <- (ds / Q) u; d = t - w; the
+// history slot `cur` of the rings receives (w, d).  With xi = (wA + wB - chiN)/2
+// the targets are tA = chiN phiB + xi, tB = chiN phiA + xi.  Partial sums per
+// block: phiA phiB, wA phiA, wB phiB, phiA + phiB, |d|^2, |w|^2, phiA, then
+// <d, D_k> over the ring slots gs.slot (for the newest, cur, that is <d, d>).
+struct GramSlots {
+  int n;
+  int cur;
+  int slot[kMaxHist];
+};
+constexpr int kNU = 7 + kMaxHist;  // partial sums per block of k_update_gram
+__global__ void __launch_bounds__(kRB) k_update_gram(const double* __restrict__ w, double* __restrict__ phi,
+                                                     double* __restrict__ d, const double* __restrict__ qtot,
+                                                     double ds, double chiN, long M3,
+                                                     double* __restrict__ Wr, double* __restrict__ Dr, GramSlots gs,
+                                                     double* __restrict__ part) {
+  const double c = ds / (qtot[0] / (double)M3);
+  const long n2 = 2 * M3;
+  double acc[kNU];
+#pragma unroll
+  for (int k = 0; k < kNU; ++k) acc[k] = 0.0;
   for (long x = blockIdx.x * (long)kRB + threadIdx.x; x < M3; x += (long)gridDim.x * kRB) {
     const double pA = c * phi[x], pB = c * phi[M3 + x];
     phi[x] = pA;
@@ -99,6 +114,10 @@ __global__ void __launch_bounds__(kRB) k_update(const double* __restrict__ w, do
     const double dA = chiN * pB + xi - wA, dB = chiN * pA + xi - wB;
     d[x] = dA;
     d[M3 + x] = dB;
+    Wr[gs.cur * n2 + x] = wA;
+    Wr[gs.cur * n2 + M3 + x] = wB;
+    Dr[gs.cur * n2 + x] = dA;
+    Dr[gs.cur * n2 + M3 + x] = dB;
     acc[0] += pA * pB;
     acc[1] += wA * pA;
     acc[2] += wB * pB;
@@ -106,24 +125,26 @@ __global__ void __launch_bounds__(kRB) k_update(const double* __restrict__ w, do
     acc[4] += dA * dA + dB * dB;
     acc[5] += wA * wA + wB * wB;
     acc[6] += pA;
-  }
-  block_sum<7>(acc, part);
-}
-
-// Partial dot products of v with the ns slots of `ring` (slot k at ring + k n).
-__global__ void __launch_bounds__(kRB) k_dots(const double* __restrict__ v, const double* __restrict__ ring,
-                                              const int* __restrict__ slots, int nslots, long n,
-                                              double* __restrict__ part) {
-  double acc[kMaxHist];
-#pragma unroll
-  for (int k = 0; k < kMaxHist; ++k) acc[k] = 0.0;
-  for (long x = blockIdx.x * (long)kRB + threadIdx.x; x < n; x += (long)gridDim.x * kRB) {
-    const double a = v[x];
 #pragma unroll
     for (int k = 0; k < kMaxHist; ++k)
-      if (k < nslots) acc[k] += a * ring[slots[k] * n + x];
+      if (k < gs.n) {
+        const int sl = gs.slot[k];
+        acc[7 + k] += sl == gs.cur ? dA * dA + dB * dB
+                                   : dA * Dr[sl * n2 + x] + dB * Dr[sl * n2 + M3 + x];
+      }
   }
-  block_sum<kMaxHist>(acc, part);
+  block_sum<kNU>(acc, part);
+}
+
+// Totals of NV partial sums over nb blocks, each summed in block order by one
+// thread (the same association as a sequential host sum).
+template <int NV>
+__global__ void k_total(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= NV) return;
+  double a = 0.0;
+  for (int b = 0; b < nb; ++b) a += part[b * NV + k];
+  out[k] = a;
 }
 
 struct MixP {
@@ -204,8 +225,8 @@ struct cfft_scft {
   double f = 0.0, chiN = 0.0, ds = 0.0;
   double lam = 1.0, lam_simple = 0.1, simple_above = 1e-2;
   hipStream_t stream = nullptr;
-  DevBuf hist, E, w, phi, d, Wr, Dr, wa, wb, part, slots_dev;
-  std::vector<double> part_h;
+  DevBuf hist, E, w, phi, d, Wr, Dr, wa, wb, part, dpart, tot;
+  double* tot_h = nullptr;  // pinned: totals of the density and update/Gram partials
   // Anderson history: ring slots (newest last), Gram matrix G[slot][slot] = <d_i, d_j>/n
   std::vector<int> order;
   std::vector<double> G;
@@ -213,7 +234,11 @@ struct cfft_scft {
   int iters = 0;
   double last[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   ~cfft_scft() {
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+    if (tot_h) (void)hipHostFree(tot_h);
   }
 };
 
@@ -221,20 +246,10 @@ namespace {
 
 unsigned red_blocks(long n) { return (unsigned)std::min<long>(kMaxRed, std::max<long>(1, nblk(n, kRB))); }
 
-// Sum NV partials per block over `nb` blocks, in FP64, in block order.
-template <int NV>
-void reduce_host(cfft_scft* c, unsigned nb, double* out) {
-  hip_check(hipMemcpyAsync(c->part_h.data(), c->part.p, sizeof(double) * NV * nb, hipMemcpyDeviceToHost,
-                           c->stream),
-            "D2H partials");
-  hip_check(hipStreamSynchronize(c->stream), "sync");
-  for (int k = 0; k < NV; ++k) out[k] = 0.0;
-  for (unsigned b = 0; b < nb; ++b)
-    for (int k = 0; k < NV; ++k) out[k] += c->part_h[b * NV + k];
-}
-
+// Sweep and unnormalised densities; the partial sums of q(ns), q+(ns) stay on
+// the device (dpart, *dnb blocks).
 template <typename T>
-void sweep_and_densities(cfft_scft* c, double* Qs) {
+void sweep_and_densities(cfft_scft* c, int* dnb) {
   const long M3 = c->M3;
   const long n2 = 2 * M3;
   T* E = static_cast<T*>(c->E.p);
@@ -253,57 +268,61 @@ void sweep_and_densities(cfft_scft* c, double* Qs) {
   const unsigned nb = red_blocks(M3);
   hipLaunchKernelGGL(k_density<T>, dim3(nb), dim3(kRB), 0, c->stream, static_cast<const T*>(c->hist.p), M3,
                      c->ns, c->nA, static_cast<const double*>(c->wa.p), static_cast<const double*>(c->wb.p),
-                     static_cast<double*>(c->phi.p), static_cast<double*>(c->part.p));
+                     static_cast<double*>(c->phi.p), static_cast<double*>(c->dpart.p));
   hip_check(hipGetLastError(), "k_density");
-  double s[2];
-  reduce_host<2>(c, nb, s);
-  Qs[0] = s[0] / M3;
-  Qs[1] = s[1] / M3;
+  *dnb = (int)nb;
 }
 
 // One SCFT iteration; out = {Q, Q+, F, residual, <phiA + phiB>, <phiA>, mixing, k}.
+// One host synchronisation per iteration (the totals of the partial sums);
+// the mixing kernel is left in flight on the context's stream, ahead of the
+// next iteration's sweep.
 void iterate(cfft_scft* c, double* out) {
   const long M3 = c->M3, n2 = 2 * M3;
-  double Q[2];
-  if (c->prec == CFFT_FP64) sweep_and_densities<double>(c, Q);
-  else sweep_and_densities<float>(c, Q);
-  require(Q[0] > 0.0 && std::isfinite(Q[0]), CFFT_ERR_INVALID, "partition function is not positive");
-  unsigned nb = red_blocks(M3);
-  hipLaunchKernelGGL(k_update, dim3(nb), dim3(kRB), 0, c->stream, static_cast<const double*>(c->w.p),
-                     static_cast<double*>(c->phi.p), static_cast<double*>(c->d.p), c->ds / Q[0], c->chiN, M3,
-                     static_cast<double*>(c->part.p));
-  hip_check(hipGetLastError(), "k_update");
-  double s[7];
-  reduce_host<7>(c, nb, s);
-  for (auto& v : s) v /= M3;
-  const double F = -std::log(Q[0]) + c->chiN * s[0] - s[1] - s[2];
-  const double res = std::sqrt(s[4] / std::max(s[5], 1e-300)) + std::fabs(s[3] - 1.0);
-  // ---- Anderson history: copy w, d into the next slot; Gram row of d ----
+  int dnb = 0;
+  if (c->prec == CFFT_FP64) sweep_and_densities<double>(c, &dnb);
+  else sweep_and_densities<float>(c, &dnb);
+  // ---- field update + Anderson history slot + Gram row, one pass ----
   const int slot = c->next_slot;
   c->next_slot = (c->next_slot + 1) % (c->m + 1);
-  hip_check(hipMemcpyAsync(static_cast<double*>(c->Wr.p) + slot * n2, c->w.p, 8 * n2, hipMemcpyDeviceToDevice,
-                           c->stream),
-            "copy w");
-  hip_check(hipMemcpyAsync(static_cast<double*>(c->Dr.p) + slot * n2, c->d.p, 8 * n2, hipMemcpyDeviceToDevice,
-                           c->stream),
-            "copy d");
   c->order.erase(std::remove(c->order.begin(), c->order.end(), slot), c->order.end());
   c->order.push_back(slot);
   const int ns = (int)c->order.size();
+  GramSlots gs{};
+  gs.n = ns;
+  gs.cur = slot;
+  for (int i = 0; i < ns; ++i) gs.slot[i] = c->order[i];
+  double* tot = static_cast<double*>(c->tot.p);
+  hipLaunchKernelGGL(k_total<2>, dim3(1), dim3(64), 0, c->stream, static_cast<const double*>(c->dpart.p), dnb,
+                     tot);
+  hip_check(hipGetLastError(), "k_total");
+  const unsigned nb = red_blocks(M3);
+  hipLaunchKernelGGL(k_update_gram, dim3(nb), dim3(kRB), 0, c->stream, static_cast<const double*>(c->w.p),
+                     static_cast<double*>(c->phi.p), static_cast<double*>(c->d.p), static_cast<const double*>(tot),
+                     c->ds, c->chiN, M3, static_cast<double*>(c->Wr.p), static_cast<double*>(c->Dr.p), gs,
+                     static_cast<double*>(c->part.p));
+  hip_check(hipGetLastError(), "k_update_gram");
+  hipLaunchKernelGGL(k_total<kNU>, dim3(1), dim3(64), 0, c->stream, static_cast<const double*>(c->part.p),
+                     (int)nb, tot + 2);
+  hip_check(hipGetLastError(), "k_total");
+  hip_check(hipMemcpyAsync(c->tot_h, tot, sizeof(double) * (2 + kNU), hipMemcpyDeviceToHost, c->stream),
+            "D2H totals");
+  hip_check(hipStreamSynchronize(c->stream), "sync");
+  const double Q[2] = {c->tot_h[0] / M3, c->tot_h[1] / M3};
+  if (!(Q[0] > 0.0 && std::isfinite(Q[0]))) {
+    c->order.pop_back();  // the newest slot holds no valid entry
+    throw Error{CFFT_ERR_INVALID, "partition function is not positive"};
+  }
+  double s[7];
+  for (int k = 0; k < 7; ++k) s[k] = c->tot_h[2 + k] / M3;
+  const double F = -std::log(Q[0]) + c->chiN * s[0] - s[1] - s[2];
+  const double res = std::sqrt(s[4] / std::max(s[5], 1e-300)) + std::fabs(s[3] - 1.0);
   {
-    std::vector<int> sl(c->order);
-    hip_check(hipMemcpyAsync(c->slots_dev.p, sl.data(), 4 * ns, hipMemcpyHostToDevice, c->stream), "slots");
-    nb = red_blocks(n2);
-    hipLaunchKernelGGL(k_dots, dim3(nb), dim3(kRB), 0, c->stream, static_cast<const double*>(c->d.p),
-                       static_cast<const double*>(c->Dr.p), static_cast<const int*>(c->slots_dev.p), ns, n2,
-                       static_cast<double*>(c->part.p));
-    hip_check(hipGetLastError(), "k_dots");
-    double g[kMaxHist];
-    reduce_host<kMaxHist>(c, nb, g);
     const int S = c->m + 1;
     for (int i = 0; i < ns; ++i) {
-      c->G[slot * S + c->order[i]] = g[i] / n2;
-      c->G[c->order[i] * S + slot] = g[i] / n2;
+      const double g = c->tot_h[2 + 7 + i] / n2;
+      c->G[slot * S + c->order[i]] = g;
+      c->G[c->order[i] * S + slot] = g;
     }
   }
   // ---- mixing ----
@@ -339,7 +358,6 @@ void iterate(cfft_scft* c, double* out) {
   hipLaunchKernelGGL(k_mix, dim3(nblk(n2, 256)), dim3(256), 0, c->stream, static_cast<double*>(c->w.p),
                      static_cast<const double*>(c->Wr.p), static_cast<const double*>(c->Dr.p), mp, n2);
   hip_check(hipGetLastError(), "k_mix");
-  hip_check(hipStreamSynchronize(c->stream), "sync");
   ++c->iters;
   const double o[8] = {Q[0], Q[1], F, res, s[3], s[6], mode, (double)mp.k};
   std::memcpy(c->last, o, sizeof(o));
@@ -382,9 +400,11 @@ int cfft_scft_create(cfft_plan* plan, double f, double chiN, int32_t ns, int32_t
     c->d.alloc(8 * n2);
     c->Wr.alloc(8 * n2 * (m_hist + 1));
     c->Dr.alloc(8 * n2 * (m_hist + 1));
-    c->part.alloc(8 * kMaxRed * kMaxHist);
-    c->part_h.resize(kMaxRed * kMaxHist);
-    c->slots_dev.alloc(4 * kMaxHist);
+    c->part.alloc(8 * kMaxRed * kNU);
+    c->dpart.alloc(8 * kMaxRed * 2);
+    c->tot.alloc(8 * (2 + kNU));
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->tot_h), sizeof(double) * (2 + kNU), hipHostMallocDefault),
+              "hipHostMalloc");
     c->G.assign((m_hist + 1) * (m_hist + 1), 0.0);
     const std::vector<double> wa = simpson(c->nA), wb = simpson(ns - c->nA);
     c->wa.alloc(8 * (size_t)(ns + 1));
@@ -475,6 +495,7 @@ int cfft_scft_get(cfft_scft* c, int32_t which, double* out) {
 int cfft_scft_propagators(cfft_scft* c, void** hist_out) {
   return guarded([&] {
     require(c && hist_out, CFFT_ERR_INVALID, "null argument");
+    hip_check(hipStreamSynchronize(c->stream), "sync");  // the last iteration's work is complete
     *hist_out = c->hist.p;
   });
 }
