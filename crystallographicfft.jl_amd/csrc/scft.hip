@@ -136,15 +136,22 @@ __global__ void __launch_bounds__(kRB) k_update_gram(const double* __restrict__ 
   block_sum<kNU>(acc, part);
 }
 
-// Totals of NV partial sums over nb blocks, each summed in block order by one
-// thread (the same association as a sequential host sum).
+// Totals of NV partial sums over nb <= kMaxRed blocks: workgroup k sums
+// quantity k, each thread a strided share of the blocks, then a fixed tree
+// over the workgroup (the same association on every call).
 template <int NV>
-__global__ void k_total(const double* __restrict__ part, int nb, double* __restrict__ out) {
-  const int k = threadIdx.x;
-  if (k >= NV) return;
+__global__ void __launch_bounds__(kRB) k_total(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  __shared__ double red[kRB];
+  const int k = blockIdx.x;
   double a = 0.0;
-  for (int b = 0; b < nb; ++b) a += part[b * NV + k];
-  out[k] = a;
+  for (int b = threadIdx.x; b < nb; b += kRB) a += part[b * NV + k];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int h = kRB / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[k] = red[0];
 }
 
 struct MixP {
@@ -293,7 +300,7 @@ void iterate(cfft_scft* c, double* out) {
   gs.cur = slot;
   for (int i = 0; i < ns; ++i) gs.slot[i] = c->order[i];
   double* tot = static_cast<double*>(c->tot.p);
-  hipLaunchKernelGGL(k_total<2>, dim3(1), dim3(64), 0, c->stream, static_cast<const double*>(c->dpart.p), dnb,
+  hipLaunchKernelGGL(k_total<2>, dim3(2), dim3(kRB), 0, c->stream, static_cast<const double*>(c->dpart.p), dnb,
                      tot);
   hip_check(hipGetLastError(), "k_total");
   const unsigned nb = red_blocks(M3);
@@ -302,7 +309,7 @@ void iterate(cfft_scft* c, double* out) {
                      c->ds, c->chiN, M3, static_cast<double*>(c->Wr.p), static_cast<double*>(c->Dr.p), gs,
                      static_cast<double*>(c->part.p));
   hip_check(hipGetLastError(), "k_update_gram");
-  hipLaunchKernelGGL(k_total<kNU>, dim3(1), dim3(64), 0, c->stream, static_cast<const double*>(c->part.p),
+  hipLaunchKernelGGL(k_total<kNU>, dim3(kNU), dim3(kRB), 0, c->stream, static_cast<const double*>(c->part.p),
                      (int)nb, tot + 2);
   hip_check(hipGetLastError(), "k_total");
   hip_check(hipMemcpyAsync(c->tot_h, tot, sizeof(double) * (2 + kNU), hipMemcpyDeviceToHost, c->stream),
