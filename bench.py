@@ -353,9 +353,13 @@ def batched_instances(cf, n, seed, workers):
     Ns 100, each started from its group's lowest non-extinct star: solved
     concurrently in mixed precision (FP32 propagators, FP64 reductions) to
     residual 1e-5, then finished in FP64 to 1e-8; plain FP64 to 1e-8 from the
-    same starts for the per-instance errors and the like-for-like time."""
+    same starts for the per-instance errors and the like-for-like time.
+    Instances of one grid and space group share a context (one sweep for all
+    of them); FP64 with a context per instance beside it."""
     from cfft_hip import batch
     insts = batch.draw_instances(n, seed)
+    ru, pu, su = batch.solve_batch(insts, precision=cf.FP64, tol=1e-8, max_iter=400, workers=workers,
+                                   group=False)
     r32, p32, s32 = batch.solve_batch(insts, precision=cf.MIXED, tol=1e-5, max_iter=400, workers=workers)
     rmf, pmf, smf = batch.solve_batch(insts, precision=cf.MIXED, tol=1e-5, max_iter=400, workers=workers,
                                       finish_tol=1e-8)
@@ -379,6 +383,9 @@ def batched_instances(cf, n, seed, workers):
         "fp64": {"tol": 1e-8, "instances_per_s": n / s64, "solve_s": s64, "plan_s": p64,
                  "converged": int(sum(a["res"] < 1e-8 for a in r64)),
                  "iters_mean": float(np.mean([a["iters"] for a in r64]))},
+        "groups": len(batch.groups_of(insts)),
+        "fp64_context_per_instance": {"instances_per_s": n / su, "solve_s": su, "plan_s": pu,
+                                      "same_results": ru == r64, "grouped_speedup": su / s64},
     }
 
 

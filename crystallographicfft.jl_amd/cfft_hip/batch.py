@@ -1,10 +1,13 @@
 """Many independent SCFT instances on one GPU.
 
-Each instance owns a plan (its grid and space group), a ``cfft_scft`` context
-and a stream; its contour sweeps are replayed from captured hipGraphs.  A pool
-of host threads drives the instances (every library call releases the GIL), so
-the instances' kernels run concurrently on their streams.  No counterpart in
-the reference.
+Instances that share grid, space group, cell and Ns form a group: one plan
+with 2 n fields and one ``cfft_scft`` context (``SCFTBatch``), so one sweep of
+two kernels per contour step advances all of them, and per instance the
+iterations are those of a one-instance solver bit for bit.  Each group has its
+own stream, its contour sweeps are replayed from captured hipGraphs, and a
+pool of host threads drives the groups (every library call releases the GIL),
+so the groups' kernels run concurrently.  ``group=False`` gives every instance
+its own plan, context and stream instead.  No counterpart in the reference.
 """
 from __future__ import annotations
 
@@ -16,7 +19,7 @@ import numpy as np
 
 from . import _lib as L
 from .plans import auto_L, find_optimal_shift
-from .scft import SCFTSolver, symmetrize_subgrid
+from .scft import SCFTBatch, SCFTSolver, symmetrize_subgrid
 from .symmetry_ops import get_ops
 
 
@@ -90,33 +93,95 @@ def _prepare(inst: Instance, precision, m_hist, finish):
     return solver, fin, start_fields(inst, ops_s, shift)
 
 
+def _prepare_group(insts, precision, m_hist, finish):
+    i0 = insts[0]
+    ops = get_ops(i0.sg, 3, i0.N)
+    shift, ops_s = find_optimal_shift(ops, i0.N)
+    mk = lambda prec: SCFTBatch(i0.N, ops_s, i0.cell * np.eye(3), [i.f for i in insts], [i.chiN for i in insts],
+                                i0.ns, prec, path=None, m_hist=m_hist)
+    solver = mk(precision)
+    fin = mk(L.FP64) if finish else None
+    return solver, fin, [start_fields(i, ops_s, shift) for i in insts]
+
+
+MAX_GROUP = 32  # instances per context: the plan takes per-field factors for up to 64 fields
+
+
+def groups_of(instances, max_group=MAX_GROUP):
+    """Indices of the instances grouped by (grid, group, cell, Ns), each group
+    at most max_group long, largest work (instances x grid points) first."""
+    by = {}
+    for k, inst in enumerate(instances):
+        by.setdefault((tuple(inst.N), inst.sg, inst.cell, inst.ns), []).append(k)
+    out = []
+    for key, idx in by.items():
+        for a in range(0, len(idx), max_group):
+            out.append(idx[a:a + max_group])
+    out.sort(key=lambda g: -len(g) * int(np.prod(instances[g[0]].N)))
+    return out
+
+
+def _result(it, out):
+    return dict(iters=it, F=out["F"], res=out["res"], Q=out["Q"])
+
+
+def _finish(r, it2, o2, out):
+    r.update(iters_finish=it2, F=o2["F"], res=o2["res"], Q=o2["Q"], F_first=out["F"], res_first=out["res"])
+
+
 def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8, m_hist=10,
-                finish_tol=None):
-    """Plan every instance, then solve all of them concurrently.  With
-    finish_tol, each instance continues in FP64 from its `precision` solution
-    until the residual is below finish_tol (its FP64 plan is made in the
-    planning phase).  Returns (results, plan_seconds, solve_seconds);
-    results[i] = dict(iters, F, res, Q[, iters_finish])."""
+                finish_tol=None, group=True):
+    """Plan every instance (or group of instances), then solve all of them
+    concurrently.  With finish_tol, each instance continues in FP64 from its
+    `precision` solution until the residual is below finish_tol (its FP64
+    plan is made in the planning phase).  Returns (results, plan_seconds,
+    solve_seconds); results[i] = dict(iters, F, res, Q[, iters_finish]) in the
+    order of `instances`."""
+    finish = finish_tol is not None
     t0 = time.perf_counter()
-    with ThreadPoolExecutor(max_workers=workers) as ex:
-        prepared = list(ex.map(lambda inst: _prepare(inst, precision, m_hist, finish_tol is not None),
-                               instances))
+    if group:
+        groups = groups_of(instances)
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            prepared = list(ex.map(lambda g: _prepare_group([instances[k] for k in g], precision, m_hist,
+                                                            finish), groups))
+    else:
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            prepared = list(ex.map(lambda inst: _prepare(inst, precision, m_hist, finish), instances))
     t1 = time.perf_counter()
 
     def run(item):
         solver, fin, (wA, wB) = item
         solver.set_fields(wA, wB)
         it, out = solver.solve(tol=tol, max_iter=max_iter)
-        r = dict(iters=it, F=out["F"], res=out["res"], Q=out["Q"])
+        r = _result(it, out)
         if fin is not None:
             fin.set_fields(*solver.fields())
             it2, o2 = fin.solve(tol=finish_tol, max_iter=max_iter)
-            r.update(iters_finish=it2, F=o2["F"], res=o2["res"], Q=o2["Q"], F_first=out["F"],
-                     res_first=out["res"])
+            _finish(r, it2, o2, out)
         return r
 
+    def run_group(item):
+        solver, fin, starts = item
+        for i, (wA, wB) in enumerate(starts):
+            solver.set_fields(i, wA, wB)
+        its, outs = solver.solve(tol=tol, max_iter=max_iter)
+        rs = [_result(it, out) for it, out in zip(its, outs)]
+        if fin is not None:
+            for i in range(solver.n):
+                fin.set_fields(i, *solver.fields(i))
+            its2, outs2 = fin.solve(tol=finish_tol, max_iter=max_iter)
+            for r, it2, o2, out in zip(rs, its2, outs2, outs):
+                _finish(r, it2, o2, out)
+        return rs
+
     with ThreadPoolExecutor(max_workers=workers) as ex:
-        results = list(ex.map(run, prepared))
+        if group:
+            results = [None] * len(instances)
+            for g, rs in zip(groups, ex.map(run_group, prepared)):
+                for k, r in zip(g, rs):
+                    results[k] = r
+        else:
+            results = list(ex.map(run, prepared))
     t2 = time.perf_counter()
     del prepared
     return results, t1 - t0, t2 - t1

@@ -131,6 +131,17 @@ class PropagatorSweep:
         return [out[b].reshape(self.M, order="F") for b in range(2)]
 
 
+def default_path(N, ops_shifted, lattice, precision):
+    """FP64: the plane-pair chain where it applies.  Mixed precision keeps the
+    default path: there every step passes through the spectral ASU, which
+    projects FP32 rounding back onto the symmetric fields, while on the
+    plane-pair chain the asymmetric part of the rounding accumulates over a
+    sweep (residual floor ~4e-6 instead of ~7e-7 at Ia-3d 128^3, Ns 200)."""
+    if precision == L.FP64 and qs_eligible(N, ops_shifted, lattice):
+        return L.PATH_QS
+    return L.PATH_AUTO
+
+
 class SCFTSolver:
     """Device SCFT of an AB diblock melt on the stride-L subgrid (C context
     ``cfft_scft``): sweep (q, q+ batched, every contour point kept), Simpson
@@ -149,15 +160,7 @@ class SCFTSolver:
         self.ns = int(ns)
         self.f = float(f)
         self.chiN = float(chiN)
-        if path is None:
-            # FP64: the plane-pair chain where it applies.  Mixed precision keeps
-            # the default path: there every step passes through the spectral
-            # ASU, which projects FP32 rounding back onto the symmetric fields,
-            # while on the plane-pair chain the asymmetric part of the rounding
-            # accumulates over a sweep (residual floor ~4e-6 instead of ~7e-7 at
-            # Ia-3d 128^3, Ns 200)
-            fp64 = precision == L.FP64
-            path = L.PATH_QS if fp64 and qs_eligible(N, ops_shifted, lattice) else L.PATH_AUTO
+        path = default_path(N, ops_shifted, lattice, precision) if path is None else path
         self.plan = DevicePlan(N, ops_shifted, 1.0 / self.ns, lattice, path, precision, batch=2)
         self.M = self.plan.M
         self.M3 = int(np.prod(self.M))
@@ -224,3 +227,81 @@ class SCFTSolver:
 
     def deviations(self):
         return self._get(L.SCFT_DEV)
+
+
+class SCFTBatch:
+    """Several melts that share grid, space group, cell and Ns in one context
+    (``cfft_scft_create_multi``): one plan with 2 n fields, one sweep per
+    iteration advances q, q+ of every instance, and each instance keeps its
+    own f, chiN, fields and Anderson history.  Per instance the iterations are
+    those of an ``SCFTSolver`` bit for bit; ``solve`` stops each instance at
+    its own convergence.  Outputs are lists of ``SCFTSolver``-style dicts."""
+
+    KEYS = SCFTSolver.KEYS
+
+    def __init__(self, N, ops_shifted, lattice, fs, chiNs, ns, precision=L.FP64, path=None, m_hist=10):
+        self.ns = int(ns)
+        self.fs = [float(v) for v in fs]
+        self.chiNs = [float(v) for v in chiNs]
+        if len(self.fs) != len(self.chiNs) or not self.fs:
+            raise L.CFFTError(L.CFFT_ERR_INVALID, "need one f and one chiN per instance")
+        self.n = len(self.fs)
+        path = default_path(N, ops_shifted, lattice, precision) if path is None else path
+        self.plan = DevicePlan(N, ops_shifted, 1.0 / self.ns, lattice, path, precision, batch=2 * self.n)
+        self.M = self.plan.M
+        self.M3 = int(np.prod(self.M))
+        f = np.array(self.fs, dtype=np.float64)
+        x = np.array(self.chiNs, dtype=np.float64)
+        h = C.c_void_p()
+        L.check(L.lib().cfft_scft_create_multi(self.plan.handle, self.n, L.f64p(f), L.f64p(x), self.ns,
+                                               int(m_hist), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                L.lib().cfft_scft_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def set_params(self, i, f, chiN):
+        L.check(L.lib().cfft_scft_set_params_inst(self._h, int(i), float(f), float(chiN)))
+        self.fs[i], self.chiNs[i] = float(f), float(chiN)
+
+    def set_mixing(self, lam=1.0, lam_simple=0.1, simple_above=1e-2):
+        L.check(L.lib().cfft_scft_set_mixing(self._h, float(lam), float(lam_simple), float(simple_above)))
+
+    def set_fields(self, i, wA, wB):
+        a = np.ascontiguousarray(np.asarray(wA, dtype=np.float64).ravel(order="F"))
+        b = np.ascontiguousarray(np.asarray(wB, dtype=np.float64).ravel(order="F"))
+        if a.size != self.M3 or b.size != self.M3:
+            raise L.CFFTError(L.CFFT_ERR_INVALID, "fields must have the subgrid size")
+        L.check(L.lib().cfft_scft_set_fields_inst(self._h, int(i), L.f64p(a), L.f64p(b)))
+
+    def _outs(self, o):
+        return [{k: float(v) for k, v in zip(self.KEYS, o[8 * i:8 * i + 8])} for i in range(self.n)]
+
+    def iterate(self):
+        o = np.zeros(8 * self.n)
+        L.check(L.lib().cfft_scft_iterate_multi(self._h, L.f64p(o)))
+        return self._outs(o)
+
+    def solve(self, tol=1e-8, max_iter=1000):
+        """(iterations per instance, result dict per instance)."""
+        o = np.zeros(8 * self.n)
+        it = np.zeros(self.n, dtype=np.int32)
+        L.check(L.lib().cfft_scft_solve_multi(self._h, float(tol), int(max_iter), L.i32p(it), L.f64p(o)))
+        return [int(v) for v in it], self._outs(o)
+
+    def _get(self, i, which):
+        out = np.zeros(2 * self.M3)
+        L.check(L.lib().cfft_scft_get_inst(self._h, int(i), which, L.f64p(out)))
+        return [out[k * self.M3:(k + 1) * self.M3].reshape(self.M, order="F") for k in range(2)]
+
+    def fields(self, i):
+        return self._get(i, L.SCFT_W)
+
+    def densities(self, i):
+        return self._get(i, L.SCFT_PHI)

@@ -60,25 +60,30 @@ __global__ void k_expw(const double* __restrict__ w, T* __restrict__ E, double f
   if (i < n) E[i] = (T)exp(factor * w[i]);
 }
 
-// Unnormalised densities from the stored propagators (slice j = [q(j), q+(j)]):
+// Unnormalised densities of one instance from the stored propagators (slice j
+// holds nf fields; the instance's q(j) is field qf, its q+(j) field qf + 1):
 //   uA = sum_{j<=nA} wa_j q(j) q+(ns-j),  uB = sum_{j>=nA} wb_{j-nA} q(j) q+(ns-j);
 // partial sums of q(ns) and q+(ns) for Q.
 template <typename T>
-__global__ void __launch_bounds__(kRB) k_density(const T* __restrict__ hist, long M3, int ns, int nA,
-                                                 const double* __restrict__ wa, const double* __restrict__ wb,
-                                                 double* __restrict__ u, double* __restrict__ part) {
+__global__ void __launch_bounds__(kRB) k_density(const T* __restrict__ hist, long M3, int nf, int qf, int ns,
+                                                 int nA, const double* __restrict__ wa,
+                                                 const double* __restrict__ wb, double* __restrict__ u,
+                                                 double* __restrict__ part) {
   double acc[2] = {0.0, 0.0};
+  const T* hq = hist + (long)qf * M3;
+  const T* hp = hist + (long)(qf + 1) * M3;
+  const long sl = (long)nf * M3;
   for (long x = blockIdx.x * (long)kRB + threadIdx.x; x < M3; x += (long)gridDim.x * kRB) {
     double a = 0.0, b = 0.0;
     for (int j = 0; j <= ns; ++j) {
-      const double pr = (double)hist[(2L * j) * M3 + x] * (double)hist[(2L * (ns - j) + 1) * M3 + x];
+      const double pr = (double)hq[j * sl + x] * (double)hp[(ns - j) * sl + x];
       if (j <= nA) a += wa[j] * pr;
       if (j >= nA) b += wb[j - nA] * pr;
     }
     u[x] = a;
     u[M3 + x] = b;
-    acc[0] += (double)hist[(2L * ns) * M3 + x];
-    acc[1] += (double)hist[(2L * ns + 1) * M3 + x];
+    acc[0] += (double)hq[ns * sl + x];
+    acc[1] += (double)hp[ns * sl + x];
   }
   block_sum<2>(acc, part);
 }
@@ -224,22 +229,33 @@ bool solve_small(std::vector<double> U, std::vector<double> V, int k, std::vecto
 
 }  // namespace
 
-struct cfft_scft {
-  cfft_plan* plan = nullptr;
-  int prec = 0, ns = 0, nA = 0, m = 0;
-  long M3 = 0;
-  size_t esz = 8;
-  double f = 0.0, chiN = 0.0, ds = 0.0;
-  double lam = 1.0, lam_simple = 0.1, simple_above = 1e-2;
-  hipStream_t stream = nullptr;
-  DevBuf hist, E, w, phi, d, Wr, Dr, wa, wb, part, dpart, tot;
-  double* tot_h = nullptr;  // pinned: totals of the density and update/Gram partials
+// One melt of a context: its parameters, fields and mixing history.
+struct ScftInst {
+  double f = 0.0, chiN = 0.0;
+  int nA = 0;
+  DevBuf w, phi, d, Wr, Dr, wa, wb;
   // Anderson history: ring slots (newest last), Gram matrix G[slot][slot] = <d_i, d_j>/n
   std::vector<int> order;
   std::vector<double> G;
   int next_slot = 0;
   int iters = 0;
   double last[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+struct cfft_scft {
+  cfft_plan* plan = nullptr;
+  int prec = 0, ns = 0, m = 0;
+  long M3 = 0;
+  size_t esz = 8;
+  double ds = 0.0;
+  double lam = 1.0, lam_simple = 0.1, simple_above = 1e-2;
+  hipStream_t stream = nullptr;
+  // shared by the instances: propagator slices [ns + 1][2 n_inst fields],
+  // interaction factors [n_inst][E_A, E_B], reduction partials (reused in
+  // stream order), per-instance totals and their pinned host copy
+  DevBuf hist, E, part, dpart, tot;
+  double* tot_h = nullptr;
+  std::vector<std::unique_ptr<ScftInst>> in;
   ~cfft_scft() {
     if (stream) {
       (void)hipStreamSynchronize(stream);
@@ -251,86 +267,110 @@ struct cfft_scft {
 
 namespace {
 
+constexpr int kTot = 2 + kNU;  // totals per instance: q(ns), q+(ns), then the k_update_gram sums
+
 unsigned red_blocks(long n) { return (unsigned)std::min<long>(kMaxRed, std::max<long>(1, nblk(n, kRB))); }
 
-// Sweep and unnormalised densities; the partial sums of q(ns), q+(ns) stay on
-// the device (dpart, *dnb blocks).
-template <typename T>
-void sweep_and_densities(cfft_scft* c, int* dnb) {
-  const long M3 = c->M3;
-  const long n2 = 2 * M3;
-  T* E = static_cast<T*>(c->E.p);
-  hipLaunchKernelGGL(k_expw<T>, dim3(nblk(n2, 256)), dim3(256), 0, c->stream,
-                     static_cast<const double*>(c->w.p), E, -0.5 * c->ds, n2);
-  hip_check(hipGetLastError(), "k_expw");
-  hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n2, 256)), dim3(256), 0, c->stream, static_cast<T*>(c->hist.p),
-                     T(1), n2);
-  hip_check(hipGetLastError(), "k_fill");
-  // q (entry 0) uses E_A on its first nA steps, q+ (entry 1) E_B on its first ns - nA
-  const void* E1[2] = {E, E + M3};
-  const void* E2[2] = {E + M3, E};
-  const int32_t sw[2] = {c->nA, c->ns - c->nA};
-  const int rc = cfft_sweep_hist_dev(c->plan, c->hist.p, E1, E2, sw, 0, c->ns, c->stream);
-  if (rc != CFFT_OK) throw Error{rc, g_last_error};
-  const unsigned nb = red_blocks(M3);
-  hipLaunchKernelGGL(k_density<T>, dim3(nb), dim3(kRB), 0, c->stream, static_cast<const T*>(c->hist.p), M3,
-                     c->ns, c->nA, static_cast<const double*>(c->wa.p), static_cast<const double*>(c->wb.p),
-                     static_cast<double*>(c->phi.p), static_cast<double*>(c->dpart.p));
-  hip_check(hipGetLastError(), "k_density");
-  *dnb = (int)nb;
+int block_steps(double f, int ns) { return (int)std::lround(f * ns); }
+
+void set_weights(cfft_scft* c, ScftInst& s, bool sync) {
+  const std::vector<double> wa = simpson(s.nA), wb = simpson(c->ns - s.nA);
+  hip_check(hipMemcpyAsync(s.wa.p, wa.data(), 8 * wa.size(), hipMemcpyHostToDevice, c->stream), "upload");
+  hip_check(hipMemcpyAsync(s.wb.p, wb.data(), 8 * wb.size(), hipMemcpyHostToDevice, c->stream), "upload");
+  if (sync) hip_check(hipStreamSynchronize(c->stream), "sync");  // the host vectors go out of scope
 }
 
-// One SCFT iteration; out = {Q, Q+, F, residual, <phiA + phiB>, <phiA>, mixing, k}.
-// One host synchronisation per iteration (the totals of the partial sums);
-// the mixing kernel is left in flight on the context's stream, ahead of the
-// next iteration's sweep.
-void iterate(cfft_scft* c, double* out) {
+// Sweep of every instance (q, q+ of instance i are fields 2i, 2i + 1):
+// E = exp(-ds w / 2) of the active instances, slice 0 = 1, then ns steps.
+template <typename T>
+void sweep_all(cfft_scft* c, const std::vector<char>& active) {
   const long M3 = c->M3, n2 = 2 * M3;
-  int dnb = 0;
-  if (c->prec == CFFT_FP64) sweep_and_densities<double>(c, &dnb);
-  else sweep_and_densities<float>(c, &dnb);
-  // ---- field update + Anderson history slot + Gram row, one pass ----
-  const int slot = c->next_slot;
-  c->next_slot = (c->next_slot + 1) % (c->m + 1);
-  c->order.erase(std::remove(c->order.begin(), c->order.end(), slot), c->order.end());
-  c->order.push_back(slot);
-  const int ns = (int)c->order.size();
-  GramSlots gs{};
-  gs.n = ns;
-  gs.cur = slot;
-  for (int i = 0; i < ns; ++i) gs.slot[i] = c->order[i];
-  double* tot = static_cast<double*>(c->tot.p);
-  hipLaunchKernelGGL(k_total<2>, dim3(2), dim3(kRB), 0, c->stream, static_cast<const double*>(c->dpart.p), dnb,
-                     tot);
-  hip_check(hipGetLastError(), "k_total");
+  const int ni = (int)c->in.size();
+  T* E = static_cast<T*>(c->E.p);
+  for (int i = 0; i < ni; ++i) {
+    if (!active[i]) continue;  // an idle instance keeps its last factors
+    hipLaunchKernelGGL(k_expw<T>, dim3(nblk(n2, 256)), dim3(256), 0, c->stream,
+                       static_cast<const double*>(c->in[i]->w.p), E + i * n2, -0.5 * c->ds, n2);
+    hip_check(hipGetLastError(), "k_expw");
+  }
+  hipLaunchKernelGGL(k_fill<T>, dim3(nblk(ni * n2, 256)), dim3(256), 0, c->stream, static_cast<T*>(c->hist.p),
+                     T(1), ni * n2);
+  hip_check(hipGetLastError(), "k_fill");
+  // q of instance i uses E_A on its first nA steps, q+ E_B on its first ns - nA
+  std::vector<const void*> E1(2 * ni), E2(2 * ni);
+  std::vector<int32_t> sw(2 * ni);
+  for (int i = 0; i < ni; ++i) {
+    const T* Ea = E + i * n2;
+    const T* Eb = Ea + M3;
+    E1[2 * i] = Ea;
+    E2[2 * i] = Eb;
+    sw[2 * i] = c->in[i]->nA;
+    E1[2 * i + 1] = Eb;
+    E2[2 * i + 1] = Ea;
+    sw[2 * i + 1] = c->ns - c->in[i]->nA;
+  }
+  const int rc = cfft_sweep_hist_dev(c->plan, c->hist.p, E1.data(), E2.data(), sw.data(), 0, c->ns, c->stream);
+  if (rc != CFFT_OK) throw Error{rc, g_last_error};
+}
+
+// Densities, field update and Gram row of instance i (device only; totals to
+// tot + i kTot).
+template <typename T>
+void update_inst(cfft_scft* c, int i) {
+  ScftInst& s = *c->in[i];
+  const long M3 = c->M3;
   const unsigned nb = red_blocks(M3);
-  hipLaunchKernelGGL(k_update_gram, dim3(nb), dim3(kRB), 0, c->stream, static_cast<const double*>(c->w.p),
-                     static_cast<double*>(c->phi.p), static_cast<double*>(c->d.p), static_cast<const double*>(tot),
-                     c->ds, c->chiN, M3, static_cast<double*>(c->Wr.p), static_cast<double*>(c->Dr.p), gs,
+  double* tot = static_cast<double*>(c->tot.p) + (long)i * kTot;
+  hipLaunchKernelGGL(k_density<T>, dim3(nb), dim3(kRB), 0, c->stream, static_cast<const T*>(c->hist.p), M3,
+                     2 * (int)c->in.size(), 2 * i, c->ns, s.nA, static_cast<const double*>(s.wa.p),
+                     static_cast<const double*>(s.wb.p), static_cast<double*>(s.phi.p),
+                     static_cast<double*>(c->dpart.p));
+  hip_check(hipGetLastError(), "k_density");
+  hipLaunchKernelGGL(k_total<2>, dim3(2), dim3(kRB), 0, c->stream, static_cast<const double*>(c->dpart.p),
+                     (int)nb, tot);
+  hip_check(hipGetLastError(), "k_total");
+  // the newest history slot and the Gram row over the ring
+  const int slot = s.next_slot;
+  s.next_slot = (s.next_slot + 1) % (c->m + 1);
+  s.order.erase(std::remove(s.order.begin(), s.order.end(), slot), s.order.end());
+  s.order.push_back(slot);
+  GramSlots gs{};
+  gs.n = (int)s.order.size();
+  gs.cur = slot;
+  for (int k = 0; k < gs.n; ++k) gs.slot[k] = s.order[k];
+  hipLaunchKernelGGL(k_update_gram, dim3(nb), dim3(kRB), 0, c->stream, static_cast<const double*>(s.w.p),
+                     static_cast<double*>(s.phi.p), static_cast<double*>(s.d.p), static_cast<const double*>(tot),
+                     c->ds, s.chiN, M3, static_cast<double*>(s.Wr.p), static_cast<double*>(s.Dr.p), gs,
                      static_cast<double*>(c->part.p));
   hip_check(hipGetLastError(), "k_update_gram");
   hipLaunchKernelGGL(k_total<kNU>, dim3(kNU), dim3(kRB), 0, c->stream, static_cast<const double*>(c->part.p),
                      (int)nb, tot + 2);
   hip_check(hipGetLastError(), "k_total");
-  hip_check(hipMemcpyAsync(c->tot_h, tot, sizeof(double) * (2 + kNU), hipMemcpyDeviceToHost, c->stream),
-            "D2H totals");
-  hip_check(hipStreamSynchronize(c->stream), "sync");
-  const double Q[2] = {c->tot_h[0] / M3, c->tot_h[1] / M3};
+}
+
+// Host part of instance i's iteration from its totals: Q, F, residual, Gram
+// row; then the mixing kernel (left in flight on the context's stream).
+// out = {Q, Q+, F, residual, <phiA + phiB>, <phiA>, mixing, k}.
+void finish_inst(cfft_scft* c, int i, double* out) {
+  ScftInst& s = *c->in[i];
+  const long M3 = c->M3, n2 = 2 * M3;
+  const double* t = c->tot_h + (long)i * kTot;
+  const double Q[2] = {t[0] / M3, t[1] / M3};
+  const int slot = s.order.back();
   if (!(Q[0] > 0.0 && std::isfinite(Q[0]))) {
-    c->order.pop_back();  // the newest slot holds no valid entry
+    s.order.pop_back();  // the newest slot holds no valid entry
     throw Error{CFFT_ERR_INVALID, "partition function is not positive"};
   }
-  double s[7];
-  for (int k = 0; k < 7; ++k) s[k] = c->tot_h[2 + k] / M3;
-  const double F = -std::log(Q[0]) + c->chiN * s[0] - s[1] - s[2];
-  const double res = std::sqrt(s[4] / std::max(s[5], 1e-300)) + std::fabs(s[3] - 1.0);
-  {
-    const int S = c->m + 1;
-    for (int i = 0; i < ns; ++i) {
-      const double g = c->tot_h[2 + 7 + i] / n2;
-      c->G[slot * S + c->order[i]] = g;
-      c->G[c->order[i] * S + slot] = g;
-    }
+  double v[7];
+  for (int k = 0; k < 7; ++k) v[k] = t[2 + k] / M3;
+  const double F = -std::log(Q[0]) + s.chiN * v[0] - v[1] - v[2];
+  const double res = std::sqrt(v[4] / std::max(v[5], 1e-300)) + std::fabs(v[3] - 1.0);
+  const int S = c->m + 1;
+  const int ns = (int)s.order.size();
+  for (int k = 0; k < ns; ++k) {
+    const double g = t[2 + 7 + k] / n2;
+    s.G[slot * S + s.order[k]] = g;
+    s.G[s.order[k] * S + slot] = g;
   }
   // ---- mixing ----
   MixP mp{};
@@ -340,35 +380,190 @@ void iterate(cfft_scft* c, double* out) {
   double mode = 0.0;
   const int k = ns - 1;
   if (res <= c->simple_above && k > 0) {
-    const int S = c->m + 1;
-    const double* G = c->G.data();
+    const double* G = s.G.data();
     auto g = [&](int a, int b) { return G[a * S + b]; };
     std::vector<double> U(k * k), V(k), cc;
-    for (int i = 0; i < k; ++i) {
-      const int si = c->order[ns - 2 - i];
-      V[i] = g(slot, slot) - g(slot, si);
-      for (int j = 0; j < k; ++j) {
-        const int sj = c->order[ns - 2 - j];
-        U[i * k + j] = g(slot, slot) - g(slot, si) - g(slot, sj) + g(si, sj);
+    for (int a = 0; a < k; ++a) {
+      const int sa = s.order[ns - 2 - a];
+      V[a] = g(slot, slot) - g(slot, sa);
+      for (int b = 0; b < k; ++b) {
+        const int sb = s.order[ns - 2 - b];
+        U[a * k + b] = g(slot, slot) - g(slot, sa) - g(slot, sb) + g(sa, sb);
       }
     }
     if (solve_small(U, V, k, cc)) {
       mp.k = k;
       mp.lam = c->lam;
-      for (int i = 0; i < k; ++i) {
-        mp.slot[i] = c->order[ns - 2 - i];
-        mp.c[i] = cc[i];
+      for (int a = 0; a < k; ++a) {
+        mp.slot[a] = s.order[ns - 2 - a];
+        mp.c[a] = cc[a];
       }
       mode = 1.0;
     }
   }
-  hipLaunchKernelGGL(k_mix, dim3(nblk(n2, 256)), dim3(256), 0, c->stream, static_cast<double*>(c->w.p),
-                     static_cast<const double*>(c->Wr.p), static_cast<const double*>(c->Dr.p), mp, n2);
+  hipLaunchKernelGGL(k_mix, dim3(nblk(n2, 256)), dim3(256), 0, c->stream, static_cast<double*>(s.w.p),
+                     static_cast<const double*>(s.Wr.p), static_cast<const double*>(s.Dr.p), mp, n2);
   hip_check(hipGetLastError(), "k_mix");
-  ++c->iters;
-  const double o[8] = {Q[0], Q[1], F, res, s[3], s[6], mode, (double)mp.k};
-  std::memcpy(c->last, o, sizeof(o));
+  ++s.iters;
+  const double o[8] = {Q[0], Q[1], F, res, v[3], v[6], mode, (double)mp.k};
+  std::memcpy(s.last, o, sizeof(o));
   if (out) std::memcpy(out, o, sizeof(o));
+}
+
+// One SCFT iteration of the active instances: one sweep of all of them, then
+// per instance densities, update and Gram row on the device; one host
+// synchronisation for all their totals; host mixing coefficients and the
+// mixing kernels.  out[8 i ..] receives instance i's results (active only).
+void iterate(cfft_scft* c, const std::vector<char>& active, double* out) {
+  const int ni = (int)c->in.size();
+  if (c->prec == CFFT_FP64) sweep_all<double>(c, active);
+  else sweep_all<float>(c, active);
+  for (int i = 0; i < ni; ++i) {
+    if (!active[i]) continue;
+    if (c->prec == CFFT_FP64) update_inst<double>(c, i);
+    else update_inst<float>(c, i);
+  }
+  hip_check(hipMemcpyAsync(c->tot_h, c->tot.p, sizeof(double) * kTot * ni, hipMemcpyDeviceToHost, c->stream),
+            "D2H totals");
+  hip_check(hipStreamSynchronize(c->stream), "sync");
+  // every active instance is finished; the first failure is reported after
+  bool failed = false;
+  Error first{CFFT_OK, ""};
+  for (int i = 0; i < ni; ++i) {
+    if (!active[i]) continue;
+    try {
+      finish_inst(c, i, out ? out + 8 * i : nullptr);
+    } catch (const Error& e) {
+      if (!failed) first = e;
+      failed = true;
+    }
+  }
+  if (failed) throw first;
+}
+
+ScftInst& instance(cfft_scft* c, int32_t i) {
+  require(c != nullptr, CFFT_ERR_INVALID, "null context");
+  require(i >= 0 && i < (int)c->in.size(), CFFT_ERR_INVALID, "instance index out of range");
+  return *c->in[i];
+}
+
+void set_fields_inst(cfft_scft* c, ScftInst& s, const double* wA, const double* wB) {
+  require(wA && wB, CFFT_ERR_INVALID, "null argument");
+  double* w = static_cast<double*>(s.w.p);
+  hip_check(hipMemcpyAsync(w, wA, 8 * c->M3, hipMemcpyHostToDevice, c->stream), "H2D");
+  hip_check(hipMemcpyAsync(w + c->M3, wB, 8 * c->M3, hipMemcpyHostToDevice, c->stream), "H2D");
+  hip_check(hipStreamSynchronize(c->stream), "sync");
+  s.order.clear();
+  s.next_slot = 0;
+  std::fill(s.G.begin(), s.G.end(), 0.0);
+}
+
+void get_inst(cfft_scft* c, ScftInst& s, int32_t which, double* out) {
+  require(out != nullptr, CFFT_ERR_INVALID, "null argument");
+  const double* src = nullptr;
+  switch (which) {
+    case CFFT_SCFT_W: src = static_cast<const double*>(s.w.p); break;
+    case CFFT_SCFT_PHI: src = static_cast<const double*>(s.phi.p); break;
+    case CFFT_SCFT_DEV: src = static_cast<const double*>(s.d.p); break;
+    default: throw Error{CFFT_ERR_INVALID, "unknown quantity"};
+  }
+  hip_check(hipMemcpyAsync(out, src, 16 * c->M3, hipMemcpyDeviceToHost, c->stream), "D2H");
+  hip_check(hipStreamSynchronize(c->stream), "sync");
+}
+
+void set_params_inst(cfft_scft* c, ScftInst& s, double f, double chiN) {
+  require(f > 0.0 && f < 1.0, CFFT_ERR_INVALID, "need 0 < f < 1");
+  const int nA = block_steps(f, c->ns);
+  require(nA >= 1 && nA < c->ns, CFFT_ERR_INVALID, "f * ns must round into [1, ns - 1]");
+  s.f = f;
+  s.chiN = chiN;
+  s.nA = nA;
+  set_weights(c, s, true);  // the weight buffers hold ns + 1 entries each (sized at creation)
+}
+
+// Iterate until every instance has residual < tol or max_iter iterations; an
+// instance stops (and its fields stay as its last mixing left them) as soon
+// as it meets either.
+void solve(cfft_scft* c, double tol, int32_t max_iter, int32_t* iters, double* out) {
+  const int ni = (int)c->in.size();
+  std::vector<char> active(ni, max_iter > 0 ? 1 : 0);
+  std::vector<int> it(ni, 0);
+  std::vector<double> o(8 * ni, 0.0);
+  for (;;) {
+    bool any = false;
+    for (int i = 0; i < ni; ++i) any = any || active[i];
+    if (!any) break;
+    iterate(c, active, o.data());
+    for (int i = 0; i < ni; ++i) {
+      if (!active[i]) continue;
+      ++it[i];
+      if (o[8 * i + 3] < tol || it[i] >= max_iter) active[i] = 0;
+    }
+  }
+  for (int i = 0; i < ni; ++i) {
+    if (iters) iters[i] = it[i];
+    if (out) std::memcpy(out + 8 * i, o.data() + 8 * i, 8 * sizeof(double));
+  }
+}
+
+cfft_scft* create(cfft_plan* plan, int32_t n_inst, const double* f, const double* chiN, int32_t ns,
+                  int32_t m_hist) {
+  require(plan && f && chiN, CFFT_ERR_INVALID, "null argument");
+  require(n_inst >= 1, CFFT_ERR_INVALID, "need at least one instance");
+  require(ns >= 2, CFFT_ERR_INVALID, "need ns >= 2");
+  require(m_hist >= 0 && m_hist < kMaxHist, CFFT_ERR_INVALID, "Anderson history must be < 32");
+  std::lock_guard<std::recursive_mutex> guard(setup_mutex());
+  int32_t info[18];
+  int rc = cfft_plan_info(plan, info);
+  if (rc != CFFT_OK) throw Error{rc, g_last_error};
+  require(info[12] == 2 * n_inst, CFFT_ERR_INVALID,
+          "the SCFT plan must have batch = 2 x instances (q and q+ of each)");
+  std::unique_ptr<cfft_scft> c(new cfft_scft());
+  c->plan = plan;
+  c->prec = info[11];
+  c->M3 = (long)info[0] * info[1] * info[2];
+  c->esz = c->prec == CFFT_FP64 ? 8 : 4;
+  c->ns = ns;
+  c->ds = 1.0 / ns;
+  c->m = m_hist;
+  hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  const long n2 = 2 * c->M3;
+  c->hist.alloc(c->esz * n2 * n_inst * (ns + 1));
+  c->E.alloc(c->esz * n2 * n_inst);
+  c->part.alloc(8 * kMaxRed * kNU);
+  c->dpart.alloc(8 * kMaxRed * 2);
+  c->tot.alloc(8 * (size_t)kTot * n_inst);
+  hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->tot_h), sizeof(double) * kTot * n_inst,
+                          hipHostMallocDefault),
+            "hipHostMalloc");
+  for (int i = 0; i < n_inst; ++i) {
+    require(f[i] > 0.0 && f[i] < 1.0, CFFT_ERR_INVALID, "need 0 < f < 1");
+    const int nA = block_steps(f[i], ns);
+    require(nA >= 1 && nA < ns, CFFT_ERR_INVALID, "f * ns must round into [1, ns - 1]");
+    std::unique_ptr<ScftInst> s(new ScftInst());
+    s->f = f[i];
+    s->chiN = chiN[i];
+    s->nA = nA;
+    s->w.alloc(8 * n2);
+    s->phi.alloc(8 * n2);
+    s->d.alloc(8 * n2);
+    s->Wr.alloc(8 * n2 * (m_hist + 1));
+    s->Dr.alloc(8 * n2 * (m_hist + 1));
+    s->wa.alloc(8 * (size_t)(ns + 1));
+    s->wb.alloc(8 * (size_t)(ns + 1));
+    s->G.assign((m_hist + 1) * (m_hist + 1), 0.0);
+    set_weights(c.get(), *s, true);
+    hip_check(hipMemsetAsync(s->w.p, 0, 8 * n2, c->stream), "memset");
+    c->in.push_back(std::move(s));
+  }
+  hip_check(hipStreamSynchronize(c->stream), "sync");
+  return c.release();
+}
+
+cfft_scft* single(cfft_scft* c) {
+  require(c != nullptr, CFFT_ERR_INVALID, "null context");
+  require(c->in.size() == 1, CFFT_ERR_INVALID, "a context of several instances takes the _inst/_multi calls");
+  return c;
 }
 
 }  // namespace
@@ -377,67 +572,27 @@ extern "C" {
 
 int cfft_scft_create(cfft_plan* plan, double f, double chiN, int32_t ns, int32_t m_hist, cfft_scft** out) {
   return guarded([&] {
-    require(plan && out, CFFT_ERR_INVALID, "null argument");
-    require(ns >= 2 && f > 0.0 && f < 1.0, CFFT_ERR_INVALID, "need ns >= 2 and 0 < f < 1");
-    require(m_hist >= 0 && m_hist < kMaxHist, CFFT_ERR_INVALID, "Anderson history must be < 32");
+    require(out != nullptr, CFFT_ERR_INVALID, "null argument");
     *out = nullptr;
-    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
-    int32_t info[18];
-    int rc = cfft_plan_info(plan, info);
-    if (rc != CFFT_OK) throw Error{rc, g_last_error};
-    require(info[12] == 2, CFFT_ERR_INVALID, "the SCFT plan must have batch = 2 (q and q+)");
-    std::unique_ptr<cfft_scft> c(new cfft_scft());
-    c->plan = plan;
-    c->prec = info[11];
-    c->M3 = (long)info[0] * info[1] * info[2];
-    c->esz = c->prec == CFFT_FP64 ? 8 : 4;
-    c->f = f;
-    c->chiN = chiN;
-    c->ns = ns;
-    c->nA = (int)std::lround(f * ns);
-    require(c->nA >= 1 && c->nA < ns, CFFT_ERR_INVALID, "f * ns must round into [1, ns - 1]");
-    c->ds = 1.0 / ns;
-    c->m = m_hist;
-    hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
-    const long n2 = 2 * c->M3;
-    c->hist.alloc(c->esz * n2 * (ns + 1));
-    c->E.alloc(c->esz * n2);
-    c->w.alloc(8 * n2);
-    c->phi.alloc(8 * n2);
-    c->d.alloc(8 * n2);
-    c->Wr.alloc(8 * n2 * (m_hist + 1));
-    c->Dr.alloc(8 * n2 * (m_hist + 1));
-    c->part.alloc(8 * kMaxRed * kNU);
-    c->dpart.alloc(8 * kMaxRed * 2);
-    c->tot.alloc(8 * (2 + kNU));
-    hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->tot_h), sizeof(double) * (2 + kNU), hipHostMallocDefault),
-              "hipHostMalloc");
-    c->G.assign((m_hist + 1) * (m_hist + 1), 0.0);
-    const std::vector<double> wa = simpson(c->nA), wb = simpson(ns - c->nA);
-    c->wa.alloc(8 * (size_t)(ns + 1));
-    c->wb.alloc(8 * (size_t)(ns + 1));
-    hip_check(hipMemcpy(c->wa.p, wa.data(), 8 * wa.size(), hipMemcpyHostToDevice), "upload");
-    hip_check(hipMemcpy(c->wb.p, wb.data(), 8 * wb.size(), hipMemcpyHostToDevice), "upload");
-    hip_check(hipMemset(c->w.p, 0, 8 * n2), "memset");
-    *out = c.release();
+    *out = create(plan, 1, &f, &chiN, ns, m_hist);
+  });
+}
+
+int cfft_scft_create_multi(cfft_plan* plan, int32_t n_inst, const double* f, const double* chiN, int32_t ns,
+                           int32_t m_hist, cfft_scft** out) {
+  return guarded([&] {
+    require(out != nullptr, CFFT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    *out = create(plan, n_inst, f, chiN, ns, m_hist);
   });
 }
 
 int cfft_scft_set_params(cfft_scft* c, double f, double chiN) {
-  return guarded([&] {
-    require(c != nullptr, CFFT_ERR_INVALID, "null context");
-    require(f > 0.0 && f < 1.0, CFFT_ERR_INVALID, "need 0 < f < 1");
-    const int nA = (int)std::lround(f * c->ns);
-    require(nA >= 1 && nA < c->ns, CFFT_ERR_INVALID, "f * ns must round into [1, ns - 1]");
-    c->f = f;
-    c->chiN = chiN;
-    c->nA = nA;
-    // the weight buffers hold ns + 1 entries each (sized at creation)
-    const std::vector<double> wa = simpson(c->nA), wb = simpson(c->ns - c->nA);
-    hip_check(hipMemcpyAsync(c->wa.p, wa.data(), 8 * wa.size(), hipMemcpyHostToDevice, c->stream), "upload");
-    hip_check(hipMemcpyAsync(c->wb.p, wb.data(), 8 * wb.size(), hipMemcpyHostToDevice, c->stream), "upload");
-    hip_check(hipStreamSynchronize(c->stream), "sync");
-  });
+  return guarded([&] { set_params_inst(single(c), *c->in[0], f, chiN); });
+}
+
+int cfft_scft_set_params_inst(cfft_scft* c, int32_t inst, double f, double chiN) {
+  return guarded([&] { set_params_inst(c, instance(c, inst), f, chiN); });
 }
 
 int cfft_scft_set_mixing(cfft_scft* c, double lam, double lam_simple, double simple_above) {
@@ -450,53 +605,44 @@ int cfft_scft_set_mixing(cfft_scft* c, double lam, double lam_simple, double sim
 }
 
 int cfft_scft_set_fields(cfft_scft* c, const double* wA, const double* wB) {
-  return guarded([&] {
-    require(c && wA && wB, CFFT_ERR_INVALID, "null argument");
-    double* w = static_cast<double*>(c->w.p);
-    hip_check(hipMemcpyAsync(w, wA, 8 * c->M3, hipMemcpyHostToDevice, c->stream), "H2D");
-    hip_check(hipMemcpyAsync(w + c->M3, wB, 8 * c->M3, hipMemcpyHostToDevice, c->stream), "H2D");
-    hip_check(hipStreamSynchronize(c->stream), "sync");
-    c->order.clear();
-    c->next_slot = 0;
-    std::fill(c->G.begin(), c->G.end(), 0.0);
-  });
+  return guarded([&] { set_fields_inst(single(c), *c->in[0], wA, wB); });
+}
+
+int cfft_scft_set_fields_inst(cfft_scft* c, int32_t inst, const double* wA, const double* wB) {
+  return guarded([&] { set_fields_inst(c, instance(c, inst), wA, wB); });
 }
 
 int cfft_scft_iterate(cfft_scft* c, double* out) {
+  return guarded([&] { iterate(single(c), std::vector<char>(1, 1), out); });
+}
+
+int cfft_scft_iterate_multi(cfft_scft* c, double* out) {
   return guarded([&] {
     require(c != nullptr, CFFT_ERR_INVALID, "null context");
-    iterate(c, out);
+    iterate(c, std::vector<char>(c->in.size(), 1), out);
   });
 }
 
 int cfft_scft_solve(cfft_scft* c, double tol, int32_t max_iter, int32_t* iters_out, double* out) {
   return guarded([&] {
+    require(max_iter >= 0, CFFT_ERR_INVALID, "bad argument");
+    solve(single(c), tol, max_iter, iters_out, out);
+  });
+}
+
+int cfft_scft_solve_multi(cfft_scft* c, double tol, int32_t max_iter, int32_t* iters_out, double* out) {
+  return guarded([&] {
     require(c != nullptr && max_iter >= 0, CFFT_ERR_INVALID, "bad argument");
-    int it = 0;
-    double o[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    while (it < max_iter) {
-      iterate(c, o);
-      ++it;
-      if (o[3] < tol) break;
-    }
-    if (iters_out) *iters_out = it;
-    if (out) std::memcpy(out, o, sizeof(o));
+    solve(c, tol, max_iter, iters_out, out);
   });
 }
 
 int cfft_scft_get(cfft_scft* c, int32_t which, double* out) {
-  return guarded([&] {
-    require(c && out, CFFT_ERR_INVALID, "null argument");
-    const double* src = nullptr;
-    switch (which) {
-      case CFFT_SCFT_W: src = static_cast<const double*>(c->w.p); break;
-      case CFFT_SCFT_PHI: src = static_cast<const double*>(c->phi.p); break;
-      case CFFT_SCFT_DEV: src = static_cast<const double*>(c->d.p); break;
-      default: throw Error{CFFT_ERR_INVALID, "unknown quantity"};
-    }
-    hip_check(hipMemcpyAsync(out, src, 16 * c->M3, hipMemcpyDeviceToHost, c->stream), "D2H");
-    hip_check(hipStreamSynchronize(c->stream), "sync");
-  });
+  return guarded([&] { get_inst(single(c), *c->in[0], which, out); });
+}
+
+int cfft_scft_get_inst(cfft_scft* c, int32_t inst, int32_t which, double* out) {
+  return guarded([&] { get_inst(c, instance(c, inst), which, out); });
 }
 
 int cfft_scft_propagators(cfft_scft* c, void** hist_out) {

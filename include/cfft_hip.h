@@ -332,9 +332,29 @@ int cfft_scft_iterate(cfft_scft* ctx, double* out);
 int cfft_scft_solve(cfft_scft* ctx, double tol, int32_t max_iter, int32_t* iters_out, double* out);
 /* Copy a quantity (CFFT_SCFT_*) to the host: 2 x M0*M1*M2 doubles. */
 int cfft_scft_get(cfft_scft* ctx, int32_t which, double* out);
-/* Device pointer to the propagator slices [ns + 1][q, q+][M0*M1*M2] (plan precision). */
+/* Device pointer to the propagator slices [ns + 1][q, q+][M0*M1*M2] (plan
+ * precision); with n instances [ns + 1][q_0, q+_0, ..., q_{n-1}, q+_{n-1}][...]. */
 int cfft_scft_propagators(cfft_scft* ctx, void** hist_out);
 int cfft_scft_destroy(cfft_scft* ctx);
+
+/* Several melts in one context (no counterpart in the reference).  n_inst
+ * instances share the grid, space group, cell and ns of a plan created with
+ * batch = 2 n_inst, and one sweep advances q, q+ of all of them (fields 2i,
+ * 2i + 1); each has its own f[i], chiN[i], fields, densities and Anderson
+ * history, with the results of the one-instance context bit for bit.  The
+ * calls above that take no instance index reject such a context, except
+ * set_mixing (shared), propagators and destroy. */
+int cfft_scft_create_multi(cfft_plan* plan, int32_t n_inst, const double* f, const double* chiN,
+                           int32_t ns, int32_t m_hist, cfft_scft** out);
+int cfft_scft_set_params_inst(cfft_scft* ctx, int32_t inst, double f, double chiN);
+int cfft_scft_set_fields_inst(cfft_scft* ctx, int32_t inst, const double* wA, const double* wB);
+/* One iteration of every instance; out[8 n_inst], instance i at out + 8 i. */
+int cfft_scft_iterate_multi(cfft_scft* ctx, double* out);
+/* Iterate until each instance has residual < tol or max_iter iterations; an
+ * instance that meets either stops iterating while the others go on (its
+ * fields stay in the sweep).  iters_out[n_inst], out[8 n_inst]. */
+int cfft_scft_solve_multi(cfft_scft* ctx, double tol, int32_t max_iter, int32_t* iters_out, double* out);
+int cfft_scft_get_inst(cfft_scft* ctx, int32_t inst, int32_t which, double* out);
 
 /* ---- warm-start network (multilayer perceptron) ---------------------------- */
 /* No counterpart in the reference.  dims[0..n_layers]: input width, hidden

@@ -59,3 +59,22 @@ def test_plane_pair_path_eligibility(sg, N, lat, expect):
     ops = get_ops(sg, 3, N)
     shift, ops_s = find_optimal_shift(ops, N)
     assert qs_eligible(N, ops_s, lat) is expect
+
+
+def test_groups_of_partitions_by_grid_group_cell_ns():
+    """Grouping for shared contexts: every instance once, each group one
+    (grid, group, cell, Ns), at most max_group long, largest work first."""
+    from cfft_hip import batch
+    insts = batch.draw_instances(64, 11)
+    gs = batch.groups_of(insts)
+    assert sorted(k for g in gs for k in g) == list(range(64))
+    for g in gs:
+        keys = {(tuple(insts[k].N), insts[k].sg, insts[k].cell, insts[k].ns) for k in g}
+        assert len(keys) == 1
+    work = [len(g) * int(np.prod(insts[g[0]].N)) for g in gs]
+    assert work == sorted(work, reverse=True)
+    keys = {(tuple(i.N), i.sg) for i in insts}
+    assert len(gs) == len(keys)
+    small = batch.groups_of(insts, max_group=2)
+    assert max(len(g) for g in small) == 2
+    assert sorted(k for g in small for k in g) == list(range(64))

@@ -288,6 +288,81 @@ def test_solver_update_equals_fresh_solver():
         assert x == y
 
 
+@pytest.mark.parametrize("sg,N,prec", [(229, (16, 16, 16), cf.FP64), (229, (16, 16, 16), cf.MIXED),
+                                       (47, (16, 16, 16), cf.FP64), (70, (32, 32, 32), cf.FP64)])
+def test_scft_batch_equals_single_instances(sg, N, prec):
+    """Three melts (different f, chiN and start fields) in one multi-instance
+    context: every iteration's results, the fields, the densities and the
+    per-instance stop of solve() are those of one-instance solvers bit for
+    bit (the sweep computes each field independently; every reduction has the
+    same block decomposition)."""
+    from cfft_hip import batch
+    from cfft_hip.scft import SCFTBatch, SCFTSolver
+    print(f"scft batch sg={sg} N={N} prec={prec}", flush=True)
+    shift, ops_s = cf.find_optimal_shift(cf.get_ops(sg, 3, N), N)
+    params = [(0.30, 16.0, 11), (0.42, 22.0, 12), (0.35, 28.0, 13)]
+    insts = [batch.Instance(N, sg, 4.0, f, x, 40, seed) for f, x, seed in params]
+    starts = [batch.start_fields(i, ops_s, shift) for i in insts]
+    lat = 4.0 * np.eye(3)
+    multi = SCFTBatch(N, ops_s, lat, [p[0] for p in params], [p[1] for p in params], 40, prec)
+    singles = [SCFTSolver(N, ops_s, lat, f, x, 40, prec) for f, x, _ in params]
+    assert multi.plan.path == singles[0].plan.path
+    for i, (wA, wB) in enumerate(starts):
+        multi.set_fields(i, wA, wB)
+        singles[i].set_fields(wA, wB)
+    for _ in range(4):
+        a = multi.iterate()
+        b = [s.iterate() for s in singles]
+        assert a == b
+    for i, s in enumerate(singles):
+        for x, y in zip(multi.fields(i) + multi.densities(i), s.fields() + s.densities()):
+            assert np.array_equal(x, y)
+    # solve(): instances stop at their own convergence
+    tol = 1e-6 if prec == cf.FP64 else 1e-4
+    for i, (wA, wB) in enumerate(starts):
+        multi.set_fields(i, wA, wB)
+        singles[i].set_fields(wA, wB)
+    its, outs = multi.solve(tol=tol, max_iter=120)
+    ref = [s.solve(tol=tol, max_iter=120) for s in singles]
+    print(f"  iterations {its}", flush=True)
+    assert its == [r[0] for r in ref]
+    assert outs == [r[1] for r in ref]
+    for i, s in enumerate(singles):
+        assert np.array_equal(multi.fields(i)[0], s.fields()[0])
+
+
+def test_scft_batch_rejects_single_calls():
+    """A context of several instances takes the indexed calls only; the plan's
+    batch must be twice the instance count."""
+    from cfft_hip.scft import SCFTBatch
+    N = (16, 16, 16)
+    _, ops_s = cf.find_optimal_shift(cf.get_ops(229, 3, N), N)
+    m = SCFTBatch(N, ops_s, 4.0 * np.eye(3), [0.3, 0.4], [20.0, 20.0], 20, cf.FP64)
+    z = np.zeros(m.M3)
+    with pytest.raises(cf.CFFTError):
+        cf._lib.check(cf._lib.lib().cfft_scft_set_fields(m._h, cf._lib.f64p(z), cf._lib.f64p(z)))
+    with pytest.raises(cf.CFFTError):
+        m.set_fields(2, z, z)
+    import ctypes as C
+    h = C.c_void_p()
+    f = np.array([0.3, 0.4, 0.5])
+    with pytest.raises(cf.CFFTError):
+        cf._lib.check(cf._lib.lib().cfft_scft_create_multi(m.plan.handle, 3, cf._lib.f64p(f), cf._lib.f64p(f),
+                                                           20, 4, C.byref(h)))
+
+
+def test_batched_grouped_equals_ungrouped():
+    """solve_batch with instances grouped by (grid, group) into shared
+    contexts gives every instance the iterations and results of solving it
+    in a context of its own."""
+    from cfft_hip import batch
+    insts = batch.draw_instances(8, seed=21, sizes=(16, 24), groups=(229, 47), ns=40)
+    assert len(batch.groups_of(insts)) < len(insts)
+    g, _, _ = batch.solve_batch(insts, precision=cf.FP64, tol=1e-7, max_iter=200, workers=4, group=True)
+    u, _, _ = batch.solve_batch(insts, precision=cf.FP64, tol=1e-7, max_iter=200, workers=4, group=False)
+    assert g == u
+
+
 def test_warm_start_pipeline_small():
     """End-to-end at 16^3 with a small network: generate converged solutions,
     train, predict, decode and solve from the prediction (fallback counted)."""
