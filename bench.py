@@ -279,11 +279,15 @@ def scft_ia3d(cf, tol64, tol32, max_iter):
         el = time.perf_counter() - t0
         out[name] = (it, o, el, solver.densities())
         del solver
-    # mixed precision to 1e-6, then an FP64 finish to 1e-8 from those fields
-    s32 = cf.scft.SCFTSolver(n3, ops_s, lat, 0.35, 18.0, 200, cf.MIXED, m_hist=10)
+    # mixed precision to the hand-off residual, then an FP64 finish to 1e-8
+    # from those fields (the mixed stage on the plane-pair chain, whose
+    # mixed-precision floor lies below the hand-off)
+    hand = cf.scft.QS_MIXED_TOL
+    s32 = cf.scft.SCFTSolver(n3, ops_s, lat, 0.35, 18.0, 200, cf.MIXED, m_hist=10,
+                             path=cf.scft.default_path(n3, ops_s, lat, cf.MIXED, hand))
     s32.set_fields(wA, wB)
     t0 = time.perf_counter()
-    ia, oa = s32.solve(tol=tol32, max_iter=max_iter)
+    ia, oa = s32.solve(tol=hand, max_iter=max_iter)
     fin = cf.scft.SCFTSolver(n3, ops_s, lat, 0.35, 18.0, 200, cf.FP64, m_hist=10)
     fin.set_fields(*s32.fields())
     ib, ob = fin.solve(tol=tol64, max_iter=max_iter)
@@ -291,7 +295,7 @@ def scft_ia3d(cf, tol64, tol32, max_iter):
     del s32, fin
     i64, o64, e64, p64 = out["fp64"]
     i32, o32, e32, p32 = out["mixed"]
-    refine = {"iters_mixed": ia, "iters_fp64": ib, "res": ob["res"], "F": ob["F"], "wall_s": e_mf,
+    refine = {"hand_off": hand, "iters_mixed": ia, "iters_fp64": ib, "res": ob["res"], "F": ob["F"], "wall_s": e_mf,
               "abs_dF_vs_fp64": abs(ob["F"] - o64["F"]), "speedup_vs_fp64": e64 / e_mf}
     return {
         "start": "double gyroid {211} star, amplitude 4, w_A low on the networks",

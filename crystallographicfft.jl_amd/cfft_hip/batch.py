@@ -19,7 +19,7 @@ import numpy as np
 
 from . import _lib as L
 from .plans import auto_L, find_optimal_shift
-from .scft import SCFTBatch, SCFTSolver, symmetrize_subgrid
+from .scft import SCFTBatch, SCFTSolver, default_path, symmetrize_subgrid
 from .symmetry_ops import get_ops
 
 
@@ -83,24 +83,26 @@ def start_fields(inst: Instance, ops_s, shift, amp=2.0, noise=0.1):
     return wA, wB
 
 
-def _prepare(inst: Instance, precision, m_hist, finish):
+def _prepare(inst: Instance, precision, m_hist, finish, tol):
     ops = get_ops(inst.sg, 3, inst.N)
     shift, ops_s = find_optimal_shift(ops, inst.N)
-    mk = lambda prec: SCFTSolver(inst.N, ops_s, inst.cell * np.eye(3), inst.f, inst.chiN, inst.ns, prec,
-                                 path=None, m_hist=m_hist)
-    solver = mk(precision)
-    fin = mk(L.FP64) if finish else None
+    lat = inst.cell * np.eye(3)
+    mk = lambda prec, path: SCFTSolver(inst.N, ops_s, lat, inst.f, inst.chiN, inst.ns, prec, path=path,
+                                       m_hist=m_hist)
+    solver = mk(precision, default_path(inst.N, ops_s, lat, precision, tol))
+    fin = mk(L.FP64, None) if finish else None
     return solver, fin, start_fields(inst, ops_s, shift)
 
 
-def _prepare_group(insts, precision, m_hist, finish):
+def _prepare_group(insts, precision, m_hist, finish, tol):
     i0 = insts[0]
     ops = get_ops(i0.sg, 3, i0.N)
     shift, ops_s = find_optimal_shift(ops, i0.N)
-    mk = lambda prec: SCFTBatch(i0.N, ops_s, i0.cell * np.eye(3), [i.f for i in insts], [i.chiN for i in insts],
-                                i0.ns, prec, path=None, m_hist=m_hist)
-    solver = mk(precision)
-    fin = mk(L.FP64) if finish else None
+    lat = i0.cell * np.eye(3)
+    mk = lambda prec, path: SCFTBatch(i0.N, ops_s, lat, [i.f for i in insts], [i.chiN for i in insts], i0.ns,
+                                      prec, path=path, m_hist=m_hist)
+    solver = mk(precision, default_path(i0.N, ops_s, lat, precision, tol))
+    fin = mk(L.FP64, None) if finish else None
     return solver, fin, [start_fields(i, ops_s, shift) for i in insts]
 
 
@@ -134,7 +136,8 @@ def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8,
     """Plan every instance (or group of instances), then solve all of them
     concurrently.  With finish_tol, each instance continues in FP64 from its
     `precision` solution until the residual is below finish_tol (its FP64
-    plan is made in the planning phase).  Returns (results, plan_seconds,
+    plan is made in the planning phase).  The path follows
+    ``scft.default_path`` for the stage's tolerance.  Returns (results, plan_seconds,
     solve_seconds); results[i] = dict(iters, F, res, Q[, iters_finish]) in the
     order of `instances`."""
     finish = finish_tol is not None
@@ -143,10 +146,10 @@ def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8,
         groups = groups_of(instances)
         with ThreadPoolExecutor(max_workers=workers) as ex:
             prepared = list(ex.map(lambda g: _prepare_group([instances[k] for k in g], precision, m_hist,
-                                                            finish), groups))
+                                                            finish, tol), groups))
     else:
         with ThreadPoolExecutor(max_workers=workers) as ex:
-            prepared = list(ex.map(lambda inst: _prepare(inst, precision, m_hist, finish), instances))
+            prepared = list(ex.map(lambda inst: _prepare(inst, precision, m_hist, finish, tol), instances))
     t1 = time.perf_counter()
 
     def run(item):

@@ -131,13 +131,21 @@ class PropagatorSweep:
         return [out[b].reshape(self.M, order="F") for b in range(2)]
 
 
-def default_path(N, ops_shifted, lattice, precision):
+# Residual the plane-pair chain reaches in mixed precision with margin (its
+# floor is ~4e-6 at Ia-3d 128^3, Ns 200)
+QS_MIXED_TOL = 1e-5
+
+
+def default_path(N, ops_shifted, lattice, precision, tol=None):
     """FP64: the plane-pair chain where it applies.  Mixed precision keeps the
-    default path: there every step passes through the spectral ASU, which
+    default path unless the solve stops at a residual tol >= QS_MIXED_TOL: on
+    the default path every step passes through the spectral ASU, which
     projects FP32 rounding back onto the symmetric fields, while on the
     plane-pair chain the asymmetric part of the rounding accumulates over a
     sweep (residual floor ~4e-6 instead of ~7e-7 at Ia-3d 128^3, Ns 200)."""
-    if precision == L.FP64 and qs_eligible(N, ops_shifted, lattice):
+    if not qs_eligible(N, ops_shifted, lattice):
+        return L.PATH_AUTO
+    if precision == L.FP64 or (tol is not None and tol >= QS_MIXED_TOL):
         return L.PATH_QS
     return L.PATH_AUTO
 

@@ -12,11 +12,13 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <limits>
 #include <memory>
 #include <vector>
 
 #include "../../include/cfft_hip.h"
 #include "common.hpp"
+#include "lstsq.hpp"
 
 using namespace cfft;
 
@@ -197,35 +199,6 @@ std::vector<double> simpson(int n) {
   return w;
 }
 
-// Least squares U c = V by Gaussian elimination with partial pivoting; false
-// when U is numerically singular.
-bool solve_small(std::vector<double> U, std::vector<double> V, int k, std::vector<double>& c) {
-  double scale = 0.0;
-  for (int i = 0; i < k; ++i) scale = std::max(scale, std::fabs(U[i * k + i]));
-  if (scale == 0.0) return false;
-  for (int col = 0; col < k; ++col) {
-    int piv = col;
-    for (int r = col + 1; r < k; ++r)
-      if (std::fabs(U[r * k + col]) > std::fabs(U[piv * k + col])) piv = r;
-    if (std::fabs(U[piv * k + col]) < 1e-14 * scale) return false;
-    if (piv != col) {
-      for (int j = 0; j < k; ++j) std::swap(U[col * k + j], U[piv * k + j]);
-      std::swap(V[col], V[piv]);
-    }
-    for (int r = col + 1; r < k; ++r) {
-      const double f = U[r * k + col] / U[col * k + col];
-      for (int j = col; j < k; ++j) U[r * k + j] -= f * U[col * k + j];
-      V[r] -= f * V[col];
-    }
-  }
-  c.assign(k, 0.0);
-  for (int r = k - 1; r >= 0; --r) {
-    double a = V[r];
-    for (int j = r + 1; j < k; ++j) a -= U[r * k + j] * c[j];
-    c[r] = a / U[r * k + r];
-  }
-  return true;
-}
 
 }  // namespace
 
@@ -391,7 +364,7 @@ void finish_inst(cfft_scft* c, int i, double* out) {
         U[a * k + b] = g(slot, slot) - g(slot, sa) - g(slot, sb) + g(sa, sb);
       }
     }
-    if (solve_small(U, V, k, cc)) {
+    if (lstsq_sym(U, V, k, cc)) {
       mp.k = k;
       mp.lam = c->lam;
       for (int a = 0; a < k; ++a) {

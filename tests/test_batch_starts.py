@@ -78,3 +78,22 @@ def test_groups_of_partitions_by_grid_group_cell_ns():
     small = batch.groups_of(insts, max_group=2)
     assert max(len(g) for g in small) == 2
     assert sorted(k for g in small for k in g) == list(range(64))
+
+
+def test_default_path_by_precision_and_tolerance():
+    """FP64 takes the plane-pair chain where it applies; mixed precision takes
+    it only for a stage that stops at residual >= QS_MIXED_TOL (above that
+    chain's mixed-precision floor), otherwise the default path."""
+    import cfft_hip as cf
+    from cfft_hip.scft import QS_MIXED_TOL, default_path
+    N = (16, 16, 16)
+    _, ops_s = find_optimal_shift(get_ops(229, 3, N), N)
+    lat = 4.0 * np.eye(3)
+    assert default_path(N, ops_s, lat, cf.FP64) == cf.PATH_QS
+    assert default_path(N, ops_s, lat, cf.MIXED) == cf.PATH_AUTO
+    assert default_path(N, ops_s, lat, cf.MIXED, QS_MIXED_TOL) == cf.PATH_QS
+    assert default_path(N, ops_s, lat, cf.MIXED, QS_MIXED_TOL / 10) == cf.PATH_AUTO
+    # a non-orthogonal cell is not eligible in either precision
+    skew = lat + 0.3 * np.eye(3, k=1)
+    assert default_path(N, ops_s, skew, cf.FP64) == cf.PATH_AUTO
+    assert default_path(N, ops_s, skew, cf.MIXED, 1e-3) == cf.PATH_AUTO
