@@ -322,7 +322,10 @@ def batched_fields(cf, torch, ops_s, wA, prec, e, batch, steps, path):
     E = torch.exp(-w * (0.5 / NS)).repeat(batch)
     q = torch.ones(batch * M3, dtype=dt, device="cuda")
     s = torch.cuda.current_stream().cuda_stream
-    plan.step_dev(q.data_ptr(), E.data_ptr(), 10, s)  # warm-up (captures the graph)
+    # warm-up with the timed call's own arguments: it captures the hipGraph
+    # that the timed call replays (a call with another step count would
+    # capture a graph of its own inside the timed region)
+    plan.step_dev(q.data_ptr(), E.data_ptr(), steps, s)
     torch.cuda.synchronize()
     q.fill_(1.0)
     torch.cuda.synchronize()

@@ -748,16 +748,22 @@ void launch_p2q(cfft_plan* p, const PipeP& pp, const PipeBufs& B, hipStream_t s)
   bool f32 = false;  // FP32-arithmetic kernels are instantiated for FP32 data only
   if constexpr (std::is_same<T, float>::value) {
     if (fp32_arith(p)) {
-#define CFFT_P2Q(L) \
-  case L: hipLaunchKernelGGL((k_p2q<T, float, L>), dim3(g), dim3(kQNT), 0, s, pp, B); break;
+#define CFFT_P2Q(L)                                                                          \
+  case L:                                                                                    \
+    if (pp.qs_sep) hipLaunchKernelGGL((k_p2q<T, float, L, true>), dim3(g), dim3(kQNT), 0, s, pp, B); \
+    else hipLaunchKernelGGL((k_p2q<T, float, L, false>), dim3(g), dim3(kQNT), 0, s, pp, B);          \
+    break;
       CFFT_LEN_SWITCH(p->H[1], CFFT_P2Q)
 #undef CFFT_P2Q
       f32 = true;
     }
   }
   if (!f32) {
-#define CFFT_P2Q(L) \
-  case L: hipLaunchKernelGGL((k_p2q<T, double, L>), dim3(g), dim3(kQNT), 0, s, pp, B); break;
+#define CFFT_P2Q(L)                                                                           \
+  case L:                                                                                     \
+    if (pp.qs_sep) hipLaunchKernelGGL((k_p2q<T, double, L, true>), dim3(g), dim3(kQNT), 0, s, pp, B); \
+    else hipLaunchKernelGGL((k_p2q<T, double, L, false>), dim3(g), dim3(kQNT), 0, s, pp, B);          \
+    break;
     CFFT_LEN_SWITCH(p->H[1], CFFT_P2Q)
 #undef CFFT_P2Q
   }

@@ -476,7 +476,7 @@ __device__ __forceinline__ void unit_lines(int u, int off, int& a, int& b, bool&
 // (1/8 folded into F0).  Octets partition the item's elements and each thread
 // owns whole octets, so the multiply runs in place between the forward and
 // the inverse transforms.
-template <typename T, typename TW, int L, int NT>
+template <typename T, typename TW, int L, int NT, bool SEP>
 __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds, cx<TW>* fq) {
   constexpr int rs = L + 1, NP = L * L, SL = L * rs, NF = 2 + 2 * L;
   constexpr int NFT = 8 * NF, FIT = (NFT + NT - 1) / NT;
@@ -515,7 +515,7 @@ __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx
   __syncthreads();
   lds_fft<T, TW, L, -1, NT>(lds, ns * L, rs, 1);         // FFT y: rows of both planes
   lds_fft2<T, TW, L, -1, NT>(lds, L, 1, ns, SL, rs);      // FFT z: columns of both planes
-  if (p.qs_sep) {
+  if constexpr (SEP) {
     // t_{s,d} depends on bit d of s only: Q_s(q) = prod_d g_d(s_d, q_d) with
     // g_d(s_d, .) = F_d[s_d << d][.], and the octet sum factors into three
     // two-point stages (z, y, x).  Work units pair rows {z, zbar} and columns
@@ -835,11 +835,13 @@ __global__ void __launch_bounds__(256) k_expand(const cx<T>* __restrict__ grep, 
 // per launch and loses a third at 16: one workgroup per CU instead of two).
 constexpr int kQNT = 256;
 
-template <typename T, typename TW, int L>
+// SEP: the separable multiply (p.qs_sep) or the general one, compiled apart so
+// that each gets its own register allocation.
+template <typename T, typename TW, int L, bool SEP>
 __global__ void __launch_bounds__(kQNT) k_p2q(PipeP p, PipeBufs B) {
   __shared__ cx<T> lds[2 * L * (L + 1)];
   __shared__ cx<TW> fq[8 * (2 + 2 * L)];
-  ph_plane_pair_q<T, TW, L, kQNT>(p, B, blockIdx.x, lds, fq);
+  ph_plane_pair_q<T, TW, L, kQNT, SEP>(p, B, blockIdx.x, lds, fq);
 }
 
 template <typename T, typename TW, int L0>
