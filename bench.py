@@ -63,8 +63,9 @@ def stage_bytes(plan, e):
         "p5_ifftx_unfold": B * (chan + 2 * M3 * e),        # channels, E -> f0
         # (QR) channels gathered at the representatives -> representative values
         "qmul": B * ((min(chan, plan.n_rep * plan.d * 2 * e) + rep) if qr else 2 * chan),
-        # channels, E -> f0 (q(s+1)) -> channels
-        "p51_unfold_fold": B * (2 * chan + 2 * M3 * e),
+        # between two steps of a plain sweep: channels and E in, channels
+        # out (q(s + 1) stays in registers; a stored sweep adds M3 e per field)
+        "p51_unfold_fold": B * (2 * chan + M3 * e),
         # (QS) channels -> FFT y, z, Q multiply, IFFT z, y -> channels
         "p2q_plane_pair": B * 2 * chan,
     }

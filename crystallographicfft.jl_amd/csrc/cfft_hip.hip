@@ -777,16 +777,22 @@ void launch_p51(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipS
   bool f32 = false;  // FP32-arithmetic kernels are instantiated for FP32 data only
   if constexpr (std::is_same<T, float>::value) {
     if (fp32_arith(p)) {
-#define CFFT_P51(L) \
-  case L: hipLaunchKernelGGL((k_p51<T, float, L>), dim3(g), dim3(kNT), 0, s, pp, B, step); break;
+#define CFFT_P51(L)                                                                             \
+  case L:                                                                                       \
+    if (B.f0_out) hipLaunchKernelGGL((k_p51<T, float, L, true>), dim3(g), dim3(kNT), 0, s, pp, B, step); \
+    else hipLaunchKernelGGL((k_p51<T, float, L, false>), dim3(g), dim3(kNT), 0, s, pp, B, step);        \
+    break;
       CFFT_LEN_SWITCH(p->H[0], CFFT_P51)
 #undef CFFT_P51
       f32 = true;
     }
   }
   if (!f32) {
-#define CFFT_P51(L) \
-  case L: hipLaunchKernelGGL((k_p51<T, double, L>), dim3(g), dim3(kNT), 0, s, pp, B, step); break;
+#define CFFT_P51(L)                                                                              \
+  case L:                                                                                        \
+    if (B.f0_out) hipLaunchKernelGGL((k_p51<T, double, L, true>), dim3(g), dim3(kNT), 0, s, pp, B, step); \
+    else hipLaunchKernelGGL((k_p51<T, double, L, false>), dim3(g), dim3(kNT), 0, s, pp, B, step);        \
+    break;
     CFFT_LEN_SWITCH(p->H[0], CFFT_P51)
 #undef CFFT_P51
   }
@@ -970,6 +976,13 @@ void exp_dev(cfft_plan* p, const T* w, T* E, double factor, long n, hipStream_t 
 // for the first step, then per step P3, P4 and either P5 fused with the next
 // step's P1 (followed by its P2) or, after the last step, P5.  Step i reads
 // `q + i * stride` and writes `q + (i + 1) * stride` (stride 0: in place).
+// Buffers of a fused unfold/fold between two steps: without a history
+// (stride 0) the intermediate q has no reader and is not written.
+PipeBufs inner_bufs(PipeBufs B, long stride) {
+  if (stride == 0) B.f0_out = nullptr;
+  return B;
+}
+
 template <typename T>
 void run_chain(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int nsteps, hipStream_t s) {
   cx<T>* F = static_cast<cx<T>*>(p->d_F.p);
@@ -980,7 +993,7 @@ void run_chain(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int 
       PipeBufs B = pipe_bufs(p, q + (i + 1) * stride, q + (i + 1) * stride, F, false);
       launch_p2q<T>(p, pp, B, s);
       if (i + 1 < nsteps && p->fuse) {
-        launch_p51<T>(p, pp, B, step0 + i, s);
+        launch_p51<T>(p, pp, inner_bufs(B, stride), step0 + i, s);
       } else {
         launch_p5<T>(p, pp, B, step0 + i, s);
         if (i + 1 < nsteps) launch_p1<T>(p, pp, B, step0 + i + 1, s);
@@ -998,7 +1011,7 @@ void run_chain(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int 
       launch_p4<T>(p, pp, B, s);
     }
     if (i + 1 < nsteps && p->fuse) {
-      launch_p51<T>(p, pp, B, step0 + i, s);
+      launch_p51<T>(p, pp, inner_bufs(B, stride), step0 + i, s);
       launch_p2<T>(p, pp, B, s);
     } else {
       launch_p5<T>(p, pp, B, step0 + i, s);
@@ -1089,14 +1102,16 @@ void run_steps_hist(cfft_plan* p, T* hist, const ESel& es, int step0, int nsteps
 
 // Average duration of one launch of a chain kernel, measured live: one
 // untimed launch, then `reps` identical launches back to back on `s` between
-// two hipEvents (the same launches a kernel trace lists one by one).
+// two hipEvents (the same launches a kernel trace lists one by one).  The
+// unfold/fold kernel runs as it does between two steps of a plain sweep
+// (q(s + 1) kept in registers).
 template <typename T>
 double time_stage(cfft_plan* p, T* q, const ESel& es, int stage, int reps, hipStream_t s) {
   PipeP pp = pipe_params(p, es);
   PipeBufs B = pipe_bufs(p, nullptr, q, p->d_F.p, false);
   auto launch = [&] {
     if (stage == CFFT_STAGE_UNFOLD_FOLD)
-      launch_p51<T>(p, pp, B, 0, s);
+      launch_p51<T>(p, pp, inner_bufs(B, 0), 0, s);  // as between two steps of a sweep
     else if (stage == CFFT_STAGE_PLANE_Q)
       launch_p2q<T>(p, pp, B, s);
     else

@@ -46,7 +46,7 @@ struct PipeP {
 
 struct PipeBufs {
   const void* f0_in;  // P1 input
-  void* f0_out;       // P5 output
+  void* f0_out;       // P5 output; P51 writes q(s + 1) there when it is non-null
   void* ch;           // channels
   void* F;            // spectral coefficients
   const int* hs;      // signed spectral frequencies (3 per point)
@@ -682,9 +682,12 @@ __device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, in
 }
 
 // ---- P5 of step s fused with P1 of step s + 1 ------------------------------
-// IFFT x + unfold + E_s writes q(s + 1) to f0_out; the same thread then folds
-// E_{s+1} q(s + 1) at the same tile points, so q(s + 1) is not read back.
-template <typename T, typename TW, int L0>
+// IFFT x + unfold + E_s gives q(s + 1), written to f0_out when STORE (a sweep
+// that keeps every contour point, or the caller's q; between two steps of a
+// plain multi-step sweep q(s + 1) has no reader and stays in registers); the
+// same thread then folds E_{s+1} q(s + 1) at the same tile points, so q(s + 1)
+// is never read back.
+template <typename T, typename TW, int L0, bool STORE>
 __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds) {
   const int H0 = p.H[0], H1 = p.H[1], H2 = p.H[2], YB = p.YB;
   const int nyb = H1 / YB;
@@ -695,7 +698,7 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
   const int rs = H0 + 1;
   cx<T>* ch = static_cast<cx<T>*>(B.ch);
   const long M3 = (long)p.M[0] * p.M[1] * p.M[2];
-  T* fb = static_cast<T*>(B.f0_out) + b * M3;
+  T* fb = STORE ? static_cast<T*>(B.f0_out) + b * M3 : nullptr;
   const T* Eo = pipe_E<T>(p, b, step, M3);
   const T* En = pipe_E<T>(p, b, step + 1, M3);
   const cx<double>* twM = static_cast<const cx<double>*>(B.twM);
@@ -731,7 +734,7 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
     for (int e = 0; e < 8; ++e) {
       T r = (T)((TW)v[e] * p.scale_unfold);
       if (Eo) r *= eo[k][e];
-      fb[corner_index(p, x, y, z, e)] = r;
+      if constexpr (STORE) fb[corner_index(p, x, y, z, e)] = r;
       v[e] = En ? r * en[k][e] : r;
     }
     wht8(v);
@@ -844,10 +847,12 @@ __global__ void __launch_bounds__(kQNT) k_p2q(PipeP p, PipeBufs B) {
   ph_plane_pair_q<T, TW, L, kQNT, SEP>(p, B, blockIdx.x, lds, fq);
 }
 
-template <typename T, typename TW, int L0>
+// STORE: q(s + 1) written to B.f0_out, or (between two steps of a plain
+// sweep) not written at all.
+template <typename T, typename TW, int L0, bool STORE>
 __global__ void __launch_bounds__(kNT) k_p51(PipeP p, PipeBufs B, int step) {
   __shared__ cx<T> lds[64 * (L0 + 1)];
-  ph_unfold_fold<T, TW, L0>(p, B, blockIdx.x, step, lds);
+  ph_unfold_fold<T, TW, L0, STORE>(p, B, blockIdx.x, step, lds);
 }
 
 // ---- persistent sweep: nsteps Strang steps in one cooperative launch ---------
