@@ -532,8 +532,19 @@ def main():
             avg_ms = stages[dom][0] / stages[dom][1]
         nb = launch_bytes(sw.plan, e, dom, min(args.steps, NS))
         achieved = nb / (avg_ms * 1e-3) / 1e9
+        # every per-step chain kernel beside the dominant one, timed the same way
+        chain = {}
+        for st in stages:
+            if st not in TIMED_STAGES:
+                continue
+            scratch = sw.q.clone()
+            ms = avg_ms if st == dom else live_launch_ms(sw.plan, st, scratch.data_ptr(), sw._E1, sw.stream())
+            del scratch
+            b = launch_bytes(sw.plan, e, st, min(args.steps, NS))
+            chain[st] = {"avg_launch_ms": ms, "bytes_per_launch": b, "achieved": b / (ms * 1e-3) / 1e9,
+                         "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
         res[name] = dict(el=el, med=med, Qq=Qq, Qd=Qd, q=qfields, stages=stages, dom=dom, avg_ms=avg_ms,
-                         achieved=achieved, plan=sw.plan, sbytes=nb,
+                         achieved=achieved, plan=sw.plan, sbytes=nb, chain=chain,
                          step_bytes=step_bytes(sw.plan, e))
         del sw
 
@@ -603,13 +614,13 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(variants["fp64"]["used"], r64["dom"], "fp64"),
                      "bytes_per_launch": r64["sbytes"],
-                     "avg_launch_ms": r64["avg_ms"]},
+                     "avg_launch_ms": r64["avg_ms"], "chain_kernels": r64["chain"]},
         "mixed": {"value": mixed_value, "dtype": "f32 data/FFT/propagator, f64 tables+reductions",
                   "speedup_vs_fp64": mixed_value / value, "abs_dQ_vs_fp64": dq,
                   "max_rel_q_err_vs_fp64": qerr,
                   "roofline": {"kernel": r32["dom"], "achieved": r32["achieved"],
                                "frac": r32["achieved"] / HBM_PEAK_GBS,
-                               "avg_launch_ms": r32["avg_ms"]}},
+                               "avg_launch_ms": r32["avg_ms"], "chain_kernels": r32["chain"]}},
         "Q": {"fp64_q": r64["Qq"], "fp64_qdagger": r64["Qd"], "mixed_q": r32["Qq"]},
         # the reference's bench method on full sweeps: median of 30 after 5 warm-ups
         "median_of_30": {"fp64_ms_per_step": r64["med"], "fp64_steps_per_s": 2e3 / r64["med"],
