@@ -429,6 +429,58 @@ __device__ void ph_expand_ifft_zy(const PipeP& p, const PipeBufs& B, long item, 
 }
 
 // ---- P2Q: FFT y, z + Q multiply + IFFT z, y on plane pairs (path QS) -------
+// Row transforms of a plane pair straight from / to global memory.  Transform
+// t < n_rows is row z = t mod L of plane t / L (p0 or p1, L x L, y fastest);
+// its lanes hold x[r + R1 m] on entry (a row is contiguous), exchange through
+// the transform's own LDS row (stride L + 1, plane stride L (L + 1)) and leave
+// the result there (from_global) or store it (to_global).  Groups past the
+// last row work on row 0 and store nothing.
+template <typename T, typename TW, int L, int DIR, int NT>
+__device__ __forceinline__ void rows_fft_from_global(const cx<T>* __restrict__ p0, const cx<T>* __restrict__ p1,
+                                                     int n_rows, cx<T>* lds) {
+  constexpr int R1 = MLS<L>::R1, R2 = MLS<L>::R2, G = PipeG<L, NT>::G, rs = L + 1;
+  const int r = threadIdx.x % R1, g = threadIdx.x / R1;
+  for (int t0 = 0; t0 < n_rows; t0 += G) {
+    const int t = t0 + g;
+    const bool ok = t < n_rows;
+    const int tt = ok ? t : 0;
+    const int pl = tt / L, z = tt - pl * L;
+    const cx<T>* src = (pl ? p1 : p0) + z * L;
+    cx<T> v[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) v[m] = src[r + R1 * m];
+    if (ok) {
+      cx<T>* slot = lds + pl * (L * rs) + z * rs;
+      fft_ml_lds<T, TW, L, DIR>(v, r, slot, 1);
+#pragma unroll
+      for (int q = 0; q < R2; ++q) slot[ml_out_pos<L>(r, q)] = v[q];
+    }
+  }
+}
+
+template <typename T, typename TW, int L, int DIR, int NT>
+__device__ __forceinline__ void rows_fft_to_global(cx<T>* lds, cx<T>* __restrict__ p0, cx<T>* __restrict__ p1,
+                                                   int n_rows) {
+  constexpr int R1 = MLS<L>::R1, R2 = MLS<L>::R2, G = PipeG<L, NT>::G, rs = L + 1;
+  const int r = threadIdx.x % R1, g = threadIdx.x / R1;
+  for (int t0 = 0; t0 < n_rows; t0 += G) {
+    const int t = t0 + g;
+    const bool ok = t < n_rows;
+    const int tt = ok ? t : 0;
+    const int pl = tt / L, z = tt - pl * L;
+    cx<T>* slot = lds + pl * (L * rs) + z * rs;
+    cx<T> v[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) v[m] = slot[r + R1 * m];
+    if (ok) {
+      fft_ml_lds<T, TW, L, DIR>(v, r, slot, 1);
+      cx<T>* dst = (pl ? p1 : p0) + z * L;
+#pragma unroll
+      for (int q = 0; q < R2; ++q) dst[ml_out_pos<L>(r, q)] = v[q];
+    }
+  }
+}
+
 // Lines of work unit u along an axis of length L whose points q = 2k + off map
 // to kbar = (L - k) mod L (off = 0) or L - 1 - k (off = 1): a pair (a, b) of
 // distinct lines (pair = true) or self-images (pair = false).  For even L and
@@ -505,15 +557,14 @@ __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx
     const int q = j < 2 ? 2 * (j ? x1 : x0) + o0 : (j < 2 + L ? 2 * (j - 2) + o1 : 2 * (j - 2 - L) + o2);
     fv[k] = tab<TW>(F, (d * 8 + sf) * Ms + q);
   }
-  plane_to_lds<T, L, L, NT>(chan + (long)x0 * NP, lds);
-  if (ns == 2) plane_to_lds<T, L, L, NT>(chan + (long)x1 * NP, lds + SL);
+  // FFT y: the rows of both planes, loaded straight into the transform lanes
+  rows_fft_from_global<T, TW, L, -1, NT>(chan + (long)x0 * NP, chan + (long)x1 * NP, ns * L, lds);
 #pragma unroll
   for (int k = 0; k < FIT; ++k) {
     const int i = threadIdx.x + k * NT;
     if (i < NFT) fq[i] = fv[k];
   }
   __syncthreads();
-  lds_fft<T, TW, L, -1, NT>(lds, ns * L, rs, 1);         // FFT y: rows of both planes
   lds_fft2<T, TW, L, -1, NT>(lds, L, 1, ns, SL, rs);      // FFT z: columns of both planes
   if constexpr (SEP) {
     // t_{s,d} depends on bit d of s only: Q_s(q) = prod_d g_d(s_d, q_d) with
@@ -614,9 +665,8 @@ __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx
   }
   __syncthreads();
   lds_fft2<T, TW, L, +1, NT>(lds, L, 1, ns, SL, rs);  // IFFT z
-  lds_fft<T, TW, L, +1, NT>(lds, ns * L, rs, 1);      // IFFT y
-  lds_to_plane<T, L, L, NT>(lds, chan + (long)x0 * NP);
-  if (ns == 2) lds_to_plane<T, L, L, NT>(lds + SL, chan + (long)x1 * NP);
+  // IFFT y, each row stored straight from its transform lanes
+  rows_fft_to_global<T, TW, L, +1, NT>(lds, chan + (long)x0 * NP, chan + (long)x1 * NP, ns * L);
   __syncthreads();
 }
 
