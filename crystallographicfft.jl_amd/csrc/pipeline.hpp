@@ -731,6 +731,63 @@ __device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, in
   __syncthreads();
 }
 
+// x transforms of the tile rows (c, yy) of tile (b, z, y0 .. y0 + YB) straight
+// from / to the channel array: transform t < n_ch * YB is row t = c * YB + yy
+// of the LDS tile A[c][yy][x] (row stride L0 + 1); its lanes hold x = r + R1 m
+// (a wave's loads cover the same 128-byte row segments as tile_to_lds),
+// exchange through the row and leave the result there (from_global) or
+// store it (to_global).  Groups past the last row work on row 0 and store
+// nothing.
+template <typename T, typename TW, int L0, int DIR>
+__device__ __forceinline__ void tile_rows_fft_from_global(const PipeP& p, const cx<T>* __restrict__ ch, int b, int z,
+                                                          int y0, cx<T>* lds) {
+  constexpr int R1 = MLS<L0>::R1, R2 = MLS<L0>::R2, G = PipeG<L0, kNT>::G, rs = L0 + 1;
+  const int n_rows = p.n_ch * p.YB;
+  const long HH = (long)p.H[2] * p.H[1];
+  const int r = threadIdx.x % R1, g = threadIdx.x / R1;
+  for (int t0 = 0; t0 < n_rows; t0 += G) {
+    const int t = t0 + g;
+    const bool ok = t < n_rows;
+    const int tt = ok ? t : 0;
+    const int c = tt >> p.ybs, yy = tt & (p.YB - 1);
+    const cx<T>* src = ch + ((long)b * p.n_ch + c) * L0 * HH + (long)z * p.H[1] + y0 + yy;
+    cx<T> v[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) v[m] = src[(long)(r + R1 * m) * HH];
+    if (ok) {
+      cx<T>* slot = lds + tt * rs;
+      fft_ml_lds<T, TW, L0, DIR>(v, r, slot, 1);
+#pragma unroll
+      for (int q = 0; q < R2; ++q) slot[ml_out_pos<L0>(r, q)] = v[q];
+    }
+  }
+}
+
+template <typename T, typename TW, int L0, int DIR>
+__device__ __forceinline__ void tile_rows_fft_to_global(const PipeP& p, cx<T>* lds, cx<T>* __restrict__ ch, int b,
+                                                        int z, int y0) {
+  constexpr int R1 = MLS<L0>::R1, R2 = MLS<L0>::R2, G = PipeG<L0, kNT>::G, rs = L0 + 1;
+  const int n_rows = p.n_ch * p.YB;
+  const long HH = (long)p.H[2] * p.H[1];
+  const int r = threadIdx.x % R1, g = threadIdx.x / R1;
+  for (int t0 = 0; t0 < n_rows; t0 += G) {
+    const int t = t0 + g;
+    const bool ok = t < n_rows;
+    const int tt = ok ? t : 0;
+    cx<T>* slot = lds + tt * rs;
+    cx<T> v[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) v[m] = slot[r + R1 * m];
+    if (ok) {
+      fft_ml_lds<T, TW, L0, DIR>(v, r, slot, 1);
+      const int c = tt >> p.ybs, yy = tt & (p.YB - 1);
+      cx<T>* dst = ch + ((long)b * p.n_ch + c) * L0 * HH + (long)z * p.H[1] + y0 + yy;
+#pragma unroll
+      for (int q = 0; q < R2; ++q) dst[(long)ml_out_pos<L0>(r, q) * HH] = v[q];
+    }
+  }
+}
+
 // ---- P5 of step s fused with P1 of step s + 1 ------------------------------
 // IFFT x + unfold + E_s gives q(s + 1), written to f0_out when STORE (a sweep
 // that keeps every contour point, or the caller's q; between two steps of a
@@ -769,9 +826,9 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
       en[k][e] = same_E ? eo[k][e] : (En ? En[idx] : T(1));
     }
   }
-  tile_to_lds<T, L0>(p, ch, b, z, y0, lds);
+  // IFFT x of the tile rows, loaded straight into the transform lanes
+  tile_rows_fft_from_global<T, TW, L0, +1>(p, ch, b, z, y0, lds);
   __syncthreads();
-  lds_fft<T, TW, L0, +1>(lds, p.n_ch * YB, rs, 1);
 #pragma unroll
   for (int k = 0; k < NPT; ++k) {
     const int pt = threadIdx.x + k * kNT;
@@ -792,8 +849,8 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
     scatter_parities<T, TW>(p, lds, yy * rs + x, YB * rs, ft[k], v);
   }
   __syncthreads();
-  lds_fft<T, TW, L0, -1>(lds, p.n_ch * YB, rs, 1);
-  lds_to_tile<T, L0>(p, lds, ch, b, z, y0);
+  // FFT x of the tile rows, each stored straight from its transform lanes
+  tile_rows_fft_to_global<T, TW, L0, -1>(p, lds, ch, b, z, y0);
   __syncthreads();
 }
 
