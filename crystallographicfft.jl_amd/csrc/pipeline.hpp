@@ -738,17 +738,44 @@ __device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, in
 // exchange through the row and leave the result there (from_global) or
 // store it (to_global).  Groups past the last row work on row 0 and store
 // nothing.
-template <typename T, typename TW, int L0, int DIR>
+// Wavefront-scope ordering of LDS accesses between the lanes of one wave.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Transform of group g for the tile rows: WAVE = false, row t = g + t0 of the
+// whole tile; WAVE = true, the rows of the wave's own tile points (wave w holds
+// the points of rows RPW w .. RPW w + RPW - 1, RPW = 64 / L0): its group gi
+// takes channel gi mod n_ch of row RPW w + gi / n_ch.
+template <int L0, bool WAVE>
+__device__ __forceinline__ bool tile_row_of(const PipeP& p, int t0, int g, int& tt) {
+  if constexpr (WAVE) {
+    constexpr int GW = 64 / MLS<L0>::R1, RPW = 64 / L0;  // groups and tile rows per wave
+    const int w = g / GW, gi = g - w * GW;
+    const int c = gi % p.n_ch, yy = RPW * w + gi / p.n_ch;
+    const bool ok = gi < RPW * p.n_ch && yy < p.YB;
+    tt = ok ? c * p.YB + yy : 0;
+    return ok;
+  } else {
+    const int t = t0 + g;
+    const bool ok = t < p.n_ch * p.YB;
+    tt = ok ? t : 0;
+    return ok;
+  }
+}
+
+template <typename T, typename TW, int L0, int DIR, bool WAVE = false>
 __device__ __forceinline__ void tile_rows_fft_from_global(const PipeP& p, const cx<T>* __restrict__ ch, int b, int z,
                                                           int y0, cx<T>* lds) {
   constexpr int R1 = MLS<L0>::R1, R2 = MLS<L0>::R2, G = PipeG<L0, kNT>::G, rs = L0 + 1;
-  const int n_rows = p.n_ch * p.YB;
+  const int n_rows = WAVE ? 1 : p.n_ch * p.YB;
   const long HH = (long)p.H[2] * p.H[1];
   const int r = threadIdx.x % R1, g = threadIdx.x / R1;
   for (int t0 = 0; t0 < n_rows; t0 += G) {
-    const int t = t0 + g;
-    const bool ok = t < n_rows;
-    const int tt = ok ? t : 0;
+    int tt;
+    const bool ok = tile_row_of<L0, WAVE>(p, t0, g, tt);
     const int c = tt >> p.ybs, yy = tt & (p.YB - 1);
     const cx<T>* src = ch + ((long)b * p.n_ch + c) * L0 * HH + (long)z * p.H[1] + y0 + yy;
     cx<T> v[R2];
@@ -763,17 +790,16 @@ __device__ __forceinline__ void tile_rows_fft_from_global(const PipeP& p, const 
   }
 }
 
-template <typename T, typename TW, int L0, int DIR>
+template <typename T, typename TW, int L0, int DIR, bool WAVE = false>
 __device__ __forceinline__ void tile_rows_fft_to_global(const PipeP& p, cx<T>* lds, cx<T>* __restrict__ ch, int b,
                                                         int z, int y0) {
   constexpr int R1 = MLS<L0>::R1, R2 = MLS<L0>::R2, G = PipeG<L0, kNT>::G, rs = L0 + 1;
-  const int n_rows = p.n_ch * p.YB;
+  const int n_rows = WAVE ? 1 : p.n_ch * p.YB;
   const long HH = (long)p.H[2] * p.H[1];
   const int r = threadIdx.x % R1, g = threadIdx.x / R1;
   for (int t0 = 0; t0 < n_rows; t0 += G) {
-    const int t = t0 + g;
-    const bool ok = t < n_rows;
-    const int tt = ok ? t : 0;
+    int tt;
+    const bool ok = tile_row_of<L0, WAVE>(p, t0, g, tt);
     cx<T>* slot = lds + tt * rs;
     cx<T> v[R2];
 #pragma unroll
@@ -826,9 +852,20 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
       en[k][e] = same_E ? eo[k][e] : (En ? En[idx] : T(1));
     }
   }
+  // With one tile point per thread and 64 % H0 = 0 each wave's points are
+  // whole tile rows; when the wave's groups can also take the x transforms of
+  // those rows (every channel), the wave only ever touches its own LDS rows
+  // and wavefront-scope ordering replaces the workgroup barriers.
+  constexpr bool ROWS_PER_WAVE = (64 % L0 == 0) && (L0 * 8 <= kNT);
+  const bool wave = ROWS_PER_WAVE && (64 / L0) * p.n_ch <= 64 / MLS<L0>::R1;
   // IFFT x of the tile rows, loaded straight into the transform lanes
-  tile_rows_fft_from_global<T, TW, L0, +1>(p, ch, b, z, y0, lds);
-  __syncthreads();
+  if (wave) {
+    tile_rows_fft_from_global<T, TW, L0, +1, true>(p, ch, b, z, y0, lds);
+    wave_sync();
+  } else {
+    tile_rows_fft_from_global<T, TW, L0, +1>(p, ch, b, z, y0, lds);
+    __syncthreads();
+  }
 #pragma unroll
   for (int k = 0; k < NPT; ++k) {
     const int pt = threadIdx.x + k * kNT;
@@ -848,10 +885,15 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
     // the entries of this thread's point only: no barrier needed before reuse
     scatter_parities<T, TW>(p, lds, yy * rs + x, YB * rs, ft[k], v);
   }
-  __syncthreads();
   // FFT x of the tile rows, each stored straight from its transform lanes
-  tile_rows_fft_to_global<T, TW, L0, -1>(p, lds, ch, b, z, y0);
-  __syncthreads();
+  if (wave) {
+    wave_sync();
+    tile_rows_fft_to_global<T, TW, L0, -1, true>(p, lds, ch, b, z, y0);
+  } else {
+    __syncthreads();
+    tile_rows_fft_to_global<T, TW, L0, -1>(p, lds, ch, b, z, y0);
+    __syncthreads();
+  }
 }
 
 // ---- kernels: one per phase -------------------------------------------------
