@@ -368,6 +368,12 @@ def batched_instances(cf, n, seed, workers):
     insts = batch.draw_instances(n, seed)
     ru, pu, su = batch.solve_batch(insts, precision=cf.FP64, tol=1e-8, max_iter=400, workers=workers,
                                    group=False)
+    rt, pt, st = batch.solve_batch(insts, precision=cf.FP64, tol=1e-8, max_iter=400, workers=workers,
+                                   tune=True)
+    sched = {}
+    for r in rt:
+        sched[r["schedule"]] = sched.get(r["schedule"], 0) + 1
+    rt_plain = [{k: v for k, v in r.items() if k != "schedule"} for r in rt]
     r32, p32, s32 = batch.solve_batch(insts, precision=cf.MIXED, tol=1e-5, max_iter=400, workers=workers)
     rmf, pmf, smf = batch.solve_batch(insts, precision=cf.MIXED, tol=1e-5, max_iter=400, workers=workers,
                                       finish_tol=1e-8)
@@ -394,6 +400,11 @@ def batched_instances(cf, n, seed, workers):
         "groups": len(batch.groups_of(insts)),
         "fp64_context_per_instance": {"instances_per_s": n / su, "solve_s": su, "plan_s": pu,
                                       "same_results": ru == r64, "grouped_speedup": su / s64},
+        # each group's launch schedule (tile rows, graph replay) timed on its
+        # own workload and the fastest applied (cfft_hip.tune)
+        "fp64_tuned": {"instances_per_s": n / st, "solve_s": st, "plan_and_tune_s": pt,
+                       "instances_by_schedule": sched, "same_results": rt_plain == r64,
+                       "speedup_vs_untuned": s64 / st},
     }
 
 

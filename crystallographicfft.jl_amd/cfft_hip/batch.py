@@ -132,12 +132,15 @@ def _finish(r, it2, o2, out):
 
 
 def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8, m_hist=10,
-                finish_tol=None, group=True):
+                finish_tol=None, group=True, tune=False):
     """Plan every instance (or group of instances), then solve all of them
     concurrently.  With finish_tol, each instance continues in FP64 from its
     `precision` solution until the residual is below finish_tol (its FP64
     plan is made in the planning phase).  The path follows
-    ``scft.default_path`` for the stage's tolerance.  Returns (results, plan_seconds,
+    ``scft.default_path`` for the stage's tolerance.  With tune (grouped
+    only), each group's launch schedule is timed on its own workload
+    (``tune.tune_schedule``, one group after another, in the planning phase)
+    and the fastest applied; results[i]["schedule"] names it.  Returns (results, plan_seconds,
     solve_seconds); results[i] = dict(iters, F, res, Q[, iters_finish]) in the
     order of `instances`."""
     finish = finish_tol is not None
@@ -147,6 +150,18 @@ def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8,
         with ThreadPoolExecutor(max_workers=workers) as ex:
             prepared = list(ex.map(lambda g: _prepare_group([instances[k] for k in g], precision, m_hist,
                                                             finish, tol), groups))
+        schedules = [None] * len(groups)
+        if tune:
+            from .tune import tune_schedule
+            for j, (solver, fin, _) in enumerate(prepared):
+                for sv in (solver, fin):
+                    if sv is None:
+                        continue
+                    p = sv.plan
+                    best, times = tune_schedule(p.N, p.ops, p.lattice, sv.ns, p.precision, p.batch, p.path)
+                    sv.set_schedule(best)
+                    if sv is solver:
+                        schedules[j] = min(times, key=times.get)
     else:
         with ThreadPoolExecutor(max_workers=workers) as ex:
             prepared = list(ex.map(lambda inst: _prepare(inst, precision, m_hist, finish, tol), instances))
@@ -180,8 +195,10 @@ def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8,
     with ThreadPoolExecutor(max_workers=workers) as ex:
         if group:
             results = [None] * len(instances)
-            for g, rs in zip(groups, ex.map(run_group, prepared)):
+            for g, sch, rs in zip(groups, schedules, ex.map(run_group, prepared)):
                 for k, r in zip(g, rs):
+                    if sch is not None:
+                        r["schedule"] = sch
                     results[k] = r
         else:
             results = list(ex.map(run, prepared))

@@ -274,6 +274,12 @@ class SCFTBatch:
                 pass
             self._h = None
 
+    def set_schedule(self, options):
+        """Plan options (e.g. the best of ``tune.tune_schedule``); results do
+        not depend on them."""
+        for k, v in options.items():
+            self.plan.set_option(int(k), int(v))
+
     def set_params(self, i, f, chiN):
         L.check(L.lib().cfft_scft_set_params_inst(self._h, int(i), float(f), float(chiN)))
         self.fs[i], self.chiNs[i] = float(f), float(chiN)

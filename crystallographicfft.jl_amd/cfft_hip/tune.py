@@ -1,0 +1,73 @@
+"""Launch-schedule tuning for a workload: the fold/unfold tile height and
+hipGraph replay of a plan, timed on the plan's own grid, group, precision and
+number of fields per launch.  No counterpart in the reference.
+
+The candidates do not change results (every schedule computes each field with
+the same arithmetic), only the decomposition of the phase kernels and how the
+step sequence is launched, so the choice is made by timing alone.  A timing
+is one contour sweep of `steps` Strang steps over `batch` fields with all
+interaction factors 1 (a plain multi-step sweep, the kernels of the stored
+sweeps apart from the q store), after one untimed sweep that captures the
+graph; the best of `reps` sweeps counts.  Results are cached per workload in
+this process.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _lib as L
+from .plans import DevicePlan
+
+_CACHE: dict = {}
+
+
+def schedule_candidates(H1):
+    """Tile heights 8, 4, 2 that divide H1, each with and without graph replay."""
+    rows = [r for r in (8, 4, 2) if H1 % r == 0] or [1]
+    return [{L.OPT_TILE_ROWS: r, L.OPT_GRAPH: g} for r in rows for g in (1, 0)]
+
+
+def _key(N, ops_shifted, lattice, ns, precision, batch, path):
+    ops = tuple((tuple(np.asarray(o.Rm).ravel().tolist()), tuple(np.asarray(o.tv).ravel().tolist()))
+                for o in ops_shifted)
+    return (tuple(N), hash(ops), tuple(np.asarray(lattice, dtype=float).ravel().tolist()), int(ns),
+            int(precision), int(batch), int(path))
+
+
+def tune_schedule(N, ops_shifted, lattice, ns, precision, batch, path, steps=None, reps=3):
+    """(best options, {label: microseconds per step}) for a plan of `batch`
+    fields with ds = 1/ns; options are {OPT_TILE_ROWS: r, OPT_GRAPH: g}."""
+    import torch
+    key = _key(N, ops_shifted, lattice, ns, precision, batch, path)
+    if key in _CACHE:
+        return _CACHE[key]
+    steps = int(ns if steps is None else steps)
+    plan = DevicePlan(N, ops_shifted, 1.0 / ns, lattice, path, precision, batch=batch)
+    M3 = int(np.prod(plan.M))
+    dt = torch.float64 if precision == L.FP64 else torch.float32
+    q = torch.ones(batch * M3, dtype=dt, device="cuda")
+    E = torch.ones(batch * M3, dtype=dt, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    times = {}
+    best, best_t = None, None
+    for opt in schedule_candidates(plan.H[1]):
+        for k, v in opt.items():
+            plan.set_option(k, v)
+        plan.step_dev(q.data_ptr(), E.data_ptr(), steps, s)  # captures the graph when enabled
+        torch.cuda.synchronize()
+        t = None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            plan.step_dev(q.data_ptr(), E.data_ptr(), steps, s)
+            torch.cuda.synchronize()
+            el = (time.perf_counter() - t0) / steps * 1e6
+            t = el if t is None else min(t, el)
+        label = f"rows{opt[L.OPT_TILE_ROWS]}_graph{opt[L.OPT_GRAPH]}"
+        times[label] = t
+        if best_t is None or t < best_t:
+            best, best_t = dict(opt), t
+    del plan
+    _CACHE[key] = (best, times)
+    return best, times

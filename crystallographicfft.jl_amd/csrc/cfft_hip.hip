@@ -56,6 +56,7 @@ struct cfft_plan {
   int persistent = 0;  // contour sweeps as one cooperative launch
   int fuse = 1;        // fuse P5 of a step with P1 of the next
   int graph = 1;       // replay multi-step chains from captured hipGraphs
+  int tile_rows = 0;   // y rows per fold/unfold tile (0: the default rule)
   hipStream_t cap_stream = nullptr;  // capture stream of the chains
   struct Cached {
     std::string key;
@@ -551,7 +552,9 @@ PipeP pipe_params(cfft_plan* p, const ESel& es) {
   for (int c = 0; c < p->n_ch; ++c) pp.offs[c] = p->offs[c][0] | (p->offs[c][1] << 1) | (p->offs[c][2] << 2);
   for (int i = 0; i < 8; ++i) pp.chmap[i] = p->chmap[i];
   pp.batch = p->batch;
-  int yb = 8;  // largest power of two <= 8 dividing H1 (shift/mask tile indexing)
+  // y rows per tile: a power of two (shift/mask tile indexing), by default the
+  // largest <= 8 dividing H1; n_ch * YB <= 64 rows fit the tile's LDS
+  int yb = p->tile_rows ? p->tile_rows : 8;
   while (p->H[1] % yb) yb >>= 1;
   pp.YB = yb;
   pp.ybs = yb == 8 ? 3 : (yb == 4 ? 2 : (yb == 2 ? 1 : 0));
@@ -1567,6 +1570,11 @@ int cfft_plan_set_option(cfft_plan* p, int32_t key, int64_t value) {
       case CFFT_OPT_PERSISTENT: p->persistent = value != 0; break;
       case CFFT_OPT_FUSE: p->fuse = value != 0; break;
       case CFFT_OPT_GRAPH: p->graph = value != 0; break;
+      case CFFT_OPT_TILE_ROWS:
+        require(value == 0 || ((value == 1 || value == 2 || value == 4 || value == 8) && p->H[1] % value == 0),
+                CFFT_ERR_INVALID, "tile rows must be 0 or 1, 2, 4, 8 dividing H1");
+        p->tile_rows = (int)value;
+        break;
       default: throw Error{CFFT_ERR_INVALID, "unknown option " + std::to_string(key)};
     }
   });

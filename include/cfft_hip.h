@@ -243,6 +243,11 @@ int cfft_sweep_hist_dev(cfft_plan* plan, void* hist, const void* const* E1, cons
  * factors, step range) and replay it on the caller's stream; at most 16
  * graphs are cached per plan.  Not used while profiling is enabled. */
 #define CFFT_OPT_GRAPH 3
+/* y rows per tile of the fold / unfold kernels (P1, P5, P51): 0 (default) =
+ * the largest power of two <= 8 dividing H1; otherwise 1, 2, 4 or 8 dividing
+ * H1 (CFFT_ERR_INVALID else).  Results do not depend on it; see
+ * cfft_hip.tune for timing the choices on a workload. */
+#define CFFT_OPT_TILE_ROWS 4
 /* No counterpart in the reference. */
 int cfft_plan_set_option(cfft_plan* plan, int32_t key, int64_t value);
 

@@ -97,3 +97,14 @@ def test_default_path_by_precision_and_tolerance():
     skew = lat + 0.3 * np.eye(3, k=1)
     assert default_path(N, ops_s, skew, cf.FP64) == cf.PATH_AUTO
     assert default_path(N, ops_s, skew, cf.MIXED, 1e-3) == cf.PATH_AUTO
+
+
+def test_schedule_candidates():
+    """Tuner candidates: tile heights 8, 4, 2 dividing H1 (1 when none does),
+    each with and without graph replay."""
+    import cfft_hip._lib as L
+    from cfft_hip.tune import schedule_candidates
+    rows = lambda H1: sorted({c[L.OPT_TILE_ROWS] for c in schedule_candidates(H1)})
+    assert rows(32) == [2, 4, 8] and rows(12) == [2, 4] and rows(6) == [2] and rows(3) == [1]
+    assert all({c[L.OPT_GRAPH] for c in schedule_candidates(32) if c[L.OPT_TILE_ROWS] == r} == {0, 1}
+               for r in (2, 4, 8))

@@ -363,6 +363,51 @@ def test_batched_grouped_equals_ungrouped():
     assert g == u
 
 
+@pytest.mark.parametrize("path", [cf.PATH_QS, cf.PATH_AUTO])
+def test_schedule_options_do_not_change_results(path):
+    """Every launch schedule the tuner may pick (tile rows 8, 4, 2; graph
+    replay on and off) gives the same sweep bit for bit; tile heights that are
+    not 1, 2, 4, 8 dividing H1 are refused."""
+    import torch
+    from cfft_hip.scft import PropagatorSweep
+    from cfft_hip.tune import schedule_candidates
+    N = (32, 32, 32)
+    ops, ops_s, shift, R, t = group_setup(229, N)
+    Lv = O.auto_L(t)
+    wA = 2.0 * O.extract_subgrid(O.make_symmetric(R, t, N, seed=21), Lv)
+    wB = -1.0 * O.extract_subgrid(O.make_symmetric(R, t, N, seed=22), Lv)
+    sw = PropagatorSweep(N, ops_s, 4.0 * np.eye(3), 0.3, 20, cf.FP64, path=path)
+    sw.set_fields(wA, wB)
+    ref = None
+    for opt in schedule_candidates(sw.plan.H[1]):
+        for k, v in opt.items():
+            sw.plan.set_option(k, v)
+        sw.reset()
+        sw.run()
+        torch.cuda.synchronize()
+        got = [x.copy() for x in sw.fields()]
+        if ref is None:
+            ref = got
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), opt
+    for bad in (3, 16, -1):
+        with pytest.raises(cf.CFFTError):
+            sw.plan.set_option(cf._lib.OPT_TILE_ROWS, bad)
+
+
+def test_tuned_batch_equals_untuned():
+    """solve_batch(tune=True) times each group's schedules, applies the
+    fastest and gives the untuned iterations and results."""
+    from cfft_hip import batch
+    insts = batch.draw_instances(5, seed=23, sizes=(16, 24), groups=(229, 47), ns=30)
+    t, _, _ = batch.solve_batch(insts, precision=cf.FP64, tol=1e-7, max_iter=150, workers=3, tune=True)
+    u, _, _ = batch.solve_batch(insts, precision=cf.FP64, tol=1e-7, max_iter=150, workers=3)
+    print("  schedules", sorted({r["schedule"] for r in t}), flush=True)
+    for a, b in zip(t, u):
+        assert a.pop("schedule").startswith("rows")
+        assert a == b
+
+
 def test_warm_start_pipeline_small():
     """End-to-end at 16^3 with a small network: generate converged solutions,
     train, predict, decode and solve from the prediction (fallback counted)."""
