@@ -308,14 +308,65 @@ __device__ void ph_fold_fftx(const PipeP& p, const PipeBufs& B, long item, int s
   __syncthreads();
 }
 
+// Row transforms of one or two planes straight from / to global memory.
+// Transform t < n_rows is row z = t mod NR of plane t / NR (p0 or p1, NR rows
+// of L entries, contiguous); its lanes hold x[r + R1 m] on entry, exchange
+// through the transform's own LDS row (stride L + 1, plane stride NR (L + 1))
+// and leave the result there (from_global) or store it (to_global).  Groups
+// past the last row work on row 0 and store nothing.
+template <typename T, typename TW, int L, int DIR, int NT, int NR = L>
+__device__ __forceinline__ void rows_fft_from_global(const cx<T>* __restrict__ p0, const cx<T>* __restrict__ p1,
+                                                     int n_rows, cx<T>* lds) {
+  constexpr int R1 = MLS<L>::R1, R2 = MLS<L>::R2, G = PipeG<L, NT>::G, rs = L + 1;
+  const int r = threadIdx.x % R1, g = threadIdx.x / R1;
+  for (int t0 = 0; t0 < n_rows; t0 += G) {
+    const int t = t0 + g;
+    const bool ok = t < n_rows;
+    const int tt = ok ? t : 0;
+    const int pl = tt / NR, z = tt - pl * NR;
+    const cx<T>* src = (pl ? p1 : p0) + z * L;
+    cx<T> v[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) v[m] = src[r + R1 * m];
+    if (ok) {
+      cx<T>* slot = lds + pl * (NR * rs) + z * rs;
+      fft_ml_lds<T, TW, L, DIR>(v, r, slot, 1);
+#pragma unroll
+      for (int q = 0; q < R2; ++q) slot[ml_out_pos<L>(r, q)] = v[q];
+    }
+  }
+}
+
+template <typename T, typename TW, int L, int DIR, int NT, int NR = L>
+__device__ __forceinline__ void rows_fft_to_global(cx<T>* lds, cx<T>* __restrict__ p0, cx<T>* __restrict__ p1,
+                                                   int n_rows) {
+  constexpr int R1 = MLS<L>::R1, R2 = MLS<L>::R2, G = PipeG<L, NT>::G, rs = L + 1;
+  const int r = threadIdx.x % R1, g = threadIdx.x / R1;
+  for (int t0 = 0; t0 < n_rows; t0 += G) {
+    const int t = t0 + g;
+    const bool ok = t < n_rows;
+    const int tt = ok ? t : 0;
+    const int pl = tt / NR, z = tt - pl * NR;
+    cx<T>* slot = lds + pl * (NR * rs) + z * rs;
+    cx<T> v[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) v[m] = slot[r + R1 * m];
+    if (ok) {
+      fft_ml_lds<T, TW, L, DIR>(v, r, slot, 1);
+      cx<T>* dst = (pl ? p1 : p0) + z * L;
+#pragma unroll
+      for (int q = 0; q < R2; ++q) dst[ml_out_pos<L>(r, q)] = v[q];
+    }
+  }
+}
+
 // ---- P2: FFT y then FFT z of one (b, c, x) plane ----------------------------
 template <typename T, typename TW, int L1, int L2>
 __device__ void ph_fft_yz(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds) {
   constexpr int rs = L1 + 1;
   cx<T>* plane = static_cast<cx<T>*>(B.ch) + item * ((long)L1 * L2);
-  plane_to_lds<T, L1, L2>(plane, lds);
+  rows_fft_from_global<T, TW, L1, -1, kNT, L2>(plane, plane, L2, lds);  // FFT y, rows from the plane
   __syncthreads();
-  lds_fft<T, TW, L1, -1>(lds, L2, rs, 1);
   lds_fft<T, TW, L2, -1>(lds, L1, 1, rs);
   lds_to_plane<T, L1, L2>(lds, plane);
   __syncthreads();
@@ -394,8 +445,10 @@ __device__ void ph_irecon_ifft_zy(const PipeP& p, const PipeBufs& B, long item, 
   }
   __syncthreads();
   lds_fft<T, TW, L2, +1>(lds, L1, 1, rs);
-  lds_fft<T, TW, L1, +1>(lds, L2, rs, 1);
-  lds_to_plane<T, L1, L2>(lds, static_cast<cx<T>*>(B.ch) + item * (long)NP);
+  {  // IFFT y, each row stored straight from its transform lanes
+    cx<T>* plane = static_cast<cx<T>*>(B.ch) + item * (long)NP;
+    rows_fft_to_global<T, TW, L1, +1, kNT, L2>(lds, plane, plane, L2);
+  }
   __syncthreads();
 }
 
@@ -423,64 +476,14 @@ __device__ void ph_expand_ifft_zy(const PipeP& p, const PipeBufs& B, long item, 
   }
   __syncthreads();
   lds_fft<T, TW, L2, +1>(lds, L1, 1, rs);
-  lds_fft<T, TW, L1, +1>(lds, L2, rs, 1);
-  lds_to_plane<T, L1, L2>(lds, static_cast<cx<T>*>(B.ch) + item * (long)NP);
+  {  // IFFT y, each row stored straight from its transform lanes
+    cx<T>* plane = static_cast<cx<T>*>(B.ch) + item * (long)NP;
+    rows_fft_to_global<T, TW, L1, +1, kNT, L2>(lds, plane, plane, L2);
+  }
   __syncthreads();
 }
 
 // ---- P2Q: FFT y, z + Q multiply + IFFT z, y on plane pairs (path QS) -------
-// Row transforms of a plane pair straight from / to global memory.  Transform
-// t < n_rows is row z = t mod L of plane t / L (p0 or p1, L x L, y fastest);
-// its lanes hold x[r + R1 m] on entry (a row is contiguous), exchange through
-// the transform's own LDS row (stride L + 1, plane stride L (L + 1)) and leave
-// the result there (from_global) or store it (to_global).  Groups past the
-// last row work on row 0 and store nothing.
-template <typename T, typename TW, int L, int DIR, int NT>
-__device__ __forceinline__ void rows_fft_from_global(const cx<T>* __restrict__ p0, const cx<T>* __restrict__ p1,
-                                                     int n_rows, cx<T>* lds) {
-  constexpr int R1 = MLS<L>::R1, R2 = MLS<L>::R2, G = PipeG<L, NT>::G, rs = L + 1;
-  const int r = threadIdx.x % R1, g = threadIdx.x / R1;
-  for (int t0 = 0; t0 < n_rows; t0 += G) {
-    const int t = t0 + g;
-    const bool ok = t < n_rows;
-    const int tt = ok ? t : 0;
-    const int pl = tt / L, z = tt - pl * L;
-    const cx<T>* src = (pl ? p1 : p0) + z * L;
-    cx<T> v[R2];
-#pragma unroll
-    for (int m = 0; m < R2; ++m) v[m] = src[r + R1 * m];
-    if (ok) {
-      cx<T>* slot = lds + pl * (L * rs) + z * rs;
-      fft_ml_lds<T, TW, L, DIR>(v, r, slot, 1);
-#pragma unroll
-      for (int q = 0; q < R2; ++q) slot[ml_out_pos<L>(r, q)] = v[q];
-    }
-  }
-}
-
-template <typename T, typename TW, int L, int DIR, int NT>
-__device__ __forceinline__ void rows_fft_to_global(cx<T>* lds, cx<T>* __restrict__ p0, cx<T>* __restrict__ p1,
-                                                   int n_rows) {
-  constexpr int R1 = MLS<L>::R1, R2 = MLS<L>::R2, G = PipeG<L, NT>::G, rs = L + 1;
-  const int r = threadIdx.x % R1, g = threadIdx.x / R1;
-  for (int t0 = 0; t0 < n_rows; t0 += G) {
-    const int t = t0 + g;
-    const bool ok = t < n_rows;
-    const int tt = ok ? t : 0;
-    const int pl = tt / L, z = tt - pl * L;
-    cx<T>* slot = lds + pl * (L * rs) + z * rs;
-    cx<T> v[R2];
-#pragma unroll
-    for (int m = 0; m < R2; ++m) v[m] = slot[r + R1 * m];
-    if (ok) {
-      fft_ml_lds<T, TW, L, DIR>(v, r, slot, 1);
-      cx<T>* dst = (pl ? p1 : p0) + z * L;
-#pragma unroll
-      for (int q = 0; q < R2; ++q) dst[ml_out_pos<L>(r, q)] = v[q];
-    }
-  }
-}
-
 // Lines of work unit u along an axis of length L whose points q = 2k + off map
 // to kbar = (L - k) mod L (off = 0) or L - 1 - k (off = 1): a pair (a, b) of
 // distinct lines (pair = true) or self-images (pair = false).  For even L and
@@ -678,8 +681,10 @@ __device__ void ph_ifft_zy(const PipeP& p, const PipeBufs& B, const cx<T>* src, 
   plane_to_lds<T, L1, L2>(src + item * ((long)L1 * L2), lds);
   __syncthreads();
   lds_fft<T, TW, L2, +1>(lds, L1, 1, rs);
-  lds_fft<T, TW, L1, +1>(lds, L2, rs, 1);
-  lds_to_plane<T, L1, L2>(lds, static_cast<cx<T>*>(B.ch) + item * ((long)L1 * L2));
+  {  // IFFT y, each row stored straight from its transform lanes
+    cx<T>* plane = static_cast<cx<T>*>(B.ch) + item * ((long)L1 * L2);
+    rows_fft_to_global<T, TW, L1, +1, kNT, L2>(lds, plane, plane, L2);
+  }
   __syncthreads();
 }
 
