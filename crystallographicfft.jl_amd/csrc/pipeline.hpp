@@ -66,12 +66,12 @@ struct PipeG {
 };
 
 // LDS row stride (in entries) of the transform tiles of the plane-pair and
-// unfold/fold kernels: L + 4 for 8-byte entries with L a multiple of 16 (the
-// rows and columns that neighbouring transform groups read then start on
-// different banks; measured faster), L + 1 otherwise (16-byte entries: no
-// gain measured).
+// unfold/fold kernels for L a multiple of 16: L + 4 for 8-byte entries, L + 2
+// for 16-byte entries (the rows and columns that neighbouring transform
+// groups read then start on different banks; each measured fastest among
+// L + 1, L + 2, L + 4); L + 1 otherwise.
 template <typename T, int L>
-constexpr int kRow = (sizeof(T) == 4 && L % 16 == 0) ? L + 4 : L + 1;
+constexpr int kRow = (L % 16 == 0) ? (sizeof(T) == 4 ? L + 4 : L + 2) : L + 1;
 
 // FFT of n1 * n2 length-LEN transforms held in LDS: element j of transform
 // t = t1 + n1 t2 is buf[t1 * ts1 + t2 * ts2 + j * es].  Every thread of the
