@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(256) k_mse(const float* __restrict__ Y, const 
   }
   red[threadIdx.x] = acc;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (unsigned s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }

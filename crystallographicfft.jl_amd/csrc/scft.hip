@@ -149,16 +149,16 @@ __global__ void __launch_bounds__(kRB) k_update_gram(const double* __restrict__ 
 template <int NV>
 __global__ void __launch_bounds__(kRB) k_total(const double* __restrict__ part, int nb, double* __restrict__ out) {
   __shared__ double red[kRB];
-  const int k = blockIdx.x;
+  const int k = blockIdx.x, tid = threadIdx.x;
   double a = 0.0;
-  for (int b = threadIdx.x; b < nb; b += kRB) a += part[b * NV + k];
-  red[threadIdx.x] = a;
+  for (int b = tid; b < nb; b += kRB) a += part[b * NV + k];
+  red[tid] = a;
   __syncthreads();
   for (int h = kRB / 2; h > 0; h >>= 1) {
-    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    if (tid < h) red[tid] += red[tid + h];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[k] = red[0];
+  if (tid == 0) out[k] = red[0];
 }
 
 struct MixP {
