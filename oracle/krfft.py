@@ -32,7 +32,7 @@ __all__ = [
     "count_special_positions_fast", "find_optimal_shift", "dual_R",
     "SpectralASU", "calc_spectral_asu", "get_k_vector", "auto_L",
     "select_rep_ops", "detect_centering_type", "strip_centering",
-    "alive_offsets", "lin_index", "make_symmetric", "fullgrid_reference",
+    "alive_offsets", "lin_index", "make_symmetric", "fullgrid_reference", "fullgrid_kernel",
     "extract_subgrid", "kernel_spec", "krfft_forward", "reverse_lookup",
     "m2_backward_table", "m2_backward", "m2_scft", "centering_fold",
     "centering_unfold", "fft_channels", "ifft_channels", "assemble_G0",
@@ -444,15 +444,19 @@ def _recip_B(lattice):
     return 2 * np.pi * np.linalg.inv(np.asarray(lattice, dtype=np.float64)).T
 
 
-def fullgrid_reference(u, N, ds, lattice, L):
-    """test/test_helpers.jl:56-73: stride-L subgrid of IFFT(K . FFT(u))."""
-    F = np.fft.fftn(u)
+def fullgrid_kernel(N, ds, lattice):
+    """K(h) = exp(-|2 pi A^-T h_c|^2 ds) on the full grid, h_c the centred
+    frequency (test/test_helpers.jl:56-73)."""
     B = _recip_B(lattice)
     grids = np.meshgrid(*[np.arange(n) for n in N], indexing="ij")
     hc = np.stack([np.where(g >= n // 2, g - n, g) for g, n in zip(grids, N)], axis=-1)
     kv = hc @ B.T
-    K = np.exp(-np.sum(kv * kv, axis=-1) * ds)
-    f = np.real(np.fft.ifftn(K * F))
+    return np.exp(-np.sum(kv * kv, axis=-1) * ds)
+
+
+def fullgrid_reference(u, N, ds, lattice, L):
+    """test/test_helpers.jl:56-73: stride-L subgrid of IFFT(K . FFT(u))."""
+    f = np.real(np.fft.ifftn(fullgrid_kernel(N, ds, lattice) * np.fft.fftn(u)))
     return extract_subgrid(f, L)
 
 

@@ -326,6 +326,41 @@ def test_full_size_im3m_128_properties():
     assert rel(p.diffuse(f0), p2.diffuse(f0)) < TOL64
 
 
+@pytest.mark.parametrize("path", [cf.PATH_M2, cf.PATH_QS])
+def test_config_pmmm_64(path):
+    """Pmmm (#47) 64^3, cell 4.0, ds 0.01 (the reference's CPU-runnable
+    configuration) at its full size: u ~ U(-1, 1) symmetrised; forward KRFFT
+    at the 32768 spectral-ASU points against the full-grid FFT and the backward
+    round trip (path M2; the plane-pair chain has no backward); then 100
+    Strang steps from q = 1 with w = 5 cos(2 pi x/a) cos(2 pi y/a)
+    cos(2 pi z/a) + N(0, 0.1) symmetrised, against the same steps on the full
+    grid (NumPy FFT, K, IFFT between the two half-step factors)."""
+    N = (64, 64, 64)
+    print(f"Pmmm 64^3 path={path}", flush=True)
+    ops, ops_s, shift, R, t = group_setup(47, N)
+    Lv = O.auto_L(t)
+    u = O.make_symmetric(R, t, N, seed=47, dist="uniform")
+    f0 = O.extract_subgrid(u, Lv)
+    plan = cf.DevicePlan(N, ops_s, DS, LAT4, path)
+    spec = O.calc_spectral_asu(R, t, N)
+    assert spec.idx.shape[0] == 32768
+    F = plan.forward(f0)
+    assert rel(F, np.fft.fftn(u)[tuple(spec.idx[:, d] for d in range(3))]) < TOL64
+    if path == cf.PATH_M2:
+        assert rel(plan.backward(F), f0) < TOL64
+    a = [2 * np.pi * (np.arange(n) + s * n) / n for n, s in zip(N, shift)]
+    X, Y, Z = np.meshgrid(*a, indexing="ij")
+    rng = np.random.default_rng(48)
+    w = O.symmetrize(5.0 * np.cos(X) * np.cos(Y) * np.cos(Z) + 0.1 * rng.standard_normal(N), R, t)
+    K = O.fullgrid_kernel(N, DS, LAT4)
+    E = np.exp(-w * DS / 2.0)
+    q = np.ones(N)
+    for _ in range(100):
+        q = E * np.real(np.fft.ifftn(K * np.fft.fftn(E * q)))
+    got = plan.step(np.ones(plan.M), O.extract_subgrid(w, Lv), DS, 100)
+    assert rel(got, O.extract_subgrid(q, Lv)) < TOL64
+
+
 @pytest.mark.parametrize("sg,N", [(229, (32, 32, 32)), (230, (32, 32, 32)), (70, (32, 32, 32))])
 def test_star_coefficients(sg, N):
     """Star coefficients: c_0 = cell mean; encode/decode of all stars is the
