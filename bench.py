@@ -408,6 +408,59 @@ def batched_instances(cf, n, seed, workers):
     }
 
 
+def config_pmmm_64(cf, torch):
+    """Pmmm (#47) 64^3, cell 4.0, ds 0.01, one field (the reference's
+    CPU-runnable configuration): device-resident forward and backward KRFFT
+    (us per call, 200 calls) and 100-step sweeps (field-steps/s, best of 5) on
+    each path that applies, FP64; w = 5 cos cos cos symmetrised (no noise)."""
+    from cfft_hip.scft import symmetrize_subgrid
+    n3 = (64, 64, 64)
+    ops = cf.get_ops(47, 3, n3)
+    shift, ops_s = cf.find_optimal_shift(ops, n3)
+    Lv = cf.auto_L(ops_s)
+    lat = CELL * np.eye(3)
+    a = [2 * np.pi * (np.arange(n) + sh * n) / n for n, sh in zip(n3, shift)]
+    X, Y, Z = np.meshgrid(*a, indexing="ij")
+    w = symmetrize_subgrid(5.0 * np.cos(X) * np.cos(Y) * np.cos(Z), ops_s, Lv)
+    s = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for name, path in (("m2", cf.PATH_M2), ("qs_chain", cf.PATH_QS)):
+        plan = cf.DevicePlan(n3, ops_s, 0.01, lat, path)
+        M3 = int(np.prod(plan.M))
+        wd = torch.from_numpy(np.asarray(w, dtype=np.float64).ravel(order="F")).cuda()
+        E = torch.exp(-wd * 0.005)
+        q = torch.ones(M3, dtype=torch.float64, device="cuda")
+        r = {}
+        F = torch.empty(2 * plan.n_spec, dtype=torch.float64, device="cuda")
+        for what, call in (("forward", lambda: plan.forward_dev(q.data_ptr(), F.data_ptr(), s)),
+                           ("backward", lambda: plan.backward_dev(F.data_ptr(), q.data_ptr(), s))):
+            if what == "backward" and path == cf.PATH_QS:
+                continue  # the plane-pair chain has no F_spec backward
+            call()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(200):
+                call()
+            torch.cuda.synchronize()
+            r[what + "_us"] = (time.perf_counter() - t0) / 200 * 1e6
+        plan.step_dev(q.data_ptr(), E.data_ptr(), 100, s)  # warm-up: captures the graph
+        torch.cuda.synchronize()
+        best = None
+        for _ in range(5):
+            q.fill_(1.0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            plan.step_dev(q.data_ptr(), E.data_ptr(), 100, s)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            best = el if best is None else min(best, el)
+        r["steps_per_s"] = 100 / best
+        r["us_per_step"] = best / 100 * 1e6
+        out[name] = r
+        del plan
+    return out
+
+
 def learned_warm_start(n_eval, workers):
     """Cold start vs the trained network's start on held-out parameter sets
     (Im-3m, Ia-3d, Fddd at the model's grid), residual 1e-8, FP64."""
@@ -567,6 +620,9 @@ def main():
             wide[name] = batched_fields(cf, torch, ops_s, wA, prec, e, args.wide, 50,
                                         STRATEGY_PATH[variants[name]["used"]])
 
+    # ---- the Pmmm 64^3 configuration (one field) ----
+    pmmm = config_pmmm_64(cf, torch)
+
     # ---- SCFT: Im-3m 128^3 plain cold start, Ia-3d 128^3 mixed vs FP64 ----
     scft = None
     if not args.no_scft:
@@ -643,6 +699,7 @@ def main():
         "step_alg_bytes_fp64": r64["step_bytes"],
         "cpu_baseline": cpu,
         "many_fields_per_launch": wide,
+        "config_pmmm_64": pmmm,
         "scft": scft,
     }
     print(json.dumps(line), flush=True)
