@@ -482,7 +482,8 @@ void solve(cfft_scft* c, double tol, int32_t max_iter, int32_t* iters, double* o
 cfft_scft* create(cfft_plan* plan, int32_t n_inst, const double* f, const double* chiN, int32_t ns,
                   int32_t m_hist) {
   require(plan && f && chiN, CFFT_ERR_INVALID, "null argument");
-  require(n_inst >= 1, CFFT_ERR_INVALID, "need at least one instance");
+  // the sweep takes per-field interaction factors for at most 64 fields
+  require(n_inst >= 1 && n_inst <= 32, CFFT_ERR_INVALID, "need 1 to 32 instances per context");
   require(ns >= 2, CFFT_ERR_INVALID, "need ns >= 2");
   require(m_hist >= 0 && m_hist < kMaxHist, CFFT_ERR_INVALID, "Anderson history must be < 32");
   std::lock_guard<std::recursive_mutex> guard(setup_mutex());

@@ -343,7 +343,7 @@ int cfft_scft_propagators(cfft_scft* ctx, void** hist_out);
 int cfft_scft_destroy(cfft_scft* ctx);
 
 /* Several melts in one context (no counterpart in the reference).  n_inst
- * instances share the grid, space group, cell and ns of a plan created with
+ * (1 to 32) instances share the grid, space group, cell and ns of a plan created with
  * batch = 2 n_inst, and one sweep advances q, q+ of all of them (fields 2i,
  * 2i + 1); each has its own f[i], chiN[i], fields, densities and Anderson
  * history, with the results of the one-instance context bit for bit.  The
