@@ -11,6 +11,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "twiddles.hpp"
 
 namespace cfft {
@@ -147,6 +149,37 @@ __device__ __forceinline__ void fft_reg(cx<T> (&a)[LEN]) {
   stockham<T, T, LEN, DIR, 1>(a);
 }
 
+// v[k] *= w^(r k), k = 1 .. R2 - 1, for the lane index r < R1 (the twiddles
+// between the two stages of the multi-lane FFT).  With FP64 twiddle
+// arithmetic the lane selects w^r once among R1 literals and forms the powers
+// by successive products (about k ulps from the literal; a literal select
+// per power costs several times the arithmetic it feeds); FP32 twiddles are
+// each selected among the literals.
+template <typename T, typename TW, int LEN, int DIR, int R1, int R2>
+__device__ __forceinline__ void lane_twiddles(cx<T> (&v)[R2], int r) {
+  if constexpr (std::is_same<TW, double>::value) {
+    cx<TW> wr = tw_c<LEN, TW>(0);
+#pragma unroll
+    for (int s = 1; s < R1; ++s)
+      if (r == s) wr = tw_c<LEN, TW>(s);
+    cx<TW> w = wr;
+#pragma unroll
+    for (int k = 1; k < R2; ++k) {
+      v[k] = twmul<T, TW, DIR>(v[k], w);
+      if (k + 1 < R2) w = cmul(w, wr);
+    }
+  } else {
+#pragma unroll
+    for (int k = 1; k < R2; ++k) {
+      cx<TW> w = tw_c<LEN, TW>(0);
+#pragma unroll
+      for (int s = 1; s < R1; ++s)
+        if (r == s) w = tw_c<LEN, TW>((s * k) % LEN);
+      v[k] = twmul<T, TW, DIR>(v[k], w);
+    }
+  }
+}
+
 // Split of a length-LEN transform over R1 cooperating lanes with R2 = LEN/R1
 // values each (R2 divisible by R1).
 template <int LEN>
@@ -166,15 +199,7 @@ __device__ __forceinline__ void fft_ml(cx<T> (&v)[MLS<LEN>::R2], int r, int g, c
   constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2;
   stockham<T, TW, R2, DIR, 1>(v);
   if constexpr (R1 > 1) {
-    // w^(r k) for the lane index r < R1: a select among R1 literals
-#pragma unroll
-    for (int k = 1; k < R2; ++k) {
-      cx<TW> w = tw_c<LEN, TW>(0);
-#pragma unroll
-      for (int s = 1; s < R1; ++s)
-        if (r == s) w = tw_c<LEN, TW>((s * k) % LEN);
-      v[k] = twmul<T, TW, DIR>(v[k], w);
-    }
+    lane_twiddles<T, TW, LEN, DIR, R1, R2>(v, r);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < R2; ++k) xch[(k * R1 + r) * GP + g] = v[k];
@@ -202,14 +227,7 @@ __device__ __forceinline__ void fft_ml_lds(cx<T> (&v)[MLS<LEN>::R2], int r, cx<T
   constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2;
   stockham<T, TW, R2, DIR, 1>(v);
   if constexpr (R1 > 1) {
-#pragma unroll
-    for (int k = 1; k < R2; ++k) {
-      cx<TW> w = tw_c<LEN, TW>(0);
-#pragma unroll
-      for (int s = 1; s < R1; ++s)
-        if (r == s) w = tw_c<LEN, TW>((s * k) % LEN);
-      v[k] = twmul<T, TW, DIR>(v[k], w);
-    }
+    lane_twiddles<T, TW, LEN, DIR, R1, R2>(v, r);
     // every lane of the group has its inputs in registers: the slots are free
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
