@@ -214,8 +214,9 @@ int cfft_profile(cfft_plan* plan, int32_t enable);
  * counts per stage since the last read; clears the record. */
 int cfft_profile_read(cfft_plan* plan, double* ms_out, int64_t* count_out);
 /* Average duration of one launch of a chain kernel (no counterpart in the
- * reference): stage CFFT_STAGE_UNFOLD_FOLD (P5 fused with P1),
- * CFFT_STAGE_FFT_Y (P2) or, on path QS, CFFT_STAGE_PLANE_Q (the plane-pair
+ * reference): stage CFFT_STAGE_UNFOLD_FOLD (P5 fused with P1, as between two
+ * steps of a plain sweep: q(s + 1) is not stored), CFFT_STAGE_FFT_Y (P2) or,
+ * on path QS, CFFT_STAGE_PLANE_Q (the plane-pair
  * kernel), launched once untimed and then reps times back to
  * back on stream between two hipEvents, on the device field q (batch * M^3
  * entries, overwritten) with interaction factors E[b] (device pointers).
