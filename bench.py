@@ -697,6 +697,16 @@ def main():
         "stage_ms_per_launch_events": {k: v[0] / v[1] for k, v in r64["stages"].items()},
         "launch_strategy": variants,
         "step_alg_bytes_fp64": r64["step_bytes"],
+        # the whole Strang step against the same roofline: the per-step bytes of
+        # the centred fwd+K+bwd formulation (one field), and the bytes the chain
+        # in use actually moves per field and step (its kernels' algorithmic
+        # bytes per launch over the fields per launch)
+        "step_roofline": {
+            "bytes_per_field_step": r64["step_bytes"],
+            "achieved_GBs": 2 * r64["step_bytes"] / (r64["el"] / K) / 1e9,
+            "frac": 2 * r64["step_bytes"] / (r64["el"] / K) / 1e9 / HBM_PEAK_GBS,
+            "chain_bytes_per_field_step": sum(v["bytes_per_launch"] for v in r64["chain"].values()) / 2,
+            "chain_achieved_GBs": sum(v["bytes_per_launch"] for v in r64["chain"].values()) / (r64["el"] / K) / 1e9},
         "cpu_baseline": cpu,
         "many_fields_per_launch": wide,
         "config_pmmm_64": pmmm,
