@@ -891,7 +891,9 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
     for (int e = 0; e < 8; ++e) {
       T r = (T)((TW)v[e] * p.scale_unfold);
       if (Eo) r *= eo[k][e];
-      if constexpr (STORE) fb[corner_index(p, x, y, z, e)] = r;
+      // a stored sweep's slice is read back only by the density pass, after
+      // the sweep: a streaming store keeps it out of the caches the chain uses
+      if constexpr (STORE) __builtin_nontemporal_store(r, fb + corner_index(p, x, y, z, e));
       v[e] = En ? r * en[k][e] : r;
     }
     wht8(v);
