@@ -102,6 +102,9 @@ struct GramSlots {
   int slot[kMaxHist];
 };
 constexpr int kNU = 7 + kMaxHist;  // partial sums per block of k_update_gram
+// NSL = gs.n (the ring slots in use), a compile-time count so that every
+// slot's loads are issued together; sums past NSL stay zero.
+template <int NSL>
 __global__ void __launch_bounds__(kRB) k_update_gram(const double* __restrict__ w, double* __restrict__ phi,
                                                      double* __restrict__ d, const double* __restrict__ qtot,
                                                      double ds, double chiN, long M3,
@@ -133,14 +136,26 @@ __global__ void __launch_bounds__(kRB) k_update_gram(const double* __restrict__ 
     acc[5] += wA * wA + wB * wB;
     acc[6] += pA;
 #pragma unroll
-    for (int k = 0; k < kMaxHist; ++k)
-      if (k < gs.n) {
-        const int sl = gs.slot[k];
-        acc[7 + k] += sl == gs.cur ? dA * dA + dB * dB
-                                   : dA * Dr[sl * n2 + x] + dB * Dr[sl * n2 + M3 + x];
-      }
+    for (int k = 0; k < NSL; ++k) {
+      const int sl = gs.slot[k];
+      acc[7 + k] += sl == gs.cur ? dA * dA + dB * dB : dA * Dr[sl * n2 + x] + dB * Dr[sl * n2 + M3 + x];
+    }
   }
   block_sum<kNU>(acc, part);
+}
+
+// k_update_gram<gs.n>, 1 <= gs.n <= kMaxHist.
+template <int N = 1>
+void launch_update_gram(dim3 grid, hipStream_t st, const double* w, double* phi, double* d, const double* qtot,
+                        double ds, double chiN, long M3, double* Wr, double* Dr, const GramSlots& gs,
+                        double* part) {
+  if constexpr (N > kMaxHist) {
+    throw Error{CFFT_ERR_INVALID, "Anderson history slots out of range"};
+  } else if (gs.n == N) {
+    hipLaunchKernelGGL(k_update_gram<N>, grid, dim3(kRB), 0, st, w, phi, d, qtot, ds, chiN, M3, Wr, Dr, gs, part);
+  } else {
+    launch_update_gram<N + 1>(grid, st, w, phi, d, qtot, ds, chiN, M3, Wr, Dr, gs, part);
+  }
 }
 
 // Totals of NV partial sums over nb <= kMaxRed blocks: workgroup k sums
@@ -311,10 +326,9 @@ void update_inst(cfft_scft* c, int i) {
   gs.n = (int)s.order.size();
   gs.cur = slot;
   for (int k = 0; k < gs.n; ++k) gs.slot[k] = s.order[k];
-  hipLaunchKernelGGL(k_update_gram, dim3(nb), dim3(kRB), 0, c->stream, static_cast<const double*>(s.w.p),
-                     static_cast<double*>(s.phi.p), static_cast<double*>(s.d.p), static_cast<const double*>(tot),
-                     c->ds, s.chiN, M3, static_cast<double*>(s.Wr.p), static_cast<double*>(s.Dr.p), gs,
-                     static_cast<double*>(c->part.p));
+  launch_update_gram(dim3(nb), c->stream, static_cast<const double*>(s.w.p), static_cast<double*>(s.phi.p),
+                     static_cast<double*>(s.d.p), static_cast<const double*>(tot), c->ds, s.chiN, M3,
+                     static_cast<double*>(s.Wr.p), static_cast<double*>(s.Dr.p), gs, static_cast<double*>(c->part.p));
   hip_check(hipGetLastError(), "k_update_gram");
   hipLaunchKernelGGL(k_total<kNU>, dim3(kNU), dim3(kRB), 0, c->stream, static_cast<const double*>(c->part.p),
                      (int)nb, tot + 2);
