@@ -33,14 +33,15 @@ def field(sg, N, seed=1):
     return ops_s, R, t, u, Lv
 
 
-@pytest.mark.parametrize("sg", PRIMITIVE + CENTRED)
-@pytest.mark.parametrize("path", [cf.PATH_M2, cf.PATH_M7])
+# M7 folds a centred lattice on the L = (2,2,2) subgrid: run it on the centred groups only.
+@pytest.mark.parametrize("path,sg", [(cf.PATH_M2, g) for g in PRIMITIVE + CENTRED]
+                         + [(cf.PATH_M7, g) for g in CENTRED])
 def test_forward_backward_diffuse_fp64(sg, path):
     N = (16, 16, 16)
     ops_s, R, t, u, Lv = field(sg, N)
     cent = O.detect_centering_type(R, t, N)
-    if path == cf.PATH_M7 and (cent == O.CENT_P or not np.all(Lv == 2)):
-        pytest.skip("M7 needs a centred lattice with L = (2,2,2)")
+    if path == cf.PATH_M7:
+        assert cent != O.CENT_P and np.all(Lv == 2), sg
     print(f"sg={sg} N={N} path={path}", flush=True)
     plan = cf.DevicePlan(N, ops_s, DS, LAT4, path)
     spec = O.calc_spectral_asu(R, t, N)
@@ -65,14 +66,14 @@ def test_forward_backward_diffuse_fp64(sg, path):
         assert rel(G0, O.assemble_G0(O.fft_channels(O.centering_fold(f0, cent)), cent, f0.shape)) < TOL64
 
 
-@pytest.mark.parametrize("sg", [47, 221, 229, 230, 70, 10])
-@pytest.mark.parametrize("path", [cf.PATH_M2Q, cf.PATH_M7Q])
+@pytest.mark.parametrize("path,sg", [(cf.PATH_M2Q, g) for g in (47, 221, 229, 230, 70, 10)]
+                         + [(cf.PATH_M7Q, g) for g in (229, 230, 70)])
 def test_q_fused_fp64(sg, path):
     N = (16, 16, 16)
     ops_s, R, t, u, Lv = field(sg, N)
     cent = O.detect_centering_type(R, t, N)
-    if path == cf.PATH_M7Q and cent == O.CENT_P:
-        pytest.skip("M7Q needs a centred lattice")
+    if path == cf.PATH_M7Q:
+        assert cent != O.CENT_P, sg
     plan = cf.DevicePlan(N, ops_s, DS, LAT4, path)
     f0 = O.extract_subgrid(u, Lv)
     qp = O.m2q_plan(R, t, N, DS, LAT4)
