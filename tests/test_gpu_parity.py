@@ -386,3 +386,34 @@ def test_star_coefficients(sg, N):
     K2 = np.sum(kv * kv, axis=-1)
     ref = np.real(np.fft.ifftn(np.where(K2 <= k2[n - 1] + 1e-9, Fu, 0.0)))
     assert rel(approx, O.extract_subgrid(ref, Lv)) < TOL64
+
+
+def test_mixed_chain_paired_tiles_im3m_64():
+    """Plane-pair chain at Im-3m 64^3 (H = 16: two y-blocks of 8 rows per
+    (b, z) and 32 unfold/fold workgroups per field), where the unfold/fold
+    kernel runs the two y-blocks sharing the 128-byte lines of FP32 channel
+    rows as blocks i and i + 8 of each 16.  Four Strang steps: mixed precision
+    within 2e-5 of FP64 and bit-identical across tile heights 8, 4 and 2 (a
+    different pairing each); FP64 within 1e-12 of the oracle's steps."""
+    N = (64, 64, 64)
+    print("Im-3m 64^3 mixed plane-pair chain, tile heights 8/4/2", flush=True)
+    ops, ops_s, shift, R, t = group_setup(229, N)
+    Lv = O.auto_L(t)
+    q = O.extract_subgrid(O.make_symmetric(R, t, N, seed=5), Lv) + 1.0
+    w = 2.0 * O.extract_subgrid(O.make_symmetric(R, t, N, seed=6), Lv)
+    p64 = cf.DevicePlan(N, ops_s, DS, LAT4, cf.PATH_QS)
+    assert p64.H[1] == 16
+    ref = p64.step(q, w, DS, nsteps=4)
+    qp = O.m2q_plan(R, t, N, DS, LAT4)
+    orb = O.subgrid_orbits(R, t, N, Lv)
+    o = q.copy()
+    for _ in range(4):
+        o = O.strang_step(o, w, DS, lambda x: O.qr_scft(x, qp, orb))
+    assert rel(ref, o) < TOL64
+    outs = []
+    for rows in (8, 4, 2):
+        p32 = cf.DevicePlan(N, ops_s, DS, LAT4, cf.PATH_QS, cf.MIXED)
+        p32.set_option(cf._lib.OPT_TILE_ROWS, rows)
+        outs.append(p32.step(q, w, DS, nsteps=4))
+        assert rel(outs[-1], ref) < TOL32, rows
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
