@@ -86,15 +86,10 @@ def step_bytes(plan, e):
     return 4 * M3 * e + 8 * plan.n_ch * int(np.prod(plan.H)) * 2 * e + 2 * plan.n_spec * 2 * e
 
 
-def time_sweeps(sw, steps, warmup, torch):
-    """Wall time of `steps` contour steps of the batch (a sweep restarts from
-    q = 1 every Ns steps), after `warmup` untimed full sweeps."""
-    for _ in range(warmup):
-        sw.reset()
-        sw.run()
+def sweep_calls(sw, steps):
+    """The library calls of `steps` contour steps of the batch: a sweep
+    restarts from q = 1 every Ns steps."""
     sw.reset()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
     done = 0
     while done < steps:
         if done > 0 and done % NS == 0:
@@ -102,6 +97,18 @@ def time_sweeps(sw, steps, warmup, torch):
         n = min(NS - done % NS, steps - done)
         sw.run(n, start=done % NS)
         done += n
+
+
+def time_sweeps(sw, steps, warmup, torch):
+    """Wall time of `steps` contour steps of the batch, after `warmup` untimed
+    repetitions of exactly the calls that are timed (same step counts and
+    start steps), so that every hipGraph the timed calls replay is captured
+    before the timed region."""
+    for _ in range(max(1, warmup)):
+        sweep_calls(sw, steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sweep_calls(sw, steps)
     torch.cuda.synchronize()
     return time.perf_counter() - t0
 
@@ -138,14 +145,21 @@ def profile_stages(sw, steps, torch):
 
 
 STRATEGY_PATH = {"m7_chain": 0, "m7_persistent": 0, "qr_chain": 5, "qs_chain": 6, "qs_chain_eager": 6}
+# the FETCH_SIZE / WRITE_SIZE record of each pipeline path's chain kernels
+# (eager launches and graph replays run the same kernels)
+TRAFFIC_RECORD = {0: "m7_chain", 5: "qr_chain", 6: "qs_chain"}
 
 
 def profiled_traffic(strategy, stage_name, prec_name):
     """HBM bytes per launch of a chain kernel at this workload (2 fields per
     launch), from the FETCH_SIZE / WRITE_SIZE profiler runs recorded under
-    profiles/ for that launch strategy (each counter scaled by what it reads
-    for a plain 64 MiB copy); None when no such record exists."""
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_im3m128_{strategy}.json")
+    profiles/ for the pipeline path of that launch strategy (each counter
+    scaled by what it reads for a plain 64 MiB copy); None when no such
+    record exists."""
+    name = TRAFFIC_RECORD.get(STRATEGY_PATH.get(strategy))
+    if name is None:
+        return None
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_im3m128_{name}.json")
     try:
         with open(path) as fh:
             rec = json.load(fh)["kernels"].get(f"{stage_name}/{prec_name}")
@@ -182,12 +196,18 @@ def live_launch_ms(plan, stage_name, q, E_ptrs, stream, reps=100, trials=3):
 
 
 def cpu_threads():
-    """Host threads for the multi-core CPU leg: the affinity set, at most 16."""
+    """Host threads for the multi-core CPU leg: every core this process may
+    use, i.e. the affinity set, bounded by OMP_NUM_THREADS when it is set (the
+    CPU share that the machine gives one GPU's job)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    try:
+        n = min(n, int(os.environ["OMP_NUM_THREADS"]))
+    except (KeyError, ValueError):
+        pass
+    return max(1, n)
 
 
 def cpu_baseline(ops_s, wA, Lv, budget_s):
@@ -484,38 +504,24 @@ def cpu_scft_reference(ops_s, wA, wB):
     """Q, F and densities of the first iteration on the CPU: the C restatement
     of the Strang step for the sweeps, Simpson densities and F from the SCFT
     oracle (FP64)."""
-    import oracle as O
     from oracle import cstep
     import cfft_hip as cf
     R, t = cf.ops_arrays(ops_s)
     if not os.path.exists(cstep.LIB):
         cstep.build()
-    ds = 1.0 / NS
-    cs = cstep.CStep(R, t, N, ds, CELL * np.eye(3), nthreads=cpu_threads())
-    EA = np.asfortranarray(np.exp(-wA * ds / 2.0))
-    EB = np.asfortranarray(np.exp(-wB * ds / 2.0))
-    nA = int(round(F_A * NS))
-    q = [np.ones(cs.M, order="F")]
-    for j in range(NS):
-        q.append(cs.step(q[-1], EA if j < nA else EB))
-    qd = np.ones(cs.M, order="F")
-    wa, wb = O.contour_weights(nA), O.contour_weights(NS - nA)
-    phiA = np.zeros(cs.M)
-    phiB = np.zeros(cs.M)
-    for j in range(NS + 1):
-        # qd holds q+(j'); the density pairs q(ns - j') with q+(j')
-        k = NS - j
-        if k <= nA:
-            phiA += wa[k] * q[k] * qd
-        if k >= nA:
-            phiB += wb[k - nA] * q[k] * qd
-        if j < NS:
-            qd = cs.step(qd, EB if j < NS - nA else EA)
-    Q = float(np.mean(q[NS]))
-    phiA *= ds / Q
-    phiB *= ds / Q
-    F = O.free_energy(wA, wB, phiA, phiB, Q, CHIN)
-    return Q, F, phiA, phiB
+    cs = cstep.CStep(R, t, N, 1.0 / NS, CELL * np.eye(3), nthreads=cpu_threads())
+    r = cstep.first_iteration(cs, wA, wB, F_A, CHIN, NS)
+    return r["Q"], r["F"], r["phiA"], r["phiB"]
+
+
+def step_roofline(r, K):
+    """Whole-step bandwidth of a timed run: the chain kernels' algorithmic
+    bytes per contour step (2 fields) over the wall time per contour step."""
+    chain = sum(v["bytes_per_launch"] for v in r["chain"].values())
+    t = r["el"] / K
+    return {"bytes_per_field_step": chain / 2, "achieved_GBs": chain / t / 1e9,
+            "frac": chain / t / 1e9 / HBM_PEAK_GBS,
+            "centred_formulation_bytes_per_field_step": r["step_bytes"]}
 
 
 def main():
@@ -569,14 +575,14 @@ def main():
             sw.set_fields(wA, wB)
             sw.plan.set_option(cf._lib.OPT_PERSISTENT, persist)
             sw.plan.set_option(cf._lib.OPT_GRAPH, graph)
-            trial[label] = time_sweeps(sw, min(args.steps, NS), 1, torch)
+            trial[label] = time_sweeps(sw, args.steps, 1, torch)
             sweeps[label] = sw
         best = min(trial, key=trial.get)
         sw = sweeps[best]
         for label in list(sweeps):
             if label != best:
                 del sweeps[label]
-        variants[name] = {k + "_ms_per_step": v / min(args.steps, NS) * 1e3 for k, v in trial.items()}
+        variants[name] = {k + "_ms_per_step": v / args.steps * 1e3 for k, v in trial.items()}
         variants[name]["used"] = best
         # full sweep once for Q
         sw.reset()
@@ -697,16 +703,12 @@ def main():
         "stage_ms_per_launch_events": {k: v[0] / v[1] for k, v in r64["stages"].items()},
         "launch_strategy": variants,
         "step_alg_bytes_fp64": r64["step_bytes"],
-        # the whole Strang step against the same roofline: the per-step bytes of
-        # the centred fwd+K+bwd formulation (one field), and the bytes the chain
-        # in use actually moves per field and step (its kernels' algorithmic
-        # bytes per launch over the fields per launch)
-        "step_roofline": {
-            "bytes_per_field_step": r64["step_bytes"],
-            "achieved_GBs": 2 * r64["step_bytes"] / (r64["el"] / K) / 1e9,
-            "frac": 2 * r64["step_bytes"] / (r64["el"] / K) / 1e9 / HBM_PEAK_GBS,
-            "chain_bytes_per_field_step": sum(v["bytes_per_launch"] for v in r64["chain"].values()) / 2,
-            "chain_achieved_GBs": sum(v["bytes_per_launch"] for v in r64["chain"].values()) / (r64["el"] / K) / 1e9},
+        # the whole Strang step against the same roofline, with the bytes the
+        # chain in use moves per field and step (its kernels' algorithmic
+        # bytes per launch over the fields per launch); the per-step bytes of
+        # the centred fwd+K+bwd formulation beside it for comparison
+        "step_roofline": step_roofline(r64, K),
+        "step_roofline_mixed": step_roofline(r32, K),
         "cpu_baseline": cpu,
         "many_fields_per_launch": wide,
         "config_pmmm_64": pmmm,

@@ -71,6 +71,7 @@ struct cfft_plan {
   };
   std::vector<Mark> marks;
   ~cfft_plan() {
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     for (auto& m : marks) {
@@ -98,10 +99,14 @@ bool chain_path(const cfft_plan* p) { return !is_q_path(p) || is_qr(p) || is_qs(
 bool fp32_arith(const cfft_plan* p) { return p->prec == CFFT_MIXED && p->path == CFFT_PATH_QS; }
 
 
+// Uploads hold the set-up lock (allocation and a blocking copy); the host
+// work of plan building between them runs unlocked, so several threads can
+// build plans at once.
 template <typename T>
 void upload_complex(DevBuf& buf, const std::vector<double>& re, const std::vector<double>& im) {
   std::vector<cx<T>> h(re.size());
   for (size_t i = 0; i < re.size(); ++i) h[i] = {(T)re[i], (T)im[i]};
+  std::lock_guard<std::recursive_mutex> guard(setup_mutex());
   buf.alloc(h.size() * sizeof(cx<T>));
   if (!h.empty()) hip_check(hipMemcpy(buf.p, h.data(), buf.bytes, hipMemcpyHostToDevice), "upload");
 }
@@ -109,11 +114,13 @@ void upload_complex(DevBuf& buf, const std::vector<double>& re, const std::vecto
 template <typename T>
 void upload_real(DevBuf& buf, const std::vector<double>& v) {
   std::vector<T> h(v.begin(), v.end());
+  std::lock_guard<std::recursive_mutex> guard(setup_mutex());
   buf.alloc(h.size() * sizeof(T));
   if (!h.empty()) hip_check(hipMemcpy(buf.p, h.data(), buf.bytes, hipMemcpyHostToDevice), "upload");
 }
 
 void upload_bytes(DevBuf& buf, const void* src, size_t n) {
+  std::lock_guard<std::recursive_mutex> guard(setup_mutex());
   buf.alloc(n);
   if (n) hip_check(hipMemcpy(buf.p, src, n, hipMemcpyHostToDevice), "upload");
 }
@@ -125,11 +132,16 @@ void upload_bytes(DevBuf& buf, const void* src, size_t n) {
 // into F_0.  Their product over d is Q_s(q) of the closed form used by the
 // other Q paths (first row of B^-1 diag(K) B, src/q_fused_krfft.jl:182-413)
 // when the cell is orthogonal and every R_s is diagonal.
-void build_qs_table(cfft_plan* p) {
-  const double* A = p->lattice;  // column-major 3 x 3
+bool orthogonal_cell(const double* A) {  // column-major 3 x 3
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j)
-      require(i == j || A[i + 3 * j] == 0.0, CFFT_ERR_UNSUPPORTED, "path QS needs an orthogonal cell");
+      if (i != j && A[i + 3 * j] != 0.0) return false;
+  return true;
+}
+
+void build_qs_table(cfft_plan* p) {
+  const double* A = p->lattice;  // column-major 3 x 3
+  require(orthogonal_cell(A), CFFT_ERR_UNSUPPORTED, "path QS needs an orthogonal cell");
   int tsel[8][3];
   bool seen[8] = {false, false, false, false, false, false, false, false};
   for (int b = 0; b < p->d; ++b) {
@@ -180,6 +192,7 @@ void build_qs_table(cfft_plan* p) {
     // same allocation: captured chains keep pointing at it
     std::vector<cx<double>> h(re.size());
     for (size_t i = 0; i < re.size(); ++i) h[i] = {re[i], im[i]};
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
     hip_check(hipMemcpy(p->d_qsep.p, h.data(), p->d_qsep.bytes, hipMemcpyHostToDevice), "upload QS factors");
   }
   p->qs_items = 0;
@@ -274,7 +287,10 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
   require(p->n_spec < 0xFFFFF, CFFT_ERR_UNSUPPORTED, "spectral ASU too large");
   p->K = kernel_spec(p->spec, ds, lattice);
 
-  hip_check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
+  {
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
+    hip_check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
+  }
   build_tables<double>(p);  // twiddles, phases and K stay FP64 in both precisions
 
   // signed spectral frequencies (get_k_vector)
@@ -910,7 +926,10 @@ bool run_sweep_persistent(cfft_plan* p, T* q, const PipeP& pp, int step0, int ns
   return true;
 }
 
-hipStream_t pick(cfft_plan* p, void* s) { return s ? static_cast<hipStream_t>(s) : p->stream; }
+// The stream a *_dev call runs on: exactly the caller's, with NULL the null
+// stream (the default stream of HIP and of PyTorch), so the call is ordered
+// after the caller's earlier work on that stream and before its later work.
+hipStream_t pick(cfft_plan*, void* s) { return static_cast<hipStream_t>(s); }
 
 template <typename T>
 void to_dev(cfft_plan* p, DevBuf& dst, const double* src, size_t n) {
@@ -1283,7 +1302,8 @@ int cfft_plan_create(const int32_t* N, const int32_t* ops, int32_t nops, double 
   return guarded([&] {
     require(N && ops && lattice && out && nops > 0, CFFT_ERR_INVALID, "null argument");
     *out = nullptr;
-    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
+    // host table building runs unlocked; allocations and uploads take the
+    // set-up lock one at a time
     std::unique_ptr<cfft_plan> p(new cfft_plan());
     int n[3] = {N[0], N[1], N[2]};
     build_plan(p.get(), n, unpack_ops(ops, nops), ds, lattice, path, precision, batch);
@@ -1323,14 +1343,19 @@ int cfft_update_kernel(cfft_plan* p, double ds, const double* lattice) {
     require(!is_q_path(p) || is_qs(p), CFFT_ERR_INVALID,
             "update_kernel applies to the fwd+K+bwd paths and path QS; replan the other Q-fused paths");
     const double* lat = lattice ? lattice : p->lattice;
-    p->K = kernel_spec(p->spec, ds, lat);
-    p->ds = ds;
-    if (lattice) std::memcpy(p->lattice, lattice, sizeof(double) * 9);
+    // every check before any plan state changes: a rejected update leaves the
+    // plan as it was
+    if (is_qs(p))
+      require(orthogonal_cell(lat), CFFT_ERR_UNSUPPORTED, "path QS needs an orthogonal cell");
+    std::vector<double> K = kernel_spec(p->spec, ds, lat);
+    require(p->d_K.bytes == K.size() * sizeof(double), CFFT_ERR_INVALID, "kernel table size changed");
     // work of this plan may be in flight on any stream; the new values go into
     // the same allocation, so captured chains keep pointing at valid memory
     std::lock_guard<std::recursive_mutex> guard(setup_mutex());
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-    require(p->d_K.bytes == p->K.size() * sizeof(double), CFFT_ERR_INVALID, "kernel table size changed");
+    p->K = std::move(K);
+    p->ds = ds;
+    if (lattice) std::memcpy(p->lattice, lattice, sizeof(double) * 9);
     if (!p->K.empty())
       hip_check(hipMemcpy(p->d_K.p, p->K.data(), p->d_K.bytes, hipMemcpyHostToDevice), "upload K");
     if (is_qs(p)) build_qs_table(p);
@@ -1624,6 +1649,7 @@ int cfft_sync(cfft_plan* p) {
   return guarded([&] {
     check_plan(p);
     hip_check(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+    hip_check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
   });
 }
 

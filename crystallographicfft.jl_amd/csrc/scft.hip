@@ -227,6 +227,7 @@ struct ScftInst {
   std::vector<double> G;
   int next_slot = 0;
   int iters = 0;
+  bool failed = false;  // stopped by a failure of its own (see iterate)
   double last[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
@@ -413,19 +414,30 @@ void iterate(cfft_scft* c, const std::vector<char>& active, double* out) {
   hip_check(hipMemcpyAsync(c->tot_h, c->tot.p, sizeof(double) * kTot * ni, hipMemcpyDeviceToHost, c->stream),
             "D2H totals");
   hip_check(hipStreamSynchronize(c->stream), "sync");
-  // every active instance is finished; the first failure is reported after
-  bool failed = false;
+  // every active instance is finished.  An instance whose iteration fails
+  // (partition function not positive and finite) is marked failed, its
+  // out[] reads {NaN, NaN, NaN, +inf, NaN, NaN, 0, 0} and the others go on;
+  // the call fails only when every instance it iterated failed.
+  int n_act = 0, n_failed = 0;
   Error first{CFFT_OK, ""};
   for (int i = 0; i < ni; ++i) {
     if (!active[i]) continue;
+    ++n_act;
     try {
       finish_inst(c, i, out ? out + 8 * i : nullptr);
     } catch (const Error& e) {
-      if (!failed) first = e;
-      failed = true;
+      if (n_failed == 0) first = e;
+      ++n_failed;
+      ScftInst& s = *c->in[i];
+      s.failed = true;
+      const double nan = std::numeric_limits<double>::quiet_NaN();
+      const double o[8] = {nan, nan, nan, std::numeric_limits<double>::infinity(), nan, nan, 0.0, 0.0};
+      std::memcpy(s.last, o, sizeof(o));
+      if (out) std::memcpy(out + 8 * i, o, sizeof(o));
     }
   }
-  if (failed) throw first;
+  if (n_failed > 0) g_last_error = first.msg;
+  if (n_act > 0 && n_failed == n_act) throw first;
 }
 
 ScftInst& instance(cfft_scft* c, int32_t i) {
@@ -440,6 +452,7 @@ void set_fields_inst(cfft_scft* c, ScftInst& s, const double* wA, const double* 
   hip_check(hipMemcpyAsync(w, wA, 8 * c->M3, hipMemcpyHostToDevice, c->stream), "H2D");
   hip_check(hipMemcpyAsync(w + c->M3, wB, 8 * c->M3, hipMemcpyHostToDevice, c->stream), "H2D");
   hip_check(hipStreamSynchronize(c->stream), "sync");
+  s.failed = false;
   s.order.clear();
   s.next_slot = 0;
   std::fill(s.G.begin(), s.G.end(), 0.0);
@@ -473,7 +486,8 @@ void set_params_inst(cfft_scft* c, ScftInst& s, double f, double chiN) {
 // as it meets either.
 void solve(cfft_scft* c, double tol, int32_t max_iter, int32_t* iters, double* out) {
   const int ni = (int)c->in.size();
-  std::vector<char> active(ni, max_iter > 0 ? 1 : 0);
+  std::vector<char> active(ni, 0);
+  for (int i = 0; i < ni; ++i) active[i] = max_iter > 0 && !c->in[i]->failed;
   std::vector<int> it(ni, 0);
   std::vector<double> o(8 * ni, 0.0);
   for (;;) {
@@ -484,7 +498,7 @@ void solve(cfft_scft* c, double tol, int32_t max_iter, int32_t* iters, double* o
     for (int i = 0; i < ni; ++i) {
       if (!active[i]) continue;
       ++it[i];
-      if (o[8 * i + 3] < tol || it[i] >= max_iter) active[i] = 0;
+      if (c->in[i]->failed || o[8 * i + 3] < tol || it[i] >= max_iter) active[i] = 0;
     }
   }
   for (int i = 0; i < ni; ++i) {
@@ -607,7 +621,12 @@ int cfft_scft_iterate(cfft_scft* c, double* out) {
 int cfft_scft_iterate_multi(cfft_scft* c, double* out) {
   return guarded([&] {
     require(c != nullptr, CFFT_ERR_INVALID, "null context");
-    iterate(c, std::vector<char>(c->in.size(), 1), out);
+    // instances that failed earlier stay stopped until their fields are set again
+    std::vector<char> active(c->in.size(), 0);
+    bool any = false;
+    for (size_t i = 0; i < c->in.size(); ++i) any = (active[i] = !c->in[i]->failed) || any;
+    require(any, CFFT_ERR_INVALID, "every instance of the context has failed; set new fields");
+    iterate(c, active, out);
   });
 }
 

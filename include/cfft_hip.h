@@ -20,8 +20,10 @@
  *     SpectralIndexing.points.
  *   - Host-pointer entry points are blocking and use FP64 at the boundary.
  *     *_dev entry points take device pointers of the plan's precision (double
- *     for CFFT_FP64, float for CFFT_MIXED) and an optional hipStream_t
- *     (NULL = the plan's own stream); they return without synchronising.
+ *     for CFFT_FP64, float for CFFT_MIXED) and a hipStream_t; the work runs
+ *     on exactly that stream (NULL = the null stream, which is PyTorch's
+ *     default stream), in order with the caller's other work on it, and the
+ *     call returns without synchronising.
  *   - A plan is not thread-safe; use one plan per stream.
  */
 #ifndef CFFT_HIP_H
@@ -133,7 +135,9 @@ int cfft_plan_tables(const cfft_plan* plan, int32_t* idx_out, int32_t* mult_out,
                      int32_t* rep_out);
 
 /* update_m2_kernel! / update_kernel! (src/q_fused_krfft.jl:1036-1052,
- * src/centering_fold.jl:939-953): recompute K for a new ds (and lattice). */
+ * src/centering_fold.jl:939-953): recompute K for a new ds (and lattice;
+ * NULL keeps the plan's).  A rejected update (a non-orthogonal cell on path
+ * QS) leaves the plan unchanged. */
 int cfft_update_kernel(cfft_plan* plan, double ds, const double* lattice);
 
 int cfft_plan_destroy(cfft_plan* plan);
@@ -252,7 +256,7 @@ int cfft_sweep_hist_dev(cfft_plan* plan, void* hist, const void* const* E1, cons
 /* No counterpart in the reference. */
 int cfft_plan_set_option(cfft_plan* plan, int32_t key, int64_t value);
 
-/* Synchronise the plan's stream. */
+/* Wait for the plan's own stream (host-pointer calls) and the null stream. */
 int cfft_sync(cfft_plan* plan);
 
 /* ---- real-space ASU and block CFFT ---------------------------------------- */
@@ -354,11 +358,17 @@ int cfft_scft_create_multi(cfft_plan* plan, int32_t n_inst, const double* f, con
                            int32_t ns, int32_t m_hist, cfft_scft** out);
 int cfft_scft_set_params_inst(cfft_scft* ctx, int32_t inst, double f, double chiN);
 int cfft_scft_set_fields_inst(cfft_scft* ctx, int32_t inst, const double* wA, const double* wB);
-/* One iteration of every instance; out[8 n_inst], instance i at out + 8 i. */
+/* One iteration of every instance; out[8 n_inst], instance i at out + 8 i.
+ * An instance whose partition function comes out non-positive or non-finite
+ * is marked failed: its out[] reads {NaN, NaN, NaN, +inf, NaN, NaN, 0, 0},
+ * the other instances go on, and it is skipped until its fields are set
+ * again.  The call fails (CFFT_ERR_INVALID, message of the first failure)
+ * only when every instance it iterated failed. */
 int cfft_scft_iterate_multi(cfft_scft* ctx, double* out);
 /* Iterate until each instance has residual < tol or max_iter iterations; an
- * instance that meets either stops iterating while the others go on (its
- * fields stay in the sweep).  iters_out[n_inst], out[8 n_inst]. */
+ * instance that meets either, or fails as in cfft_scft_iterate_multi, stops
+ * iterating while the others go on (its fields stay in the sweep).
+ * iters_out[n_inst], out[8 n_inst]. */
 int cfft_scft_solve_multi(cfft_scft* ctx, double tol, int32_t max_iter, int32_t* iters_out, double* out);
 int cfft_scft_get_inst(cfft_scft* ctx, int32_t inst, int32_t which, double* out);
 

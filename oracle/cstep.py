@@ -155,3 +155,37 @@ class CStep:
         Ep = None if E is None else np.asfortranarray(E, dtype=np.float64).ctypes.data_as(PD)
         lib().cstep_step(self._h, qf.ctypes.data_as(PD), o.ctypes.data_as(PD), Ep)
         return o
+
+
+def first_iteration(cs, wA, wB, f, chiN, ns):
+    """Q, F and the densities of one SCFT sweep with the C step: the
+    computation of oracle.scft.sweep / free_energy (Simpson densities over the
+    stored q, pairing q(j) with q+(ns - j)), with q+ advanced alongside so that
+    only q is stored.  Returns dict(Q, F, phiA, phiB, q, qd) with q, qd the
+    end points q(1), q+(1)."""
+    from .scft import contour_weights, free_energy
+    ds = 1.0 / ns
+    EA = np.asfortranarray(np.exp(-wA * ds / 2.0))
+    EB = np.asfortranarray(np.exp(-wB * ds / 2.0))
+    nA = int(round(f * ns))
+    q = [np.ones(cs.M, order="F")]
+    for j in range(ns):
+        q.append(cs.step(q[-1], EA if j < nA else EB))
+    qd = np.ones(cs.M, order="F")
+    wa, wb = contour_weights(nA), contour_weights(ns - nA)
+    phiA = np.zeros(cs.M)
+    phiB = np.zeros(cs.M)
+    for j in range(ns + 1):
+        # qd holds q+(j); the density pairs q(ns - j) with q+(j)
+        k = ns - j
+        if k <= nA:
+            phiA += wa[k] * q[k] * qd
+        if k >= nA:
+            phiB += wb[k - nA] * q[k] * qd
+        if j < ns:
+            qd = cs.step(qd, EB if j < ns - nA else EA)
+    Q = float(np.mean(q[ns]))
+    phiA *= ds / Q
+    phiB *= ds / Q
+    F = free_energy(wA, wB, phiA, phiB, Q, chiN)
+    return dict(Q=Q, F=F, phiA=phiA, phiB=phiB, q=q[ns], qd=qd)

@@ -255,6 +255,51 @@ def test_graph_replays_match_eager(prec):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("side_stream", [False, True])
+@pytest.mark.parametrize("graph", [0, 1])
+def test_sweep_ordered_with_caller_stream(side_stream, graph):
+    """*_dev calls run on exactly the caller's stream (NULL = the null stream,
+    PyTorch's default): set_fields -> reset -> run issued with no host
+    synchronisation, each reset queued behind a long matrix product on the
+    same stream, gives the eager result bit for bit on every repetition,
+    on the default stream and on a side stream."""
+    import torch
+    from cfft_hip.scft import PropagatorSweep
+    print(f"ordered sweep side_stream={side_stream} graph={graph}", flush=True)
+    N = (32, 32, 32)
+    ops, ops_s, shift, R, t = group_setup(229, N)
+    Lv = O.auto_L(t)
+    wA = 2.0 * O.extract_subgrid(O.make_symmetric(R, t, N, seed=21), Lv)
+    wB = -1.0 * O.extract_subgrid(O.make_symmetric(R, t, N, seed=22), Lv)
+    ref = PropagatorSweep(N, ops_s, 4.0 * np.eye(3), 0.3, 30, cf.FP64)
+    ref.plan.set_option(cf._lib.OPT_GRAPH, 0)
+    ref.set_fields(wA, wB)
+    torch.cuda.synchronize()
+    ref.reset()
+    torch.cuda.synchronize()
+    ref.run()
+    torch.cuda.synchronize()
+    eager = [x.copy() for x in ref.fields()]
+    sw = PropagatorSweep(N, ops_s, 4.0 * np.eye(3), 0.3, 30, cf.FP64)
+    sw.plan.set_option(cf._lib.OPT_GRAPH, graph)
+    big = torch.randn(3072, 3072, device="cuda")
+    torch.cuda.synchronize()
+    stream = torch.cuda.Stream() if side_stream else torch.cuda.default_stream()
+    outs = []
+    with torch.cuda.stream(stream):
+        sw.set_fields(wA, wB)
+        for _ in range(3):
+            big = (big @ big).clamp_(-1.0, 1.0)  # keeps the stream busy ahead of the reset
+            sw.reset()
+            sw.run()
+            outs.append(sw.q.clone())
+        stream.synchronize()
+    for k, q in enumerate(outs):
+        got = q.cpu().numpy()
+        for b in range(2):
+            assert np.array_equal(got[b].reshape(sw.M, order="F"), eager[b]), f"repetition {k} field {b}"
+
+
 def test_batched_instances_match_sequential():
     """Concurrent instances (own plan, context and stream each, hipGraph sweeps)
     give the same solutions as solving them one at a time."""
