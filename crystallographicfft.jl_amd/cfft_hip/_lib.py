@@ -37,7 +37,7 @@ EXPORTED = [
     "cfft_fill_map", "cfft_plan_create", "cfft_plan_info", "cfft_plan_tables",
     "cfft_update_kernel", "cfft_plan_destroy", "cfft_forward", "cfft_backward",
     "cfft_forward_g0", "cfft_diffuse", "cfft_step", "cfft_sub_to_full", "cfft_full_to_sub",
-    "cfft_forward_dev", "cfft_backward_dev", "cfft_diffuse_dev", "cfft_step_dev",
+    "cfft_forward_dev", "cfft_backward_dev", "cfft_diffuse_dev", "cfft_diffuse_noalloc_dev", "cfft_step_dev",
     "cfft_exp_dev", "cfft_sync", "cfft_step_ptrs_dev", "cfft_profile", "cfft_profile_read",
     "cfft_time_stage_dev", "cfft_sweep_dev", "cfft_plan_set_option",
     "cfft_calc_asu", "cfft_pack_asu", "cfft_block_plan_create", "cfft_block_plan_info",
@@ -112,6 +112,7 @@ def _declare(lib):
         "cfft_forward_dev": (C.c_int, [P, P, P, P]),
         "cfft_backward_dev": (C.c_int, [P, P, P, P]),
         "cfft_diffuse_dev": (C.c_int, [P, P, P]),
+        "cfft_diffuse_noalloc_dev": (C.c_int, [P, P, P]),
         "cfft_step_dev": (C.c_int, [P, P, P, I32, P]),
         "cfft_exp_dev": (C.c_int, [P, P, P, D, I64, P]),
         "cfft_sync": (C.c_int, [P]),

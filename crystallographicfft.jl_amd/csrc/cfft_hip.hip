@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -31,6 +32,7 @@ std::recursive_mutex& cfft::setup_mutex() {
 
 struct cfft_plan {
   int N[3], M[3], L[3], H[3];
+  int device = 0;  // device current when the plan was created
   int d = 1, n_spec = 0, cent = 0, n_ch = 0, path = 0, prec = 0, batch = 1, P = 1;
   double ds = 0.0, lattice[9];
   std::vector<Op> ops;
@@ -606,11 +608,12 @@ PipeP pipe_params(cfft_plan* p, const ESel& es) {
   return pp;
 }
 
-PipeBufs pipe_bufs(cfft_plan* p, const void* f0_in, void* f0_out, void* F, bool withK) {
+// ch: the channel buffer of the call (nullptr: the plan's own).
+PipeBufs pipe_bufs(cfft_plan* p, const void* f0_in, void* f0_out, void* F, bool withK, void* ch = nullptr) {
   PipeBufs B{};
   B.f0_in = f0_in;
   B.f0_out = f0_out;
-  B.ch = p->d_ch.p;
+  B.ch = ch ? ch : p->d_ch.p;
   B.F = F;
   B.hs = static_cast<const int*>(p->d_hs.p);
   B.K = withK ? static_cast<const double*>(p->d_K.p) : nullptr;
@@ -818,34 +821,50 @@ void launch_p51(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipS
   launch_check<T>("k_p51");
 }
 
+// Work buffers of one diffusion call: channel spectra, the Q-multiplied
+// channels (paths M2Q, M7Q) and the spectral ASU values (paths M2, M7).
+struct Work {
+  void* ch;
+  void* ch2;
+  void* F;
+};
+
+Work own_work(const cfft_plan* p) { return {p->d_ch.p, p->d_ch2.p, p->d_F.p}; }
+
 // f0 -> channel spectra (P1, P2).
 template <typename T>
-void run_forward_channels(cfft_plan* p, const T* f0, const PipeP& pp, int step, hipStream_t s) {
-  PipeBufs B = pipe_bufs(p, f0, nullptr, nullptr, false);
+void run_forward_channels(cfft_plan* p, const T* f0, const PipeP& pp, int step, hipStream_t s,
+                          void* ch = nullptr) {
+  PipeBufs B = pipe_bufs(p, f0, nullptr, nullptr, false, ch);
   launch_p1<T>(p, pp, B, step, s);
   launch_p2<T>(p, pp, B, s);
 }
 
 template <typename T>
-void run_recon(cfft_plan* p, const PipeP& pp, cx<T>* F, bool withK, hipStream_t s) {
-  PipeBufs B = pipe_bufs(p, nullptr, nullptr, F, withK);
+void run_recon(cfft_plan* p, const PipeP& pp, cx<T>* F, bool withK, hipStream_t s, void* ch = nullptr) {
+  PipeBufs B = pipe_bufs(p, nullptr, nullptr, F, withK, ch);
   launch_p3<T>(p, pp, B, s);
 }
 
 // F -> f0 (P4, P5).
 template <typename T>
-void run_backward(cfft_plan* p, const PipeP& pp, const cx<T>* F, T* f0, int step, hipStream_t s) {
-  PipeBufs B = pipe_bufs(p, nullptr, f0, const_cast<cx<T>*>(F), false);
+void run_backward(cfft_plan* p, const PipeP& pp, const cx<T>* F, T* f0, int step, hipStream_t s,
+                  void* ch = nullptr) {
+  PipeBufs B = pipe_bufs(p, nullptr, f0, const_cast<cx<T>*>(F), false, ch);
   launch_p4<T>(p, pp, B, s);
   launch_p5<T>(p, pp, B, step, s);
 }
 
 // One diffusion operator application per step, with the interaction factors
-// of `pp` on both sides: out <- E D (E in).  `in` may equal `out`.
+// of `pp` on both sides: out <- E D (E in).  `in` may equal `out`.  The work
+// buffers are the plan's own unless `wk` names others (path QR: the plan's
+// own always).
 template <typename T>
-void run_diffuse(cfft_plan* p, const T* in, T* out_f0, const PipeP& pp, int step, hipStream_t s) {
+void run_diffuse(cfft_plan* p, const T* in, T* out_f0, const PipeP& pp, int step, hipStream_t s,
+                 const Work* wk = nullptr) {
+  const Work w = wk ? *wk : own_work(p);
   if (is_qs(p)) {
-    PipeBufs B = pipe_bufs(p, in, out_f0, nullptr, false);
+    PipeBufs B = pipe_bufs(p, in, out_f0, nullptr, false, w.ch);
     launch_p1<T>(p, pp, B, step, s);
     launch_p2q<T>(p, pp, B, s);
     launch_p5<T>(p, pp, B, step, s);
@@ -859,16 +878,16 @@ void run_diffuse(cfft_plan* p, const T* in, T* out_f0, const PipeP& pp, int step
     return;
   }
   if (is_q_path(p)) {
-    run_forward_channels<T>(p, in, pp, step, s);
-    cx<T>* out = static_cast<cx<T>*>(p->d_ch2.p);
+    run_forward_channels<T>(p, in, pp, step, s, w.ch);
+    cx<T>* out = static_cast<cx<T>*>(w.ch2);
     {
       StageTimer st(p, s, CFFT_STAGE_QMUL);
       hipLaunchKernelGGL(k_qmul<T>, dim3(nblk(p->n_alive * p->batch, 256)), dim3(256), 0, s,
-                         static_cast<const cx<T>*>(p->d_ch.p), out, static_cast<const cx<T>*>(p->d_Q.p),
+                         static_cast<const cx<T>*>(w.ch), out, static_cast<const cx<T>*>(p->d_Q.p),
                          static_cast<const int*>(p->d_gidx.p), p->d, p->n_alive, p->batch);
       launch_check<T>("k_qmul");
     }
-    PipeBufs B = pipe_bufs(p, nullptr, out_f0, nullptr, false);
+    PipeBufs B = pipe_bufs(p, nullptr, out_f0, nullptr, false, w.ch);
     if (cubic(p)) {
       StageTimer st(p, s, CFFT_STAGE_IFFT_Z);
       const unsigned g = (unsigned)n_planes(p);
@@ -885,10 +904,10 @@ void run_diffuse(cfft_plan* p, const T* in, T* out_f0, const PipeP& pp, int step
     launch_p5<T>(p, pp, B, step, s);
     return;
   }
-  cx<T>* F = static_cast<cx<T>*>(p->d_F.p);
-  run_forward_channels<T>(p, in, pp, step, s);
-  run_recon<T>(p, pp, F, true, s);
-  run_backward<T>(p, pp, F, out_f0, step, s);
+  cx<T>* F = static_cast<cx<T>*>(w.F);
+  run_forward_channels<T>(p, in, pp, step, s, w.ch);
+  run_recon<T>(p, pp, F, true, s, w.ch);
+  run_backward<T>(p, pp, F, out_f0, step, s, w.ch);
 }
 
 // Persistent cooperative sweep of nsteps Strang steps (cubic H-grids, fwd+K+bwd
@@ -1307,6 +1326,7 @@ int cfft_plan_create(const int32_t* N, const int32_t* ops, int32_t nops, double 
     std::unique_ptr<cfft_plan> p(new cfft_plan());
     int n[3] = {N[0], N[1], N[2]};
     build_plan(p.get(), n, unpack_ops(ops, nops), ds, lattice, path, precision, batch);
+    hip_check(hipGetDevice(&p->device), "hipGetDevice");  // the device build_plan allocated on
     *out = p.release();
   });
 }
@@ -1494,6 +1514,69 @@ int cfft_diffuse_dev(cfft_plan* p, void* f0, void* stream) {
       using T = decltype(tag);
       PipeP pp = pipe_params(p, ESel{});
       run_diffuse<T>(p, static_cast<T*>(f0), static_cast<T*>(f0), pp, 0, s);
+    };
+    if (p->prec == CFFT_FP64) run(double{}); else run(float{});
+  });
+}
+
+// ---- diffusion without allocation ------------------------------------------
+// The work buffers of cfft_diffuse_noalloc_dev live in one statically sized
+// device array, loaded with the code object on every device that runs the
+// library and held there until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is a stand-in, not a device
+// address): each device's address comes from hipGetSymbolAddress, taken once
+// per device with that device current, and only that address is used.
+constexpr size_t kScratchBytes = size_t(16) << 20;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::atomic<void*> g_scratch_ptr[kScratchDevices];
+
+void* scratch_ptr(int dev) {
+  require(dev >= 0 && dev < kScratchDevices, CFFT_ERR_UNSUPPORTED, "device index beyond 63");
+  void* ptr = g_scratch_ptr[dev].load(std::memory_order_acquire);
+  if (ptr) return ptr;
+  std::lock_guard<std::recursive_mutex> guard(setup_mutex());
+  ptr = g_scratch_ptr[dev].load(std::memory_order_relaxed);
+  if (!ptr) {
+    hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    require(ptr != nullptr, CFFT_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+    g_scratch_ptr[dev].store(ptr, std::memory_order_release);
+  }
+  return ptr;
+}
+
+size_t round256(size_t n) { return (n + 255) & ~size_t(255); }
+}  // namespace
+
+int cfft_diffuse_noalloc_dev(cfft_plan* p, void* f0, void* stream) {
+  return guarded([&] {
+    check_plan(p);
+    require(f0 != nullptr, CFFT_ERR_INVALID, "null argument");
+    require(!is_qr(p), CFFT_ERR_UNSUPPORTED,
+            "no-allocation diffusion supports paths M2, M7, M2Q, M7Q and QS (path QR keeps plan buffers)");
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    require(dev == p->device, CFFT_ERR_INVALID, "the plan's device is not the current device");
+    // the work buffers the path reads and writes, each 256-byte aligned
+    const size_t ch = round256(2 * p->esz * (size_t)p->n_alive * p->batch);
+    const size_t ch2 = (is_q_path(p) && !is_qs(p)) ? ch : 0;
+    const size_t F = is_q_path(p) ? 0 : round256(2 * p->esz * (size_t)std::max(p->n_spec, 1) * p->batch);
+    const size_t need = ch + ch2 + F;
+    if (need > kScratchBytes)
+      throw Error{CFFT_ERR_SIZE, "no-allocation diffusion needs " + std::to_string(need) +
+                                     " bytes of work memory; the maximum is " + std::to_string(kScratchBytes) +
+                                     " (16 MiB): use cfft_diffuse_dev"};
+    unsigned char* base = static_cast<unsigned char*>(scratch_ptr(dev));
+    // every work buffer is written in full before it is read, so the
+    // previous call's contents need no clearing
+    const Work w{base, ch2 ? base + ch : nullptr, F ? base + ch + ch2 : nullptr};
+    hipStream_t s = pick(p, stream);
+    auto run = [&](auto tag) {
+      using T = decltype(tag);
+      PipeP pp = pipe_params(p, ESel{});
+      run_diffuse<T>(p, static_cast<T*>(f0), static_cast<T*>(f0), pp, 0, s, &w);
     };
     if (p->prec == CFFT_FP64) run(double{}); else run(float{});
   });

@@ -175,6 +175,18 @@ int cfft_full_to_sub(cfft_plan* plan, const double* u, double* f0);
 int cfft_forward_dev(cfft_plan* plan, const void* f0, void* F_spec, void* stream);
 int cfft_backward_dev(cfft_plan* plan, const void* F_spec, void* f0, void* stream);
 int cfft_diffuse_dev(cfft_plan* plan, void* f0, void* stream);
+/* As cfft_diffuse_dev, allocating nothing: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The work
+ * buffers (channel spectra, Q-multiplied channels, spectral values) live in
+ * one static device array of 16 MiB per device, held until the process
+ * ends; a plan needing more (e.g. more than 8 fields of Im-3m 128^3 FP64)
+ * gets CFFT_ERR_SIZE and nothing runs.  Paths M2, M7, M2Q, M7Q and QS (QR:
+ * CFFT_ERR_UNSUPPORTED); the plan's device must be current.  Every call on a
+ * device shares that device's array: calls that overlap in time on one
+ * device (two streams, two threads) race and give wrong results.  The array
+ * needs no clearing (each call writes its buffers before reading them). */
+int cfft_diffuse_noalloc_dev(cfft_plan* plan, void* f0, void* stream);
 /* E = exp(-w dt/2) per batch entry (batch x M^3, plan precision), applied on
  * both sides of D: q <- E D (E q). */
 int cfft_step_dev(cfft_plan* plan, void* q, const void* E, int32_t nsteps, void* stream);
