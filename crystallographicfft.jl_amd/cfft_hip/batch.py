@@ -19,7 +19,7 @@ import numpy as np
 
 from . import _lib as L
 from .plans import auto_L, find_optimal_shift
-from .scft import SCFTBatch, SCFTSolver, default_path, symmetrize_subgrid
+from .scft import SCFTBatch, SCFTSolver, default_path
 from .symmetry_ops import get_ops
 
 
@@ -53,33 +53,103 @@ def draw_instances(n, seed, sizes=(32, 48, 64, 96), groups=(47, 70, 223, 229, 23
 _STARS = ((1, 0, 0), (1, 1, 0), (1, 1, 1), (2, 0, 0), (2, 1, 0), (2, 1, 1), (2, 2, 0), (2, 2, 1), (3, 1, 0))
 
 
+def star_wave(h, fn, N, ops_s, shift, Lv):
+    """Group average of fn(2 pi h.(X + b)/N) (fn = np.cos or np.sin, b the
+    magic shift in grid units) at the stride-L subgrid points X = L x:
+    (1/|G|) sum_g fn(2 pi (kappa_g . X + c_g)) with kappa_g = R_g^T (h / N)
+    and c_g = h.(t_g + b)/N, summed over the distinct kappa_g (the star of h)
+    as products of 1-D exponentials.  Equals symmetrize_subgrid of the full-grid
+    wave (the gather form of test/test_helpers.jl:11-29) without building it."""
+    N = np.asarray(N, dtype=np.int64)
+    Lv = np.asarray(Lv, dtype=np.int64)
+    M = N // Lv
+    b = np.asarray(shift, dtype=np.float64) * N
+    hv = np.asarray(h, dtype=np.float64)
+    terms = {}
+    for op in ops_s:
+        kap = (hv / N) @ np.asarray(op.Rm, dtype=np.float64)
+        c = float(np.sum(hv * (np.asarray(op.tv, dtype=np.float64) + b) / N))
+        key = tuple(np.round(kap * N, 9))
+        a = np.exp(2j * np.pi * (c - np.floor(c)))
+        k0, a0 = terms.get(key, (kap, 0.0))
+        terms[key] = (k0, a0 + a)
+    acc = np.zeros(tuple(M), dtype=np.complex128)
+    for kap, a in terms.values():
+        if abs(a) < 1e-13 * len(ops_s):
+            continue
+        e = [np.exp(2j * np.pi * kap[d] * Lv[d] * np.arange(M[d])) for d in range(3)]
+        acc += a * e[0][:, None, None] * e[1][None, :, None] * e[2][None, None, :]
+    acc /= len(ops_s)
+    return acc.real if fn is np.cos else acc.imag
+
+
 def lowest_star(N, ops_s, shift, Lv):
     """Symmetrised cos 2 pi h.x (or sin) for the smallest |h| whose star is not
     extinct in the group (x in cell units at (i + b)/N), normalised to unit RMS,
     on the subgrid, and h.  {100} is extinct in Pm-3n, Fddd, Im-3m and Ia-3d, so
     a fixed cos x + cos y + cos z start would symmetrise to zero there."""
-    x = [(np.arange(n) + s * n) / n for n, s in zip(N, shift)]
-    X, Y, Z = np.meshgrid(*x, indexing="ij")
     for h in _STARS:
-        arg = 2 * np.pi * (h[0] * X + h[1] * Y + h[2] * Z)
         for fn in (np.cos, np.sin):
-            u = symmetrize_subgrid(fn(arg), ops_s, Lv)
+            u = star_wave(h, fn, N, ops_s, shift, Lv)
             rms = float(np.sqrt(np.mean(u * u)))
             if rms > 1e-3:
                 return u / rms, h
     raise ValueError("no harmonic up to |h|^2 = 10 survives the group average")
 
 
+def subgrid_maps(N, ops_s, Lv):
+    """Index maps of the stride-L subgrid (x fastest) under the ops that map
+    it onto itself, one per distinct map: x' = R (L x) + t over L, mod M."""
+    N = np.asarray(N, dtype=np.int64)
+    Lv = np.asarray(Lv, dtype=np.int64)
+    M = N // Lv
+    ax = [np.arange(m, dtype=np.int64) * l for m, l in zip(M, Lv)]
+    grid = (ax[0][:, None, None], ax[1][None, :, None], ax[2][None, None, :])
+    maps = []
+    seen = set()
+    for op in ops_s:
+        R = np.asarray(op.Rm, dtype=np.int64)
+        t = np.asarray(op.tv, dtype=np.int64)
+        # the op keeps the subgrid iff t and every column L_j R_:j are
+        # multiples of L (row-wise)
+        if np.any(t % Lv != 0) or np.any((R * Lv[None, :]) % Lv[:, None] != 0):
+            continue
+        y = [np.mod(R[i, 0] * grid[0] + R[i, 1] * grid[1] + R[i, 2] * grid[2] + t[i], N[i]) // Lv[i]
+             for i in range(3)]
+        idx = (y[0] + M[0] * (y[1] + M[1] * y[2])).ravel(order="F")
+        key = idx.tobytes()
+        if key not in seen:
+            seen.add(key)
+            maps.append(idx)
+    return maps
+
+
+def project_subgrid(f, maps):
+    """Mean of a subgrid field over the maps: the field symmetric under the
+    group closest to f (a field invariant under the ops that keep the subgrid
+    is the subgrid of exactly one symmetric full-grid field)."""
+    v = np.asarray(f, dtype=np.float64).ravel(order="F")
+    acc = np.zeros_like(v)
+    for idx in maps:
+        acc += v[idx]
+    return (acc / len(maps)).reshape(np.shape(f), order="F")
+
+
 def start_fields(inst: Instance, ops_s, shift, amp=2.0, noise=0.1):
     """Seeded cosine-plus-noise start: w_A = +amp c + n_A, w_B = -amp c + n_B,
-    c = the group's lowest non-extinct star (unit RMS), n_A, n_B symmetrised
-    seeded N(0, noise) fields; on the subgrid."""
+    c = the group's lowest non-extinct star (unit RMS), n_A, n_B seeded
+    N(0, noise sqrt(n_maps / |G|)) subgrid fields projected onto the symmetric
+    ones (the scale gives the point variance of a full-grid N(0, noise) field
+    averaged over the group); on the subgrid."""
     N = inst.N
     Lv = auto_L(ops_s)
+    M = tuple(n // l for n, l in zip(N, Lv))
     rng = np.random.default_rng(inst.seed)
     c, _ = lowest_star(N, ops_s, shift, Lv)
-    wA = amp * c + symmetrize_subgrid(noise * rng.standard_normal(N), ops_s, Lv)
-    wB = -amp * c + symmetrize_subgrid(noise * rng.standard_normal(N), ops_s, Lv)
+    maps = subgrid_maps(N, ops_s, Lv)
+    sig = noise * np.sqrt(len(maps) / len(ops_s))
+    wA = amp * c + project_subgrid(sig * rng.standard_normal(M), maps)
+    wB = -amp * c + project_subgrid(sig * rng.standard_normal(M), maps)
     return wA, wB
 
 

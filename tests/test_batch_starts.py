@@ -108,3 +108,54 @@ def test_schedule_candidates():
     assert rows(32) == [2, 4, 8] and rows(12) == [2, 4] and rows(6) == [2] and rows(3) == [1]
     assert all({c[L.OPT_GRAPH] for c in schedule_candidates(32) if c[L.OPT_TILE_ROWS] == r} == {0, 1}
                for r in (2, 4, 8))
+
+
+@pytest.mark.parametrize("sg", [47, 70, 223, 229, 230, 225])
+def test_star_wave_equals_full_grid_group_average(sg):
+    """The star-sum form of the symmetrised plane wave against the brute-force
+    group average of the full-grid wave (gather form of
+    test/test_helpers.jl:11-29), every candidate star, cos and sin."""
+    N = (16, 16, 16)
+    ops = get_ops(sg, 3, N)
+    shift, ops_s = find_optimal_shift(ops, N)
+    Lv = auto_L(ops_s)
+    x = [(np.arange(n) + s * n) / n for n, s in zip(N, shift)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    for h in batch._STARS:
+        for fn in (np.cos, np.sin):
+            ref = symmetrize_subgrid(fn(2 * np.pi * (h[0] * X + h[1] * Y + h[2] * Z)), ops_s, Lv)
+            assert np.max(np.abs(batch.star_wave(h, fn, N, ops_s, shift, Lv) - ref)) < 1e-13
+
+
+@pytest.mark.parametrize("sg,n", [(47, 16), (70, 16), (223, 24), (229, 16), (230, 24)])
+def test_subgrid_projection(sg, n):
+    """Projection onto the symmetric subgrid fields: the subgrid of a
+    symmetric full-grid field is a fixed point, a projected random field is
+    invariant under every op that keeps the subgrid (brute force over the
+    full-grid ops) and the projection is idempotent and keeps the sum."""
+    N = (n, n, n)
+    ops = get_ops(sg, 3, N)
+    shift, ops_s = find_optimal_shift(ops, N)
+    Lv = auto_L(ops_s)
+    maps = batch.subgrid_maps(N, ops_s, Lv)
+    rng = np.random.default_rng(sg)
+    sym = symmetrize_subgrid(rng.standard_normal(N), ops_s, Lv)
+    assert np.max(np.abs(batch.project_subgrid(sym, maps) - sym)) < 1e-14
+    M = sym.shape
+    f = rng.standard_normal(M)
+    p = batch.project_subgrid(f, maps)
+    assert np.max(np.abs(batch.project_subgrid(p, maps) - p)) < 1e-14
+    assert abs(p.sum() - f.sum()) < 1e-10
+    # invariance, from the full-grid ops directly
+    g = np.meshgrid(*[np.arange(m) * l for m, l in zip(M, Lv)], indexing="ij")
+    Xs = np.stack([a.ravel(order="F") for a in g], axis=1)
+    pv = p.ravel(order="F")
+    n_keep = 0
+    for op in ops_s:
+        Yf = np.mod(Xs @ op.Rm.T + op.tv, np.asarray(N))
+        if np.all(Yf % np.asarray(Lv) == 0):
+            n_keep += 1
+            y = Yf // np.asarray(Lv)
+            img = y[:, 0] + M[0] * (y[:, 1] + M[1] * y[:, 2])
+            assert np.max(np.abs(pv[img] - pv)) < 1e-14
+    assert n_keep >= len(maps) >= 1
