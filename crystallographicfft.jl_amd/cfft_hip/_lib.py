@@ -38,6 +38,7 @@ EXPORTED = [
     "cfft_update_kernel", "cfft_plan_destroy", "cfft_forward", "cfft_backward",
     "cfft_forward_g0", "cfft_diffuse", "cfft_step", "cfft_sub_to_full", "cfft_full_to_sub",
     "cfft_forward_dev", "cfft_backward_dev", "cfft_diffuse_dev", "cfft_diffuse_noalloc_dev", "cfft_step_dev",
+    "cfft_symmetrize_f64_dev",
     "cfft_exp_dev", "cfft_sync", "cfft_step_ptrs_dev", "cfft_profile", "cfft_profile_read",
     "cfft_time_stage_dev", "cfft_sweep_dev", "cfft_plan_set_option",
     "cfft_calc_asu", "cfft_pack_asu", "cfft_block_plan_create", "cfft_block_plan_info",
@@ -58,6 +59,7 @@ OPT_PERSISTENT = 1
 OPT_FUSE = 2
 OPT_GRAPH = 3
 OPT_TILE_ROWS = 4
+OPT_FP32_TWIDDLES = 5
 
 STAGES = ["p1_fold_fftx", "fft_x", "p2_fft_yz", "fft_z", "p3_recon", "p4_irecon_ifft_zy", "ifft_x",
           "ifft_y", "ifft_z", "p5_ifftx_unfold", "qmul", "sweep", "p51_unfold_fold", "p2q_plane_pair"]
@@ -113,6 +115,7 @@ def _declare(lib):
         "cfft_backward_dev": (C.c_int, [P, P, P, P]),
         "cfft_diffuse_dev": (C.c_int, [P, P, P]),
         "cfft_diffuse_noalloc_dev": (C.c_int, [P, P, P]),
+        "cfft_symmetrize_f64_dev": (C.c_int, [P, P, P, I64, P]),
         "cfft_step_dev": (C.c_int, [P, P, P, I32, P]),
         "cfft_exp_dev": (C.c_int, [P, P, P, D, I64, P]),
         "cfft_sync": (C.c_int, [P]),

@@ -88,8 +88,9 @@ class PropagatorSweep:
         self.dtype = torch.float64 if precision == L.FP64 else torch.float32
         self.dev = torch.device("cuda")
         self.q = torch.ones(2, self.M3, dtype=self.dtype, device=self.dev)
-        self.EA = torch.empty(self.M3, dtype=self.dtype, device=self.dev)
-        self.EB = torch.empty(self.M3, dtype=self.dtype, device=self.dev)
+        # interaction factors: FP64 in both precisions
+        self.EA = torch.empty(self.M3, dtype=torch.float64, device=self.dev)
+        self.EB = torch.empty(self.M3, dtype=torch.float64, device=self.dev)
         self.w = torch.empty(2, self.M3, dtype=self.dtype, device=self.dev)
 
     def stream(self):

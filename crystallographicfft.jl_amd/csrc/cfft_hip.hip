@@ -53,12 +53,15 @@ struct cfft_plan {
   DevBuf d_f0, d_ch, d_ch2, d_F, d_E, d_u;
   DevBuf d_qmap, d_grep;  // path QR: element -> (rep, phase); rep values
   DevBuf d_qsep;          // path QS: axis factors F[d][s][q]
+  DevBuf d_symops;        // ops that keep the subgrid, acting on it (12 int32 each)
+  int n_sym = 0;
   std::vector<int32_t> fmap_host;
   // options (cfft_plan_set_option)
   int persistent = 0;  // contour sweeps as one cooperative launch
   int fuse = 1;        // fuse P5 of a step with P1 of the next
   int graph = 1;       // replay multi-step chains from captured hipGraphs
   int tile_rows = 0;   // y rows per fold/unfold tile (0: the default rule)
+  int arith32 = 0;     // mixed precision on path QS: FP32 twiddle and fold-factor arithmetic (option)
   hipStream_t cap_stream = nullptr;  // capture stream of the chains
   struct Cached {
     std::string key;
@@ -98,7 +101,7 @@ bool chain_path(const cfft_plan* p) { return !is_q_path(p) || is_qr(p) || is_qs(
 // Mixed precision on path QS: the fold, unfold, transform and Q-multiply
 // arithmetic of its kernels in FP32 (tables stay FP64 in memory and are
 // rounded where they are used); the other paths form twiddle products in FP64.
-bool fp32_arith(const cfft_plan* p) { return p->prec == CFFT_MIXED && p->path == CFFT_PATH_QS; }
+bool fp32_arith(const cfft_plan* p) { return p->prec == CFFT_MIXED && p->path == CFFT_PATH_QS && p->arith32; }
 
 
 // Uploads hold the set-up lock (allocation and a blocking copy); the host
@@ -463,12 +466,37 @@ void build_plan(cfft_plan* p, const int* N, const std::vector<Op>& ops, double d
     }
   }
 
+  // The ops that map the stride-L subgrid onto itself, as maps of the
+  // subgrid: x'_i = sum_j R_ij L_j / L_i x_j + t_i / L_i (mod M_i), one entry
+  // per distinct map.  A field symmetric under the group is invariant under
+  // each; averaging over them projects onto those fields.
+  {
+    std::vector<std::array<int, 12>> sym;
+    for (const Op& g : ops) {
+      bool keeps = true;
+      std::array<int, 12> e{};
+      for (int i = 0; i < 3 && keeps; ++i) {
+        keeps = g.t[i] % p->L[i] == 0;
+        for (int j = 0; j < 3 && keeps; ++j) {
+          keeps = (g.R[i][j] * p->L[j]) % p->L[i] == 0;
+          e[3 * i + j] = g.R[i][j] * p->L[j] / p->L[i];
+        }
+        e[9 + i] = ((g.t[i] / p->L[i]) % p->M[i] + p->M[i]) % p->M[i];
+      }
+      if (keeps && std::find(sym.begin(), sym.end(), e) == sym.end()) sym.push_back(e);
+    }
+    p->n_sym = (int)sym.size();
+    std::vector<int32_t> flat;
+    for (const auto& e : sym) flat.insert(flat.end(), e.begin(), e.end());
+    upload_bytes(p->d_symops, flat.data(), flat.size() * sizeof(int32_t));
+  }
+
   const size_t es = p->esz;
   p->d_f0.alloc(es * p->M3 * batch);
   p->d_ch.alloc(2 * es * p->n_alive * batch);
   if (is_q_path(p) && !is_qr(p)) p->d_ch2.alloc(2 * es * p->n_alive * batch);
   p->d_F.alloc(2 * es * (size_t)std::max(p->n_spec, 1) * batch);
-  p->d_E.alloc(es * p->M3 * batch);
+  p->d_E.alloc(sizeof(double) * p->M3 * batch);  // interaction factors: FP64 in both precisions
 }
 
 
@@ -945,6 +973,33 @@ bool run_sweep_persistent(cfft_plan* p, T* q, const PipeP& pp, int step0, int ns
   return true;
 }
 
+// Projection of subgrid fields onto those invariant under the ops that keep
+// the subgrid (ops: n_sym x 12 int32, R' row-major then t'; the identity is
+// one of them): out(x) = mean_g in(R' x + t' mod M).  Fields of M0 M1 M2
+// entries (x fastest) one after another; n = total entries.
+__global__ void k_symmetrize(const double* __restrict__ in, double* __restrict__ out,
+                             const int32_t* __restrict__ ops, int n_sym, int M0, int M1, int M2, long n) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long M3 = (long)M0 * M1 * M2;
+  const long b = i / M3;
+  const long r = i - b * M3;
+  const int x = (int)(r % M0), y = (int)((r / M0) % M1), z = (int)(r / ((long)M0 * M1));
+  const double* f = in + b * M3;
+  double s = 0.0;
+  for (int g = 0; g < n_sym; ++g) {
+    const int32_t* o = ops + 12 * g;
+    int xp = (o[0] * x + o[1] * y + o[2] * z + o[9]) % M0;
+    int yp = (o[3] * x + o[4] * y + o[5] * z + o[10]) % M1;
+    int zp = (o[6] * x + o[7] * y + o[8] * z + o[11]) % M2;
+    xp += xp < 0 ? M0 : 0;
+    yp += yp < 0 ? M1 : 0;
+    zp += zp < 0 ? M2 : 0;
+    s += f[xp + (long)M0 * (yp + (long)M1 * zp)];
+  }
+  out[i] = s / n_sym;
+}
+
 // The stream a *_dev call runs on: exactly the caller's, with NULL the null
 // stream (the default stream of HIP and of PyTorch), so the call is ordered
 // after the caller's earlier work on that stream and before its later work.
@@ -1008,8 +1063,8 @@ void diffuse_host(cfft_plan* p, double* f0) {
 }
 
 template <typename T>
-void exp_dev(cfft_plan* p, const T* w, T* E, double factor, long n, hipStream_t s) {
-  hipLaunchKernelGGL(k_exp<T>, dim3(nblk(n, 256)), dim3(256), 0, s, w, E, (T)factor, n);
+void exp_dev(cfft_plan* p, const T* w, double* E, double factor, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_exp<T>, dim3(nblk(n, 256)), dim3(256), 0, s, w, E, factor, n);
   launch_check<T>("k_exp");
 }
 
@@ -1081,7 +1136,7 @@ constexpr size_t kMaxGraphs = 16;
 template <typename T>
 void run_steps_graph(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int nsteps, hipStream_t s) {
   std::string key(reinterpret_cast<const char*>(&pp), sizeof(PipeP));
-  const long hdr[5] = {reinterpret_cast<long>(q), stride, step0, nsteps, p->fuse};
+  const long hdr[6] = {reinterpret_cast<long>(q), stride, step0, nsteps, p->fuse, p->arith32};
   key.append(reinterpret_cast<const char*>(hdr), sizeof(hdr));
   for (size_t i = 0; i < p->graphs.size(); ++i)
     if (p->graphs[i].key == key) {
@@ -1190,9 +1245,9 @@ double time_stage(cfft_plan* p, T* q, const ESel& es, int stage, int reps, hipSt
 template <typename T>
 void step_host(cfft_plan* p, double* q, const double* w, double dt, int nsteps) {
   to_dev<T>(p, p->d_f0, q, p->M3 * p->batch);
-  to_dev<T>(p, p->d_E, w, p->M3 * p->batch);
-  exp_dev<T>(p, static_cast<const T*>(p->d_E.p), static_cast<T*>(p->d_E.p), -dt / 2.0,
-             p->M3 * p->batch, p->stream);
+  to_dev<double>(p, p->d_E, w, p->M3 * p->batch);  // w in FP64, E in place
+  exp_dev<double>(p, static_cast<const double*>(p->d_E.p), static_cast<double*>(p->d_E.p), -dt / 2.0,
+                  p->M3 * p->batch, p->stream);
   ESel es;
   es.mode = 1;
   es.contig = p->d_E.p;
@@ -1519,6 +1574,20 @@ int cfft_diffuse_dev(cfft_plan* p, void* f0, void* stream) {
   });
 }
 
+int cfft_symmetrize_f64_dev(cfft_plan* p, const void* in, void* out, int64_t n_fields, void* stream) {
+  return guarded([&] {
+    check_plan(p);
+    require(in && out && n_fields >= 0, CFFT_ERR_INVALID, "bad argument");
+    require(in != out, CFFT_ERR_INVALID, "symmetrize reads and writes different arrays");
+    const long n = p->M3 * n_fields;
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_symmetrize, dim3(nblk(n, 256)), dim3(256), 0, pick(p, stream),
+                       static_cast<const double*>(in), static_cast<double*>(out),
+                       static_cast<const int32_t*>(p->d_symops.p), p->n_sym, p->M[0], p->M[1], p->M[2], n);
+    launch_check<double>("k_symmetrize");
+  });
+}
+
 // ---- diffusion without allocation ------------------------------------------
 // The work buffers of cfft_diffuse_noalloc_dev live in one statically sized
 // device array, loaded with the code object on every device that runs the
@@ -1683,6 +1752,7 @@ int cfft_plan_set_option(cfft_plan* p, int32_t key, int64_t value) {
                 CFFT_ERR_INVALID, "tile rows must be 0 or 1, 2, 4, 8 dividing H1");
         p->tile_rows = (int)value;
         break;
+      case CFFT_OPT_FP32_TWIDDLES: p->arith32 = value != 0; break;
       default: throw Error{CFFT_ERR_INVALID, "unknown option " + std::to_string(key)};
     }
   });
@@ -1724,7 +1794,7 @@ int cfft_exp_dev(cfft_plan* p, const void* w, void* E, double factor, int64_t n,
     if (p->prec == CFFT_FP64)
       exp_dev<double>(p, static_cast<const double*>(w), static_cast<double*>(E), factor, n, s);
     else
-      exp_dev<float>(p, static_cast<const float*>(w), static_cast<float*>(E), factor, n, s);
+      exp_dev<float>(p, static_cast<const float*>(w), static_cast<double*>(E), factor, n, s);
   });
 }
 

@@ -173,9 +173,9 @@ __global__ void __launch_bounds__(256) k_qmul(const cx<T>* __restrict__ ch, cx<T
 }
 
 template <typename T>
-__global__ void k_exp(const T* __restrict__ w, T* __restrict__ E, T factor, long n) {
+__global__ void k_exp(const T* __restrict__ w, double* __restrict__ E, double factor, long n) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i < n) E[i] = exp(w[i] * factor);
+  if (i < n) E[i] = exp((double)w[i] * factor);
 }
 
 template <typename T>

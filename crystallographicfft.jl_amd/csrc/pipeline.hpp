@@ -110,10 +110,19 @@ __device__ __forceinline__ void lds_fft(cx<T>* buf, int n_tr, int ts, int es) {
   lds_fft2<T, TW, LEN, DIR, NT>(buf, n_tr, ts, 1, 0, es);
 }
 
-template <typename T>
-__device__ __forceinline__ const T* pipe_E(const PipeP& p, int b, int step, long M3) {
-  if (p.E_contig) return p.E1[0] ? static_cast<const T*>(p.E1[0]) + b * M3 : nullptr;
-  return static_cast<const T*>(step < p.sw[b] ? p.E1[b] : p.E2[b]);
+// Interaction factors E = exp(-w ds/2) of batch entry b at contour step
+// `step`: FP64 in both precisions (an FP32 rounding of E would be the same
+// relative error at a point on every step, compounding along the contour
+// instead of averaging out).  Every candidate is read before the choice
+// (selects, no branches), so the reads issue together.
+__device__ __forceinline__ const double* pipe_E(const PipeP& p, int b, int step, long M3) {
+  const void* c0 = p.E1[0];
+  const void* e1 = p.E1[b];
+  const void* e2 = p.E2[b];
+  const int sw = p.sw[b];
+  const double* contig = c0 ? static_cast<const double*>(c0) + b * M3 : nullptr;
+  const double* split = static_cast<const double*>(step < sw ? e1 : e2);
+  return p.E_contig ? contig : split;
 }
 
 __device__ __forceinline__ long corner_index(const PipeP& p, int x, int y, int z, int e) {
@@ -271,19 +280,29 @@ __device__ __forceinline__ void scatter_parities(const PipeP& p, cx<T>* lds, int
   }
 }
 
+// Tile item -> (y-block, z, batch entry): 32-bit unsigned arithmetic (item
+// counts stay far below 2^31; 64-bit division would be emulated in scalar
+// code ahead of the tile's first load).
+__device__ __forceinline__ void tile_decode(unsigned item, unsigned nyb, unsigned H2, int& yb, int& z, int& b) {
+  const unsigned q = item / nyb;
+  yb = (int)(item - q * nyb);
+  const unsigned bb = q / H2;
+  z = (int)(q - bb * H2);
+  b = (int)bb;
+}
+
 // ---- P1: fold + E + FFT x --------------------------------------------------
 // tile item = (b, z, y-block); LDS A[c][yy][x] with row stride H0 + 1.
 template <typename T, typename TW, int L0>
 __device__ void ph_fold_fftx(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds) {
   const int H0 = p.H[0], H1 = p.H[1], H2 = p.H[2], YB = p.YB;
   const int nyb = H1 / YB;
-  const int yb = (int)(item % nyb);
-  const int z = (int)((item / nyb) % H2);
-  const int b = (int)(item / ((long)nyb * H2));
+  int yb, z, b;
+  tile_decode((unsigned)item, (unsigned)nyb, (unsigned)H2, yb, z, b);
   const int y0 = yb * YB;
   const long M3 = (long)p.M[0] * p.M[1] * p.M[2];
   const T* fb = static_cast<const T*>(B.f0_in) + b * M3;
-  const T* Eb = pipe_E<T>(p, b, step, M3);
+  const double* Eb = pipe_E(p, b, step, M3);
   const cx<double>* twM = static_cast<const cx<double>*>(B.twM);
   const int rs = H0 + 1;
   constexpr int NPT = TilePts<L0>::NPT;
@@ -298,7 +317,7 @@ __device__ void ph_fold_fftx(const PipeP& p, const PipeBufs& B, long item, int s
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const long idx = corner_index(p, x, y, z, e);
-      v[k][e] = Eb ? fb[idx] * Eb[idx] : fb[idx];
+      v[k][e] = Eb ? (T)((double)fb[idx] * Eb[idx]) : fb[idx];
     }
   }
 #pragma unroll
@@ -538,43 +557,59 @@ __device__ __forceinline__ void unit_lines(int u, int off, int& a, int& b, bool&
 // F_d[s][q] = sum_{a in {0,1}} K_d(q + M_d a) exp(-2 pi i (q + M_d a) t_{s,d} / N_d)
 // (1/8 folded into F0).  Octets partition the item's elements and each thread
 // owns whole octets, so the multiply runs in place between the forward and
-// the inverse transforms.
+// the inverse transforms.  The multiply is FP64 in both precisions: factors
+// rounded to FP32 would scale every mode by the same wrong amount on every
+// contour step (about 1e-7 per step at the k = 0 mode, so 1e-5 in Q after
+// 100 steps); in FP64 only the rounding of each result to the data type
+// remains, which changes from step to step.
 template <typename T, typename TW, int L, int NT, bool SEP>
-__device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds, cx<TW>* fq) {
+__device__ __forceinline__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds, cx<double>* fq) {
+  using TQ = double;  // arithmetic of the Q multiply
   constexpr int rs = kRow<T, L>, NP = L * L, SL = L * rs, NF = 2 + 2 * L;
   constexpr int NFT = 8 * NF, FIT = (NFT + NT - 1) / NT;
-  const int b = (int)(item / p.qs_items);
-  int r = (int)(item - (long)b * p.qs_items);
-  int c = 0;
-  for (; c < p.n_ch - 1; ++c) {
-    const int nc = (p.offs[c] & 1) ? (L + 1) / 2 : L / 2 + 1;  // orbits of x -> xbar
-    if (r < nc) break;
-    r -= nc;
+  // item -> (b, c, x0) in 32-bit arithmetic; the channel walk reads the
+  // offsets at static indices, so they load together ahead of it
+  const unsigned it = (unsigned)item, nq = (unsigned)p.qs_items;
+  const int b = (int)(it / nq);
+  int r = (int)(it - (unsigned)b * nq);
+  // selects, no branches and no indexing by c afterwards (an offset array
+  // indexed at run time would be copied to scratch memory)
+  int c = 0, off = p.offs[0];
+#pragma unroll
+  for (int cc = 0; cc < 7; ++cc) {
+    const int nc = (p.offs[cc] & 1) ? (L + 1) / 2 : L / 2 + 1;  // orbits of x -> xbar
+    const bool adv = c == cc && cc < p.n_ch - 1 && r >= nc;
+    r = adv ? r - nc : r;
+    c = adv ? cc + 1 : c;
+    off = adv ? p.offs[cc + 1] : off;
   }
-  const int o0 = p.offs[c] & 1, o1 = (p.offs[c] >> 1) & 1, o2 = (p.offs[c] >> 2) & 1;
+  const int o0 = off & 1, o1 = (off >> 1) & 1, o2 = (off >> 2) & 1;
   const int x0 = r, x1 = o0 ? L - 1 - r : (L - r) % L;
   const int ns = x1 == x0 ? 1 : 2;
   cx<T>* chan = static_cast<cx<T>*>(B.ch) + ((long)b * p.n_ch + c) * ((long)L * NP);
   // this item's axis factors: fq[s * NF + j], j = 0, 1: x0, x1; 2 + y; 2 + L + z
   const cx<double>* F = static_cast<const cx<double>*>(B.qsep);  // FP64 in both precisions
   const int Ms = p.M[0];
-  cx<TW> fv[FIT];
+  // The item's axis factors go straight from global memory into LDS
+  // (global_load_lds: no registers held across the row transforms, nothing
+  // to spill or to wait for ahead of the row loads); the barrier after the
+  // row transforms waits for them.  fq holds FIT * NT entries: the lanes past
+  // the last factor reload the last one into the padding.
+  typedef __attribute__((address_space(3))) unsigned char lds_byte;
+  static_assert(sizeof(cx<double>) == 16, "one 16-byte LDS DMA load per factor");
 #pragma unroll
   for (int k = 0; k < FIT; ++k) {
     const int i0 = threadIdx.x + k * NT;
-    const int i = i0 < NFT ? i0 : NFT - 1;  // unconditional load, clamped index
+    const int i = i0 < NFT ? i0 : NFT - 1;  // clamped source index
     const int sf = i / NF, j = i - sf * NF;
     const int d = j < 2 ? 0 : (j < 2 + L ? 1 : 2);
     const int q = j < 2 ? 2 * (j ? x1 : x0) + o0 : (j < 2 + L ? 2 * (j - 2) + o1 : 2 * (j - 2 - L) + o2);
-    fv[k] = tab<TW>(F, (d * 8 + sf) * Ms + q);
+    const int slot0 = k * NT + ((int)threadIdx.x / 64) * 64;  // the wave's first slot
+    lds_byte* dst = (lds_byte*)fq + 16 * slot0;
+    __builtin_amdgcn_global_load_lds((const void*)(F + (d * 8 + sf) * Ms + q), dst, 16, 0, 0);
   }
   // FFT y: the rows of both planes, loaded straight into the transform lanes
   rows_fft_from_global<T, TW, L, -1, NT, L, rs>(chan + (long)x0 * NP, chan + (long)x1 * NP, ns * L, lds);
-#pragma unroll
-  for (int k = 0; k < FIT; ++k) {
-    const int i = threadIdx.x + k * NT;
-    if (i < NFT) fq[i] = fv[k];
-  }
   __syncthreads();
   lds_fft2<T, TW, L, -1, NT>(lds, L, 1, ns, SL, rs);      // FFT z: columns of both planes
   if constexpr (SEP) {
@@ -590,17 +625,17 @@ __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx
       unit_lines<L>(un % NU, o1, ya, yb, yp);
       unit_lines<L>(un / NU, o2, za, zb, zp);
       int pos[8];
-      cx<TW> u[8];
+      cx<TQ> u[8];
 #pragma unroll
       for (int o = 0; o < 8; ++o) {
         pos[o] = ((o & 1) && ns == 2 ? SL : 0) + ((o & 4) ? zb : za) * rs + ((o & 2) ? yb : ya);
         const cx<T> g = lds[pos[o]];
-        u[o] = {(TW)g.x, (TW)g.y};
+        u[o] = {(TQ)g.x, (TQ)g.y};
       }
-      const cx<TW> gz[2][2] = {{fq[2 + L + za], fq[4 * NF + 2 + L + za]}, {fq[2 + L + zb], fq[4 * NF + 2 + L + zb]}};
-      const cx<TW> gy[2][2] = {{fq[2 + ya], fq[2 * NF + 2 + ya]}, {fq[2 + yb], fq[2 * NF + 2 + yb]}};
-      const cx<TW> gx[2][2] = {{fq[0], fq[NF]}, {fq[1], fq[NF + 1]}};
-      cx<TW> w[8];
+      const cx<TQ> gz[2][2] = {{fq[2 + L + za], fq[4 * NF + 2 + L + za]}, {fq[2 + L + zb], fq[4 * NF + 2 + L + zb]}};
+      const cx<TQ> gy[2][2] = {{fq[2 + ya], fq[2 * NF + 2 + ya]}, {fq[2 + yb], fq[2 * NF + 2 + yb]}};
+      const cx<TQ> gx[2][2] = {{fq[0], fq[NF]}, {fq[1], fq[NF + 1]}};
+      cx<TQ> w[8];
 #pragma unroll
       for (int o = 0; o < 8; ++o)
         w[o] = cadd(cmul(gz[o >> 2][0], u[o]), cmul(gz[o >> 2][1], csel(zp, u[o ^ 4], u[o])));
@@ -610,7 +645,7 @@ __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx
       // a single plane (x0 = x1) loaded its entries twice: u[o ^ 1] is then u[o]'s twin
 #pragma unroll
       for (int o = 0; o < 8; ++o) {
-        const cx<TW> a = cadd(cmul(gx[o & 1][0], u[o]), cmul(gx[o & 1][1], u[o ^ 1]));
+        const cx<TQ> a = cadd(cmul(gx[o & 1][0], u[o]), cmul(gx[o & 1][1], u[o ^ 1]));
         lds[pos[o]] = {(T)a.x, (T)a.y};
       }
     }
@@ -631,9 +666,9 @@ __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx
         pos[o] = ((o & 1) && ns == 2 ? SL : 0) + ((o & 4) ? zb : za) * rs + ((o & 2) ? yb : ya);
         v[o] = lds[pos[o]];
       }
-      cx<TW> acc[8];
+      cx<TQ> acc[8];
 #pragma unroll
-      for (int o = 0; o < 8; ++o) acc[o] = {TW(0), TW(0)};
+      for (int o = 0; o < 8; ++o) acc[o] = {TQ(0), TQ(0)};
       // sign patterns in four classes by their y and z flips; within a class
       // the entries src[k] = v[k with the flips of self-paired axes undone]
       // are read at the static index o ^ sf
@@ -655,17 +690,17 @@ __device__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx
 #pragma unroll
         for (int xb = 0; xb < 2; ++xb) {
           const int sf = 2 * cls + xb;
-          const cx<TW>* f = fq + sf * NF;
-          const cx<TW> fx0 = f[0], fx1 = f[1];
-          const cx<TW> fy0 = f[2 + ya], fy1 = f[2 + yb];
-          const cx<TW> fz0 = f[2 + L + za], fz1 = f[2 + L + zb];
-          const cx<TW> fxy[4] = {cmul(fx0, fy0), cmul(fx1, fy0), cmul(fx0, fy1), cmul(fx1, fy1)};
+          const cx<TQ>* f = fq + sf * NF;
+          const cx<TQ> fx0 = f[0], fx1 = f[1];
+          const cx<TQ> fy0 = f[2 + ya], fy1 = f[2 + yb];
+          const cx<TQ> fz0 = f[2 + L + za], fz1 = f[2 + L + zb];
+          const cx<TQ> fxy[4] = {cmul(fx0, fy0), cmul(fx1, fy0), cmul(fx0, fy1), cmul(fx1, fy1)};
 #pragma unroll
           for (int o = 0; o < 8; ++o) {
-            const cx<TW> qc = cmul(fxy[o & 3], (o & 4) ? fz1 : fz0);
+            const cx<TQ> qc = cmul(fxy[o & 3], (o & 4) ? fz1 : fz0);
             const cx<T> g = src[o ^ sf];
-            acc[o].x += qc.x * (TW)g.x - qc.y * (TW)g.y;
-            acc[o].y += qc.x * (TW)g.y + qc.y * (TW)g.x;
+            acc[o].x += qc.x * (TQ)g.x - qc.y * (TQ)g.y;
+            acc[o].y += qc.x * (TQ)g.y + qc.y * (TQ)g.x;
           }
         }
       }
@@ -703,17 +738,16 @@ template <typename T, typename TW, int L0>
 __device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds) {
   const int H0 = p.H[0], H1 = p.H[1], H2 = p.H[2], YB = p.YB;
   const int nyb = H1 / YB;
-  const int yb = (int)(item % nyb);
-  const int z = (int)((item / nyb) % H2);
-  const int b = (int)(item / ((long)nyb * H2));
+  int yb, z, b;
+  tile_decode((unsigned)item, (unsigned)nyb, (unsigned)H2, yb, z, b);
   const int y0 = yb * YB;
   const int rs = H0 + 1;
   const long M3 = (long)p.M[0] * p.M[1] * p.M[2];
   T* fb = static_cast<T*>(B.f0_out) + b * M3;
-  const T* Eb = pipe_E<T>(p, b, step, M3);
+  const double* Eb = pipe_E(p, b, step, M3);
   const cx<double>* twM = static_cast<const cx<double>*>(B.twM);
   constexpr int NPT = TilePts<L0>::NPT;
-  T ev[NPT][8];
+  double ev[NPT][8];
   FoldTw<TW> ft[NPT];
 #pragma unroll
   for (int k = 0; k < NPT; ++k) {
@@ -722,7 +756,7 @@ __device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, in
     const int x = pt % H0, y = y0 + pt / H0;
     ft[k] = load_fold_tw<TW>(twM, x, y, z, p);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) ev[k][e] = Eb ? Eb[corner_index(p, x, y, z, e)] : T(1);
+    for (int e = 0; e < 8; ++e) ev[k][e] = Eb ? Eb[corner_index(p, x, y, z, e)] : 1.0;
   }
   tile_to_lds<T, L0>(p, static_cast<const cx<T>*>(B.ch), b, z, y0, lds);
   __syncthreads();
@@ -738,7 +772,7 @@ __device__ void ph_ifftx_unfold(const PipeP& p, const PipeBufs& B, long item, in
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const T r = (T)((TW)v[e] * p.scale_unfold);
-      fb[corner_index(p, x, y, z, e)] = Eb ? r * ev[k][e] : r;
+      fb[corner_index(p, x, y, z, e)] = Eb ? (T)((double)r * ev[k][e]) : r;
     }
   }
   __syncthreads();
@@ -837,19 +871,18 @@ template <typename T, typename TW, int L0, bool STORE>
 __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds) {
   const int H0 = p.H[0], H1 = p.H[1], H2 = p.H[2], YB = p.YB;
   const int nyb = H1 / YB;
-  const int yb = (int)(item % nyb);
-  const int z = (int)((item / nyb) % H2);
-  const int b = (int)(item / ((long)nyb * H2));
+  int yb, z, b;
+  tile_decode((unsigned)item, (unsigned)nyb, (unsigned)H2, yb, z, b);
   const int y0 = yb * YB;
   constexpr int rs = kRow<T, L0>;  // H0 = L0
   cx<T>* ch = static_cast<cx<T>*>(B.ch);
   const long M3 = (long)p.M[0] * p.M[1] * p.M[2];
   T* fb = STORE ? static_cast<T*>(B.f0_out) + b * M3 : nullptr;
-  const T* Eo = pipe_E<T>(p, b, step, M3);
-  const T* En = pipe_E<T>(p, b, step + 1, M3);
+  const double* Eo = pipe_E(p, b, step, M3);
+  const double* En = pipe_E(p, b, step + 1, M3);
   const cx<double>* twM = static_cast<const cx<double>*>(B.twM);
   constexpr int NPT = TilePts<L0>::NPT;
-  T eo[NPT][8], en[NPT][8];
+  double eo[NPT][8], en[NPT][8];
   FoldTw<TW> ft[NPT];
   const bool same_E = En == Eo;
 #pragma unroll
@@ -861,8 +894,8 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const long idx = corner_index(p, x, y, z, e);
-      eo[k][e] = Eo ? Eo[idx] : T(1);
-      en[k][e] = same_E ? eo[k][e] : (En ? En[idx] : T(1));
+      eo[k][e] = Eo ? Eo[idx] : 1.0;
+      en[k][e] = same_E ? eo[k][e] : (En ? En[idx] : 1.0);
     }
   }
   // With one tile point per thread and 64 % H0 = 0 each wave's points are
@@ -890,11 +923,11 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       T r = (T)((TW)v[e] * p.scale_unfold);
-      if (Eo) r *= eo[k][e];
+      if (Eo) r = (T)((double)r * eo[k][e]);
       // a stored sweep's slice is read back only by the density pass, after
       // the sweep: a streaming store keeps it out of the caches the chain uses
       if constexpr (STORE) __builtin_nontemporal_store(r, fb + corner_index(p, x, y, z, e));
-      v[e] = En ? r * en[k][e] : r;
+      v[e] = En ? (T)((double)r * en[k][e]) : r;
     }
     wht8(v);
     // the entries of this thread's point only: no barrier needed before reuse
@@ -1006,8 +1039,9 @@ constexpr int kQNT = 256;
 // that each gets its own register allocation.
 template <typename T, typename TW, int L, bool SEP>
 __global__ void __launch_bounds__(kQNT) k_p2q(PipeP p, PipeBufs B) {
+  constexpr int NFQ = (8 * (2 + 2 * L) + kQNT - 1) / kQNT * kQNT;  // whole loads of kQNT factors
   __shared__ cx<T> lds[2 * L * kRow<T, L>];
-  __shared__ cx<TW> fq[8 * (2 + 2 * L)];
+  __shared__ cx<double> fq[NFQ];  // axis factors, FP64 in both precisions
   ph_plane_pair_q<T, TW, L, kQNT, SEP>(p, B, blockIdx.x, lds, fq);
 }
 

@@ -56,10 +56,9 @@ __global__ void k_fill(T* __restrict__ x, T v, long n) {
 }
 
 // E = exp(factor * w) from FP64 w into the plan's precision.
-template <typename T>
-__global__ void k_expw(const double* __restrict__ w, T* __restrict__ E, double factor, long n) {
+__global__ void k_expw(const double* __restrict__ w, double* __restrict__ E, double factor, long n) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i < n) E[i] = (T)exp(factor * w[i]);
+  if (i < n) E[i] = exp(factor * w[i]);
 }
 
 // Unnormalised densities of one instance from the stored propagators (slice j
@@ -239,6 +238,13 @@ struct cfft_scft {
   double ds = 0.0;
   double lam = 1.0, lam_simple = 0.1, simple_above = 1e-2;
   hipStream_t stream = nullptr;
+  // Densities projected onto the symmetric fields before the field update
+  // (mixed precision on the per-element Q paths: there FP32 rounding of each
+  // element is independent, the propagators leave the symmetric subspace by
+  // rounding-sized amounts and the fields would carry that from iteration to
+  // iteration; the other paths pass every step through the spectral ASU).
+  bool project = false;
+  DevBuf phi_raw;  // unprojected densities of one instance (2 M^3)
   // shared by the instances: propagator slices [ns + 1][2 n_inst fields],
   // interaction factors [n_inst][E_A, E_B], reduction partials (reused in
   // stream order), per-instance totals and their pinned host copy
@@ -275,10 +281,10 @@ template <typename T>
 void sweep_all(cfft_scft* c, const std::vector<char>& active) {
   const long M3 = c->M3, n2 = 2 * M3;
   const int ni = (int)c->in.size();
-  T* E = static_cast<T*>(c->E.p);
+  double* E = static_cast<double*>(c->E.p);  // interaction factors: FP64 in both precisions
   for (int i = 0; i < ni; ++i) {
     if (!active[i]) continue;  // an idle instance keeps its last factors
-    hipLaunchKernelGGL(k_expw<T>, dim3(nblk(n2, 256)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_expw, dim3(nblk(n2, 256)), dim3(256), 0, c->stream,
                        static_cast<const double*>(c->in[i]->w.p), E + i * n2, -0.5 * c->ds, n2);
     hip_check(hipGetLastError(), "k_expw");
   }
@@ -289,8 +295,8 @@ void sweep_all(cfft_scft* c, const std::vector<char>& active) {
   std::vector<const void*> E1(2 * ni), E2(2 * ni);
   std::vector<int32_t> sw(2 * ni);
   for (int i = 0; i < ni; ++i) {
-    const T* Ea = E + i * n2;
-    const T* Eb = Ea + M3;
+    const double* Ea = E + i * n2;
+    const double* Eb = Ea + M3;
     E1[2 * i] = Ea;
     E2[2 * i] = Eb;
     sw[2 * i] = c->in[i]->nA;
@@ -310,11 +316,15 @@ void update_inst(cfft_scft* c, int i) {
   const long M3 = c->M3;
   const unsigned nb = red_blocks(M3);
   double* tot = static_cast<double*>(c->tot.p) + (long)i * kTot;
+  double* phi = static_cast<double*>(c->project ? c->phi_raw.p : s.phi.p);
   hipLaunchKernelGGL(k_density<T>, dim3(nb), dim3(kRB), 0, c->stream, static_cast<const T*>(c->hist.p), M3,
                      2 * (int)c->in.size(), 2 * i, c->ns, s.nA, static_cast<const double*>(s.wa.p),
-                     static_cast<const double*>(s.wb.p), static_cast<double*>(s.phi.p),
-                     static_cast<double*>(c->dpart.p));
+                     static_cast<const double*>(s.wb.p), phi, static_cast<double*>(c->dpart.p));
   hip_check(hipGetLastError(), "k_density");
+  if (c->project) {  // the projection keeps every sum over the grid (each op permutes the points)
+    const int rc = cfft_symmetrize_f64_dev(c->plan, phi, s.phi.p, 2, c->stream);
+    if (rc != CFFT_OK) throw Error{rc, g_last_error};
+  }
   hipLaunchKernelGGL(k_total<2>, dim3(2), dim3(kRB), 0, c->stream, static_cast<const double*>(c->dpart.p),
                      (int)nb, tot);
   hip_check(hipGetLastError(), "k_total");
@@ -523,6 +533,8 @@ cfft_scft* create(cfft_plan* plan, int32_t n_inst, const double* f, const double
   std::unique_ptr<cfft_scft> c(new cfft_scft());
   c->plan = plan;
   c->prec = info[11];
+  c->project = c->prec == CFFT_MIXED &&
+               (info[10] == CFFT_PATH_QS || info[10] == CFFT_PATH_M2Q || info[10] == CFFT_PATH_M7Q);
   c->M3 = (long)info[0] * info[1] * info[2];
   c->esz = c->prec == CFFT_FP64 ? 8 : 4;
   c->ns = ns;
@@ -531,10 +543,11 @@ cfft_scft* create(cfft_plan* plan, int32_t n_inst, const double* f, const double
   hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
   const long n2 = 2 * c->M3;
   c->hist.alloc(c->esz * n2 * n_inst * (ns + 1));
-  c->E.alloc(c->esz * n2 * n_inst);
+  c->E.alloc(sizeof(double) * n2 * n_inst);
   c->part.alloc(8 * kMaxRed * kNU);
   c->dpart.alloc(8 * kMaxRed * 2);
   c->tot.alloc(8 * (size_t)kTot * n_inst);
+  if (c->project) c->phi_raw.alloc(8 * n2);
   hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->tot_h), sizeof(double) * kTot * n_inst,
                           hipHostMallocDefault),
             "hipHostMalloc");

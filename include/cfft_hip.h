@@ -20,7 +20,9 @@
  *     SpectralIndexing.points.
  *   - Host-pointer entry points are blocking and use FP64 at the boundary.
  *     *_dev entry points take device pointers of the plan's precision (double
- *     for CFFT_FP64, float for CFFT_MIXED) and a hipStream_t; the work runs
+ *     for CFFT_FP64, float for CFFT_MIXED) for fields, except the interaction
+ *     factors E = exp(-w dt/2), which are double in both precisions, and a
+ *     hipStream_t; the work runs
  *     on exactly that stream (NULL = the null stream, which is PyTorch's
  *     default stream), in order with the caller's other work on it, and the
  *     call returns without synchronising.
@@ -187,8 +189,8 @@ int cfft_diffuse_dev(cfft_plan* plan, void* f0, void* stream);
  * device (two streams, two threads) race and give wrong results.  The array
  * needs no clearing (each call writes its buffers before reading them). */
 int cfft_diffuse_noalloc_dev(cfft_plan* plan, void* f0, void* stream);
-/* E = exp(-w dt/2) per batch entry (batch x M^3, plan precision), applied on
- * both sides of D: q <- E D (E q). */
+/* E = exp(-w dt/2) per batch entry (batch x M^3, FP64 in both precisions),
+ * applied on both sides of D: q <- E D (E q), each product formed in FP64. */
 int cfft_step_dev(cfft_plan* plan, void* q, const void* E, int32_t nsteps, void* stream);
 /* As cfft_step_dev with one factor array per batch entry: E is a host array
  * of `batch` device pointers (batch <= 64); the SCFT sweep uses it to switch
@@ -203,7 +205,15 @@ int cfft_step_ptrs_dev(cfft_plan* plan, void* q, const void* const* E, int32_t n
  * pipeline phases. */
 int cfft_sweep_dev(cfft_plan* plan, void* q, const void* const* E1, const void* const* E2,
                    const int32_t* sw, int32_t step0, int32_t nsteps, void* stream);
-/* E <- exp(-w * factor), n entries of the plan precision. */
+/* Projection of n_fields FP64 subgrid fields (M^3 each, x fastest, one after
+ * another) onto the fields symmetric under the plan's ops that map the
+ * stride-L subgrid onto itself: out(x) = mean over those ops g of in(g x).
+ * in and out must differ.  No counterpart in the reference (its symmetric
+ * test fields are averaged over the ops on the full grid,
+ * test/test_helpers.jl:11-29); the SCFT context uses it on the densities in
+ * mixed precision on paths QS, M2Q and M7Q. */
+int cfft_symmetrize_f64_dev(cfft_plan* plan, const void* in, void* out, int64_t n_fields, void* stream);
+/* E <- exp(-w * factor), n entries: w of the plan precision, E FP64. */
 int cfft_exp_dev(cfft_plan* plan, const void* w, void* E, double factor, int64_t n, void* stream);
 
 /* ---- per-stage timer (hipEvents around each kernel launch) ------------- */
@@ -265,6 +275,12 @@ int cfft_sweep_hist_dev(cfft_plan* plan, void* hist, const void* const* E1, cons
  * H1 (CFFT_ERR_INVALID else).  Results do not depend on it; see
  * cfft_hip.tune for timing the choices on a workload. */
 #define CFFT_OPT_TILE_ROWS 4
+/* Mixed precision on path QS: 0 (default) forms the transforms' twiddle
+ * products and the fold/unfold factors in FP64 (data and butterflies FP32),
+ * 1 in FP32 (about 4 % faster per step; the roundings of those factors then
+ * repeat on every contour step and bias F by about 1e-6).  The plane-pair Q
+ * multiply and the interaction factors are FP64 either way. */
+#define CFFT_OPT_FP32_TWIDDLES 5
 /* No counterpart in the reference. */
 int cfft_plan_set_option(cfft_plan* plan, int32_t key, int64_t value);
 
