@@ -22,7 +22,8 @@ FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = [os.path.join(CSRC, f) for f in ("cfft_hip.hip", "blocks.hip", "scft.hip", "mlp.hip", "plan.cpp")]
+SOURCES = [os.path.join(CSRC, f) for f in ("cfft_hip.hip", "blocks.hip", "scft.hip", "mlp.hip", "dense.hip",
+                                            "plan.cpp")]
 
 
 def _run(cmd, verbose):

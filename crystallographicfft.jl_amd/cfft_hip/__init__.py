@@ -36,5 +36,6 @@ from .families import (FractalCFFTPlan, G0ASUBackwardPlan, G0ASUPlan, GeneralG0A
                        plan_krfft_g0asu, plan_krfft_g0asu_backward, plan_krfft_g0asu_general,
                        plan_krfft_recursive, plan_krfft_selective, plan_krfft_sparse,
                        plan_staged_pm3m, plan_staged_pm3m_full)
-from .diffusion_solver import (DiffusionSolver, QFusedDiffusionSolver, apply_diffusion_operator,
-                               interaction_step, plan_diffusion, step_diffusion)
+from .diffusion_solver import (DiffusionSolver, MatrixDiffusionSolver, QFusedDiffusionSolver,
+                               apply_diffusion_operator, calc_transform_matrix, interaction_step,
+                               plan_diffusion, step_diffusion)

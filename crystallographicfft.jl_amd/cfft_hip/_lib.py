@@ -38,7 +38,8 @@ EXPORTED = [
     "cfft_update_kernel", "cfft_plan_destroy", "cfft_forward", "cfft_backward",
     "cfft_forward_g0", "cfft_diffuse", "cfft_step", "cfft_sub_to_full", "cfft_full_to_sub",
     "cfft_forward_dev", "cfft_backward_dev", "cfft_diffuse_dev", "cfft_diffuse_noalloc_dev", "cfft_step_dev",
-    "cfft_symmetrize_f64_dev",
+    "cfft_symmetrize_f64_dev", "cfft_dense_create", "cfft_dense_apply", "cfft_dense_step",
+    "cfft_dense_destroy",
     "cfft_exp_dev", "cfft_sync", "cfft_step_ptrs_dev", "cfft_profile", "cfft_profile_read",
     "cfft_time_stage_dev", "cfft_sweep_dev", "cfft_plan_set_option",
     "cfft_calc_asu", "cfft_pack_asu", "cfft_block_plan_create", "cfft_block_plan_info",
@@ -116,6 +117,10 @@ def _declare(lib):
         "cfft_diffuse_dev": (C.c_int, [P, P, P]),
         "cfft_diffuse_noalloc_dev": (C.c_int, [P, P, P]),
         "cfft_symmetrize_f64_dev": (C.c_int, [P, P, P, I64, P]),
+        "cfft_dense_create": (C.c_int, [I64, PD, P]),
+        "cfft_dense_apply": (C.c_int, [P, PD]),
+        "cfft_dense_step": (C.c_int, [P, PD, PD, D]),
+        "cfft_dense_destroy": (C.c_int, [P]),
         "cfft_step_dev": (C.c_int, [P, P, P, I32, P]),
         "cfft_exp_dev": (C.c_int, [P, P, P, D, I64, P]),
         "cfft_sync": (C.c_int, [P]),

@@ -321,6 +321,21 @@ int cfft_block_forward(cfft_bplan* plan, const double* in, double* out);
 int cfft_block_inverse(cfft_bplan* plan, const double* in, double* out);
 int cfft_block_plan_destroy(cfft_bplan* plan);
 
+/* ---- dense real-ASU diffusion operator ------------------------------------ */
+typedef struct cfft_dense cfft_dense;
+
+/* MatrixDiffusionSolver (src/diffusion_solver.jl:20-71), plan_diffusion's
+ * method=:matrix: the host builds A = Re(M diag(K) M^+) over the n real-ASU
+ * points (M the real-ASU x spectral-ASU transform matrix, :29-60; M^+ its
+ * pseudo-inverse, :175); A (n x n FP64, row-major) is held on the device. */
+int cfft_dense_create(int64_t n, const double* A, cfft_dense** out);
+/* apply_diffusion_operator!(solver, u) (:66-71) for a real u of n values, in
+ * place (u <- A u). */
+int cfft_dense_apply(cfft_dense* op, double* u);
+/* step_diffusion!(solver, u, w, dt) (:254-282): u <- e^{-w dt/2} A e^{-w dt/2} u. */
+int cfft_dense_step(cfft_dense* op, double* u, const double* w, double dt);
+int cfft_dense_destroy(cfft_dense* op);
+
 /* ---- full-grid centring pre-fold KRFFT ------------------------------------ */
 typedef struct cfft_pplan cfft_pplan;
 

@@ -114,8 +114,9 @@ def test_plan_errors_before_device_use():
         cf.DevicePlan(N, ops_s, 0.01, None, 17)
     assert e.value.code == _lib.CFFT_ERR_UNKNOWN_METHOD
     with pytest.raises(cf.CFFTError) as e:
-        cf.plan_diffusion(N, np.eye(3), 47, 3, 0.01, method="matrix")
+        cf.plan_diffusion(N, np.eye(3), 47, 3, 0.01, method="krfft")
     assert e.value.code == _lib.CFFT_ERR_UNKNOWN_METHOD
+    assert "Unknown method: krfft" in str(e.value)
     # N not divisible by L (src/krfft.jl:309-311)
     with pytest.raises(cf.CFFTError) as e:
         cf.build_fill_map(ops_s, [2, 2, 2], (15, 16, 16))
