@@ -396,7 +396,6 @@ def batched_instances(cf, n, seed, workers):
                                       finish_tol=1e-8)
     r64, p64, s64 = batch.solve_batch(insts, precision=cf.FP64, tol=1e-8, max_iter=400, workers=workers)
     dF = np.array([abs(a["F"] - b["F"]) for a, b in zip(r32, r64)])
-    dQ = np.array([abs(a["Q"] - b["Q"]) / abs(b["Q"]) for a, b in zip(r32, r64)])
     dFf = np.array([abs(a["F"] - b["F"]) for a, b in zip(rmf, r64)])
     return {
         "instances": n, "workers": workers,
@@ -404,7 +403,6 @@ def batched_instances(cf, n, seed, workers):
                   "converged": int(sum(a["res"] < 1e-5 for a in r32)),
                   "iters_mean": float(np.mean([a["iters"] for a in r32])),
                   "abs_dF_vs_fp64": {"max": float(dF.max()), "median": float(np.median(dF))},
-                  "rel_dQ_vs_fp64": {"max": float(dQ.max()), "median": float(np.median(dQ))},
                   "other_solution": int(np.sum(dF > 1e-3))},
         "mixed_then_fp64": {"tol": 1e-8, "instances_per_s": n / smf, "solve_s": smf, "plan_s": pmf,
                             "converged": int(sum(a["res"] < 1e-8 for a in rmf)),
@@ -493,7 +491,10 @@ def learned_warm_start(n_eval, workers):
     out = W.summary(rows)
     out["grid"] = grid
     out["trained_on"] = ws.meta.get("n_train")
-    out["full_evaluation"] = ws.meta.get("eval")
+    # the 200-set evaluation recorded in the model file when it was trained
+    # (not measured in this run; the lines above are)
+    out["recorded_at_training"] = {"source": "models/warmstart.npz metadata, not this run",
+                                   "evaluation": ws.meta.get("eval")}
     return out
 
 
