@@ -395,6 +395,14 @@ def batched_instances(cf, n, seed, workers):
     rmf, pmf, smf = batch.solve_batch(insts, precision=cf.MIXED, tol=1e-5, max_iter=400, workers=workers,
                                       finish_tol=1e-8)
     r64, p64, s64 = batch.solve_batch(insts, precision=cf.FP64, tol=1e-8, max_iter=400, workers=workers)
+    # the same solves with the groups advanced together (one multi-plan sweep
+    # per iteration, cfft_hip.scft.SCFTEnsemble)
+    re32, pe32, se32 = batch.solve_batch(insts, precision=cf.MIXED, tol=1e-5, max_iter=400, workers=workers,
+                                         ensemble=True)
+    re64, pe64, se64 = batch.solve_batch(insts, precision=cf.FP64, tol=1e-8, max_iter=400, workers=workers,
+                                         ensemble=True)
+    in_ens = sum(bool(r.get("ensemble")) for r in re64)
+    strip = lambda rs: [{k: v for k, v in r.items() if k != "ensemble"} for r in rs]
     dF = np.array([abs(a["F"] - b["F"]) for a, b in zip(r32, r64)])
     dFf = np.array([abs(a["F"] - b["F"]) for a, b in zip(rmf, r64)])
     return {
@@ -413,6 +421,11 @@ def batched_instances(cf, n, seed, workers):
                  "converged": int(sum(a["res"] < 1e-8 for a in r64)),
                  "iters_mean": float(np.mean([a["iters"] for a in r64]))},
         "groups": len(batch.groups_of(insts)),
+        "fp64_ensemble": {"instances_per_s": n / se64, "solve_s": se64, "plan_s": pe64,
+                          "instances_in_ensemble": in_ens, "same_results": strip(re64) == r64,
+                          "speedup_vs_grouped": s64 / se64},
+        "mixed_ensemble": {"instances_per_s": n / se32, "solve_s": se32, "plan_s": pe32,
+                           "same_results": strip(re32) == r32, "speedup_vs_grouped": s32 / se32},
         "fp64_context_per_instance": {"instances_per_s": n / su, "solve_s": su, "plan_s": pu,
                                       "same_results": ru == r64, "grouped_speedup": su / s64},
         # each group's launch schedule (tile rows, graph replay) timed on its

@@ -49,6 +49,8 @@ EXPORTED = [
     "cfft_scft_set_fields", "cfft_scft_set_params", "cfft_scft_iterate", "cfft_scft_solve", "cfft_scft_get",
     "cfft_scft_propagators", "cfft_scft_destroy", "cfft_scft_create_multi", "cfft_scft_set_params_inst",
     "cfft_scft_set_fields_inst", "cfft_scft_iterate_multi", "cfft_scft_solve_multi", "cfft_scft_get_inst",
+    "cfft_scft_ensemble_create", "cfft_scft_ensemble_iterate", "cfft_scft_ensemble_solve",
+    "cfft_scft_ensemble_destroy",
     "cfft_mlp_create", "cfft_mlp_info",
     "cfft_mlp_get_params", "cfft_mlp_set_params", "cfft_mlp_forward", "cfft_mlp_gradient",
     "cfft_mlp_train", "cfft_mlp_destroy",
@@ -157,6 +159,10 @@ def _declare(lib):
         "cfft_scft_iterate_multi": (C.c_int, [P, PD]),
         "cfft_scft_solve_multi": (C.c_int, [P, D, I32, PI32, PD]),
         "cfft_scft_get_inst": (C.c_int, [P, I32, I32, PD]),
+        "cfft_scft_ensemble_create": (C.c_int, [C.POINTER(P), I32, C.POINTER(P)]),
+        "cfft_scft_ensemble_iterate": (C.c_int, [P, PD]),
+        "cfft_scft_ensemble_solve": (C.c_int, [P, D, I32, PI32, PD]),
+        "cfft_scft_ensemble_destroy": (C.c_int, [P]),
         "cfft_mlp_create": (C.c_int, [I32, PI32, C.c_uint64, C.POINTER(P)]),
         "cfft_mlp_info": (C.c_int, [P, PI64]),
         "cfft_mlp_get_params": (C.c_int, [P, PF]),

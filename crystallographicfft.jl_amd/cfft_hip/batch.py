@@ -7,7 +7,11 @@ iterations are those of a one-instance solver bit for bit.  Each group has its
 own stream, its contour sweeps are replayed from captured hipGraphs, and a
 pool of host threads drives the groups (every library call releases the GIL),
 so the groups' kernels run concurrently.  ``group=False`` gives every instance
-its own plan, context and stream instead.  No counterpart in the reference.
+its own plan, context and stream instead.  ``ensemble=True`` advances the
+groups together instead (``SCFTEnsemble``): one launch per chain stage and
+contour step covers every group's plane pairs or tiles, the sweep replayed
+from one hipGraph, each instance's results unchanged bit for bit.  No
+counterpart in the reference.
 """
 from __future__ import annotations
 
@@ -19,7 +23,7 @@ import numpy as np
 
 from . import _lib as L
 from .plans import auto_L, find_optimal_shift
-from .scft import SCFTBatch, SCFTSolver, default_path
+from .scft import SCFTBatch, SCFTEnsemble, SCFTSolver, default_path
 from .symmetry_ops import get_ops
 
 
@@ -201,8 +205,18 @@ def _finish(r, it2, o2, out):
     r.update(iters_finish=it2, F=o2["F"], res=o2["res"], Q=o2["Q"], F_first=out["F"], res_first=out["res"])
 
 
+ENSEMBLE_LENS = (8, 12, 16, 24, 32)  # channel transform lengths of the ensemble kernels
+
+
+def ensemble_ok(plan):
+    """Whether a plan can join an ensemble: path QS, a cubic channel grid of a
+    supported length (the ensemble runs the FP64-twiddle kernels)."""
+    return (plan.path == L.PATH_QS and plan.H[0] == plan.H[1] == plan.H[2]
+            and plan.H[0] in ENSEMBLE_LENS)
+
+
 def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8, m_hist=10,
-                finish_tol=None, group=True, tune=False):
+                finish_tol=None, group=True, tune=False, ensemble=False):
     """Plan every instance (or group of instances), then solve all of them
     concurrently.  With finish_tol, each instance continues in FP64 from its
     `precision` solution until the residual is below finish_tol (its FP64
@@ -210,9 +224,13 @@ def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8,
     ``scft.default_path`` for the stage's tolerance.  With tune (grouped
     only), each group's launch schedule is timed on its own workload
     (``tune.tune_schedule``, one group after another, in the planning phase)
-    and the fastest applied; results[i]["schedule"] names it.  Returns (results, plan_seconds,
-    solve_seconds); results[i] = dict(iters, F, res, Q[, iters_finish]) in the
-    order of `instances`."""
+    and the fastest applied; results[i]["schedule"] names it.  With ensemble
+    (grouped only), the groups whose plans qualify (``ensemble_ok``) run as
+    one ``SCFTEnsemble`` per stage (and one for the FP64 finish), the others
+    as before, concurrently with it; tile-height schedules from the tuner
+    apply to both.  Returns (results, plan_seconds, solve_seconds);
+    results[i] = dict(iters, F, res, Q[, iters_finish]) in the order of
+    `instances`."""
     finish = finish_tol is not None
     t0 = time.perf_counter()
     if group:
@@ -262,14 +280,47 @@ def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8,
                 _finish(r, it2, o2, out)
         return rs
 
+    def run_ensemble(items):
+        solvers = [sv for sv, _, _ in items]
+        ens = SCFTEnsemble(solvers)
+        for sv, _, starts in items:
+            for i, (wA, wB) in enumerate(starts):
+                sv.set_fields(i, wA, wB)
+        res = ens.solve(tol=tol, max_iter=max_iter)
+        rss = [[_result(it, out) for it, out in zip(its, outs)] for its, outs in res]
+        if finish:
+            fins = [fin for _, fin, _ in items]
+            ens2 = SCFTEnsemble(fins)
+            for sv, fin, _ in items:
+                for i in range(sv.n):
+                    fin.set_fields(i, *sv.fields(i))
+            res2 = ens2.solve(tol=finish_tol, max_iter=max_iter)
+            for rs, (its2, outs2), (_, outs) in zip(rss, res2, res):
+                for r, it2, o2, out in zip(rs, its2, outs2, outs):
+                    _finish(r, it2, o2, out)
+        return rss
+
     with ThreadPoolExecutor(max_workers=workers) as ex:
         if group:
             results = [None] * len(instances)
-            for g, sch, rs in zip(groups, schedules, ex.map(run_group, prepared)):
-                for k, r in zip(g, rs):
-                    if sch is not None:
-                        r["schedule"] = sch
+            ens_idx = []
+            if ensemble:
+                ens_idx = [j for j, (sv, fin, _) in enumerate(prepared)
+                           if ensemble_ok(sv.plan) and (fin is None or ensemble_ok(fin.plan))]
+            rest = [j for j in range(len(prepared)) if j not in set(ens_idx)]
+            fut = ex.submit(run_ensemble, [prepared[j] for j in ens_idx]) if ens_idx else None
+            for j, rs in zip(rest, ex.map(run_group, [prepared[j] for j in rest])):
+                for k, r in zip(groups[j], rs):
+                    if schedules[j] is not None:
+                        r["schedule"] = schedules[j]
                     results[k] = r
+            if fut is not None:
+                for j, rs in zip(ens_idx, fut.result()):
+                    for k, r in zip(groups[j], rs):
+                        if schedules[j] is not None:
+                            r["schedule"] = schedules[j]
+                        r["ensemble"] = True
+                        results[k] = r
         else:
             results = list(ex.map(run, prepared))
     t2 = time.perf_counter()

@@ -320,3 +320,53 @@ class SCFTBatch:
 
     def densities(self, i):
         return self._get(i, L.SCFT_PHI)
+
+
+class SCFTEnsemble:
+    """Several ``SCFTBatch`` contexts (any grids, space groups and instance
+    counts; one precision and Ns; path QS with channel grids H in {8, 12, 16,
+    24, 32}) advanced together through ``cfft_scft_ensemble_*``: per
+    iteration one launch per chain stage and contour step covers the plane
+    pairs (or tiles) of every context, from a work list, and the sweep is
+    replayed from one captured hipGraph.  Per instance the results are those
+    of its own context bit for bit.  The batches must outlive the ensemble."""
+
+    def __init__(self, batches):
+        self.batches = list(batches)
+        if not self.batches:
+            raise L.CFFTError(L.CFFT_ERR_INVALID, "need at least one context")
+        arr = (C.c_void_p * len(self.batches))(*[b._h.value for b in self.batches])
+        h = C.c_void_p()
+        L.check(L.lib().cfft_scft_ensemble_create(arr, len(self.batches), C.byref(h)))
+        self._h = h
+        self.n = sum(b.n for b in self.batches)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                L.lib().cfft_scft_ensemble_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def _split(self, o, it=None):
+        out, k = [], 0
+        for b in self.batches:
+            outs = b._outs(o[8 * k:8 * (k + b.n)])
+            out.append(outs if it is None else ([int(v) for v in it[k:k + b.n]], outs))
+            k += b.n
+        return out
+
+    def iterate(self):
+        """One iteration of every instance: per context, a list of result dicts."""
+        o = np.zeros(8 * self.n)
+        L.check(L.lib().cfft_scft_ensemble_iterate(self._h, L.f64p(o)))
+        return self._split(o)
+
+    def solve(self, tol=1e-8, max_iter=1000):
+        """Per context, (iterations per instance, result dict per instance)."""
+        o = np.zeros(8 * self.n)
+        it = np.zeros(self.n, dtype=np.int32)
+        L.check(L.lib().cfft_scft_ensemble_solve(self._h, float(tol), int(max_iter), L.i32p(it), L.f64p(o)))
+        return self._split(o, it)

@@ -18,6 +18,7 @@
 #include "../../include/cfft_hip.h"
 #include "kernels.hpp"
 #include "pipeline.hpp"
+#include "ensemble.hpp"
 #include "plan.hpp"
 #include "common.hpp"
 
@@ -1256,6 +1257,47 @@ void step_host(cfft_plan* p, double* q, const double* w, double dt, int nsteps) 
 }
 
 }  // namespace
+
+// ---- what a multi-plan sweep needs of a plan ---------------------------------
+namespace cfft {
+
+EnsPlanInfo ens_plan_info(cfft_plan* p) {
+  check_plan(p);
+  EnsPlanInfo i{};
+  i.path = p->path;
+  i.prec = p->prec;
+  i.arith32 = fp32_arith(p) ? 1 : 0;
+  i.device = p->device;
+  i.L = p->H[0];
+  i.cubic = cubic(p) ? 1 : 0;
+  i.batch = p->batch;
+  i.sep = p->qs_sep ? 1 : 0;
+  i.pair_items = (long)p->qs_items * p->batch;
+  i.tile_items = n_tiles(p, pipe_params(p, ESel{}));
+  i.M3 = p->M3;
+  return i;
+}
+
+// The same launch parameters and buffers as the plan's own history sweep
+// (cfft_sweep_hist_dev), so an ensemble computes what the plan computes.
+void ens_slot_params(cfft_plan* p, const EnsPart& part, EnsSlot* out) {
+  check_plan(p);
+  require((int)part.E1.size() == p->batch && (int)part.E2.size() == p->batch && (int)part.sw.size() == p->batch,
+          CFFT_ERR_INVALID, "one pair of interaction factors and one switch step per field");
+  ESel es;
+  es.mode = 2;
+  es.E1 = part.E1.data();
+  es.E2 = part.E2.data();
+  es.sw = part.sw.data();
+  std::memset(static_cast<void*>(out), 0, sizeof(EnsSlot));
+  out->p = pipe_params(p, es);
+  out->B = pipe_bufs(p, nullptr, nullptr, p->d_F.p, false);
+  out->q = part.hist;
+  out->stride = p->M3 * p->batch;
+  out->L = p->H[0];
+}
+
+}  // namespace cfft
 
 // ============================================================================
 // extern "C" entry points

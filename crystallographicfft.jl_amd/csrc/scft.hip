@@ -18,6 +18,7 @@
 
 #include "../../include/cfft_hip.h"
 #include "common.hpp"
+#include "ensemble.hpp"
 #include "lstsq.hpp"
 
 using namespace cfft;
@@ -275,57 +276,69 @@ void set_weights(cfft_scft* c, ScftInst& s, bool sync) {
   if (sync) hip_check(hipStreamSynchronize(c->stream), "sync");  // the host vectors go out of scope
 }
 
-// Sweep of every instance (q, q+ of instance i are fields 2i, 2i + 1):
-// E = exp(-ds w / 2) of the active instances, slice 0 = 1, then ns steps.
+// Before a sweep of every instance (q, q+ of instance i are fields 2i,
+// 2i + 1): E = exp(-ds w / 2) of the active instances and slice 0 = 1, on
+// stream st; `part` receives the sweep's per-field factors and switch steps.
 template <typename T>
-void sweep_all(cfft_scft* c, const std::vector<char>& active) {
+void sweep_prepare(cfft_scft* c, const std::vector<char>& active, hipStream_t st, EnsPart& part) {
   const long M3 = c->M3, n2 = 2 * M3;
   const int ni = (int)c->in.size();
   double* E = static_cast<double*>(c->E.p);  // interaction factors: FP64 in both precisions
   for (int i = 0; i < ni; ++i) {
     if (!active[i]) continue;  // an idle instance keeps its last factors
-    hipLaunchKernelGGL(k_expw, dim3(nblk(n2, 256)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_expw, dim3(nblk(n2, 256)), dim3(256), 0, st,
                        static_cast<const double*>(c->in[i]->w.p), E + i * n2, -0.5 * c->ds, n2);
     hip_check(hipGetLastError(), "k_expw");
   }
-  hipLaunchKernelGGL(k_fill<T>, dim3(nblk(ni * n2, 256)), dim3(256), 0, c->stream, static_cast<T*>(c->hist.p),
-                     T(1), ni * n2);
+  hipLaunchKernelGGL(k_fill<T>, dim3(nblk(ni * n2, 256)), dim3(256), 0, st, static_cast<T*>(c->hist.p), T(1),
+                     ni * n2);
   hip_check(hipGetLastError(), "k_fill");
   // q of instance i uses E_A on its first nA steps, q+ E_B on its first ns - nA
-  std::vector<const void*> E1(2 * ni), E2(2 * ni);
-  std::vector<int32_t> sw(2 * ni);
+  part.plan = c->plan;
+  part.hist = c->hist.p;
+  part.E1.assign(2 * ni, nullptr);
+  part.E2.assign(2 * ni, nullptr);
+  part.sw.assign(2 * ni, 0);
   for (int i = 0; i < ni; ++i) {
     const double* Ea = E + i * n2;
     const double* Eb = Ea + M3;
-    E1[2 * i] = Ea;
-    E2[2 * i] = Eb;
-    sw[2 * i] = c->in[i]->nA;
-    E1[2 * i + 1] = Eb;
-    E2[2 * i + 1] = Ea;
-    sw[2 * i + 1] = c->ns - c->in[i]->nA;
+    part.E1[2 * i] = Ea;
+    part.E2[2 * i] = Eb;
+    part.sw[2 * i] = c->in[i]->nA;
+    part.E1[2 * i + 1] = Eb;
+    part.E2[2 * i + 1] = Ea;
+    part.sw[2 * i + 1] = c->ns - c->in[i]->nA;
   }
-  const int rc = cfft_sweep_hist_dev(c->plan, c->hist.p, E1.data(), E2.data(), sw.data(), 0, c->ns, c->stream);
+}
+
+// Sweep of every instance on the context's own stream.
+template <typename T>
+void sweep_all(cfft_scft* c, const std::vector<char>& active) {
+  EnsPart part;
+  sweep_prepare<T>(c, active, c->stream, part);
+  const int rc = cfft_sweep_hist_dev(c->plan, c->hist.p, part.E1.data(), part.E2.data(), part.sw.data(), 0, c->ns,
+                                     c->stream);
   if (rc != CFFT_OK) throw Error{rc, g_last_error};
 }
 
 // Densities, field update and Gram row of instance i (device only; totals to
 // tot + i kTot).
 template <typename T>
-void update_inst(cfft_scft* c, int i) {
+void update_inst(cfft_scft* c, int i, hipStream_t st) {
   ScftInst& s = *c->in[i];
   const long M3 = c->M3;
   const unsigned nb = red_blocks(M3);
   double* tot = static_cast<double*>(c->tot.p) + (long)i * kTot;
   double* phi = static_cast<double*>(c->project ? c->phi_raw.p : s.phi.p);
-  hipLaunchKernelGGL(k_density<T>, dim3(nb), dim3(kRB), 0, c->stream, static_cast<const T*>(c->hist.p), M3,
+  hipLaunchKernelGGL(k_density<T>, dim3(nb), dim3(kRB), 0, st, static_cast<const T*>(c->hist.p), M3,
                      2 * (int)c->in.size(), 2 * i, c->ns, s.nA, static_cast<const double*>(s.wa.p),
                      static_cast<const double*>(s.wb.p), phi, static_cast<double*>(c->dpart.p));
   hip_check(hipGetLastError(), "k_density");
   if (c->project) {  // the projection keeps every sum over the grid (each op permutes the points)
-    const int rc = cfft_symmetrize_f64_dev(c->plan, phi, s.phi.p, 2, c->stream);
+    const int rc = cfft_symmetrize_f64_dev(c->plan, phi, s.phi.p, 2, st);
     if (rc != CFFT_OK) throw Error{rc, g_last_error};
   }
-  hipLaunchKernelGGL(k_total<2>, dim3(2), dim3(kRB), 0, c->stream, static_cast<const double*>(c->dpart.p),
+  hipLaunchKernelGGL(k_total<2>, dim3(2), dim3(kRB), 0, st, static_cast<const double*>(c->dpart.p),
                      (int)nb, tot);
   hip_check(hipGetLastError(), "k_total");
   // the newest history slot and the Gram row over the ring
@@ -337,11 +350,11 @@ void update_inst(cfft_scft* c, int i) {
   gs.n = (int)s.order.size();
   gs.cur = slot;
   for (int k = 0; k < gs.n; ++k) gs.slot[k] = s.order[k];
-  launch_update_gram(dim3(nb), c->stream, static_cast<const double*>(s.w.p), static_cast<double*>(s.phi.p),
+  launch_update_gram(dim3(nb), st, static_cast<const double*>(s.w.p), static_cast<double*>(s.phi.p),
                      static_cast<double*>(s.d.p), static_cast<const double*>(tot), c->ds, s.chiN, M3,
                      static_cast<double*>(s.Wr.p), static_cast<double*>(s.Dr.p), gs, static_cast<double*>(c->part.p));
   hip_check(hipGetLastError(), "k_update_gram");
-  hipLaunchKernelGGL(k_total<kNU>, dim3(kNU), dim3(kRB), 0, c->stream, static_cast<const double*>(c->part.p),
+  hipLaunchKernelGGL(k_total<kNU>, dim3(kNU), dim3(kRB), 0, st, static_cast<const double*>(c->part.p),
                      (int)nb, tot + 2);
   hip_check(hipGetLastError(), "k_total");
 }
@@ -349,7 +362,7 @@ void update_inst(cfft_scft* c, int i) {
 // Host part of instance i's iteration from its totals: Q, F, residual, Gram
 // row; then the mixing kernel (left in flight on the context's stream).
 // out = {Q, Q+, F, residual, <phiA + phiB>, <phiA>, mixing, k}.
-void finish_inst(cfft_scft* c, int i, double* out) {
+void finish_inst(cfft_scft* c, int i, double* out, hipStream_t st) {
   ScftInst& s = *c->in[i];
   const long M3 = c->M3, n2 = 2 * M3;
   const double* t = c->tot_h + (long)i * kTot;
@@ -399,7 +412,7 @@ void finish_inst(cfft_scft* c, int i, double* out) {
       mode = 1.0;
     }
   }
-  hipLaunchKernelGGL(k_mix, dim3(nblk(n2, 256)), dim3(256), 0, c->stream, static_cast<double*>(s.w.p),
+  hipLaunchKernelGGL(k_mix, dim3(nblk(n2, 256)), dim3(256), 0, st, static_cast<double*>(s.w.p),
                      static_cast<const double*>(s.Wr.p), static_cast<const double*>(s.Dr.p), mp, n2);
   hip_check(hipGetLastError(), "k_mix");
   ++s.iters;
@@ -408,33 +421,19 @@ void finish_inst(cfft_scft* c, int i, double* out) {
   if (out) std::memcpy(out, o, sizeof(o));
 }
 
-// One SCFT iteration of the active instances: one sweep of all of them, then
-// per instance densities, update and Gram row on the device; one host
-// synchronisation for all their totals; host mixing coefficients and the
-// mixing kernels.  out[8 i ..] receives instance i's results (active only).
-void iterate(cfft_scft* c, const std::vector<char>& active, double* out) {
+// Host part of the iteration of the active instances of c once their totals
+// are in c->tot_h.  An instance whose iteration fails (partition function
+// not positive and finite) is marked failed, its out[] reads {NaN, NaN, NaN,
+// +inf, NaN, NaN, 0, 0} and the others go on; n_act and n_failed count the
+// instances finished and failed, `first` is the first failure.
+void finish_active(cfft_scft* c, const std::vector<char>& active, double* out, hipStream_t st, int& n_act,
+                   int& n_failed, Error& first) {
   const int ni = (int)c->in.size();
-  if (c->prec == CFFT_FP64) sweep_all<double>(c, active);
-  else sweep_all<float>(c, active);
-  for (int i = 0; i < ni; ++i) {
-    if (!active[i]) continue;
-    if (c->prec == CFFT_FP64) update_inst<double>(c, i);
-    else update_inst<float>(c, i);
-  }
-  hip_check(hipMemcpyAsync(c->tot_h, c->tot.p, sizeof(double) * kTot * ni, hipMemcpyDeviceToHost, c->stream),
-            "D2H totals");
-  hip_check(hipStreamSynchronize(c->stream), "sync");
-  // every active instance is finished.  An instance whose iteration fails
-  // (partition function not positive and finite) is marked failed, its
-  // out[] reads {NaN, NaN, NaN, +inf, NaN, NaN, 0, 0} and the others go on;
-  // the call fails only when every instance it iterated failed.
-  int n_act = 0, n_failed = 0;
-  Error first{CFFT_OK, ""};
   for (int i = 0; i < ni; ++i) {
     if (!active[i]) continue;
     ++n_act;
     try {
-      finish_inst(c, i, out ? out + 8 * i : nullptr);
+      finish_inst(c, i, out ? out + 8 * i : nullptr, st);
     } catch (const Error& e) {
       if (n_failed == 0) first = e;
       ++n_failed;
@@ -446,6 +445,28 @@ void iterate(cfft_scft* c, const std::vector<char>& active, double* out) {
       if (out) std::memcpy(out + 8 * i, o, sizeof(o));
     }
   }
+}
+
+// One SCFT iteration of the active instances: one sweep of all of them, then
+// per instance densities, update and Gram row on the device; one host
+// synchronisation for all their totals; host mixing coefficients and the
+// mixing kernels.  out[8 i ..] receives instance i's results (active only);
+// the call fails only when every instance it iterated failed.
+void iterate(cfft_scft* c, const std::vector<char>& active, double* out) {
+  const int ni = (int)c->in.size();
+  if (c->prec == CFFT_FP64) sweep_all<double>(c, active);
+  else sweep_all<float>(c, active);
+  for (int i = 0; i < ni; ++i) {
+    if (!active[i]) continue;
+    if (c->prec == CFFT_FP64) update_inst<double>(c, i, c->stream);
+    else update_inst<float>(c, i, c->stream);
+  }
+  hip_check(hipMemcpyAsync(c->tot_h, c->tot.p, sizeof(double) * kTot * ni, hipMemcpyDeviceToHost, c->stream),
+            "D2H totals");
+  hip_check(hipStreamSynchronize(c->stream), "sync");
+  int n_act = 0, n_failed = 0;
+  Error first{CFFT_OK, ""};
+  finish_active(c, active, out, c->stream, n_act, n_failed, first);
   if (n_failed > 0) g_last_error = first.msg;
   if (n_act > 0 && n_failed == n_act) throw first;
 }
@@ -583,6 +604,147 @@ cfft_scft* single(cfft_scft* c) {
 
 }  // namespace
 
+// Several contexts advanced together: per iteration one multi-plan sweep of
+// all of them (per class of transform length, one launch per chain stage and
+// contour step), then each instance's densities, update and mixing as in its
+// own context; each context's work goes to its class's stream, and one host
+// synchronisation per iteration covers all of them.
+struct cfft_scft_ensemble {
+  std::vector<cfft_scft*> ctx;
+  std::vector<EnsPart> parts;
+  EnsSweep* sweep = nullptr;
+  int prec = 0;
+  ~cfft_scft_ensemble() {
+    if (sweep) ens_sweep_destroy(sweep);  // waits for its streams
+  }
+};
+
+namespace {
+
+// One iteration of the active instances of every context (act[k][i]);
+// out[8 j ..] receives the results of instance j in context order (active
+// only).  Work stays in flight on the class streams after the return (the
+// mixing kernels).
+void ens_iterate(cfft_scft_ensemble* e, const std::vector<std::vector<char>>& act, double* out) {
+  const int nk = (int)e->ctx.size();
+  std::vector<char> any(nk, 0);
+  for (int k = 0; k < nk; ++k)
+    for (char a : act[k]) any[k] = any[k] || a;
+  for (int k = 0; k < nk; ++k) {
+    if (!any[k]) continue;
+    hipStream_t st = ens_part_stream(e->sweep, k);
+    if (e->prec == CFFT_FP64) sweep_prepare<double>(e->ctx[k], act[k], st, e->parts[k]);
+    else sweep_prepare<float>(e->ctx[k], act[k], st, e->parts[k]);
+  }
+  std::vector<std::vector<char>> fields(nk);  // q, q+ of instance i are fields 2i, 2i + 1
+  for (int k = 0; k < nk; ++k) {
+    fields[k].assign(2 * act[k].size(), 0);
+    for (size_t i = 0; i < act[k].size(); ++i) fields[k][2 * i] = fields[k][2 * i + 1] = act[k][i];
+  }
+  ens_sweep_run(e->sweep, e->parts, fields);
+  for (int k = 0; k < nk; ++k) {
+    if (!any[k]) continue;
+    cfft_scft* c = e->ctx[k];
+    hipStream_t st = ens_part_stream(e->sweep, k);
+    for (int i = 0; i < (int)c->in.size(); ++i) {
+      if (!act[k][i]) continue;
+      if (e->prec == CFFT_FP64) update_inst<double>(c, i, st);
+      else update_inst<float>(c, i, st);
+    }
+    hip_check(hipMemcpyAsync(c->tot_h, c->tot.p, sizeof(double) * kTot * c->in.size(), hipMemcpyDeviceToHost, st),
+              "D2H totals");
+  }
+  ens_sweep_sync(e->sweep);
+  int n_act = 0, n_failed = 0;
+  Error first{CFFT_OK, ""};
+  long off = 0;
+  for (int k = 0; k < nk; ++k) {
+    cfft_scft* c = e->ctx[k];
+    if (any[k])
+      finish_active(c, act[k], out ? out + 8 * off : nullptr, ens_part_stream(e->sweep, k), n_act, n_failed, first);
+    off += (long)c->in.size();
+  }
+  if (n_failed > 0) g_last_error = first.msg;
+  if (n_act > 0 && n_failed == n_act) throw first;
+}
+
+// Work of the contexts' own streams completes before the ensemble's stream
+// touches their buffers.
+void ens_enter(cfft_scft_ensemble* e) {
+  for (cfft_scft* c : e->ctx) hip_check(hipStreamSynchronize(c->stream), "sync");
+}
+
+void ens_solve(cfft_scft_ensemble* e, double tol, int32_t max_iter, int32_t* iters, double* out) {
+  const int nk = (int)e->ctx.size();
+  std::vector<std::vector<char>> act(nk);
+  std::vector<std::vector<int>> it(nk);
+  long ntot = 0;
+  for (int k = 0; k < nk; ++k) {
+    const int ni = (int)e->ctx[k]->in.size();
+    act[k].assign(ni, 0);
+    it[k].assign(ni, 0);
+    for (int i = 0; i < ni; ++i) act[k][i] = max_iter > 0 && !e->ctx[k]->in[i]->failed;
+    ntot += ni;
+  }
+  std::vector<double> o(8 * ntot, 0.0);
+  for (;;) {
+    bool go = false;
+    for (int k = 0; k < nk; ++k)
+      for (char a : act[k]) go = go || a;
+    if (!go) break;
+    ens_iterate(e, act, o.data());
+    long off = 0;
+    for (int k = 0; k < nk; ++k) {
+      cfft_scft* c = e->ctx[k];
+      for (int i = 0; i < (int)c->in.size(); ++i) {
+        if (act[k][i]) {
+          ++it[k][i];
+          if (c->in[i]->failed || o[8 * (off + i) + 3] < tol || it[k][i] >= max_iter) act[k][i] = 0;
+        }
+      }
+      off += (long)c->in.size();
+    }
+  }
+  long off = 0;
+  for (int k = 0; k < nk; ++k) {
+    for (int i = 0; i < (int)e->ctx[k]->in.size(); ++i) {
+      if (iters) iters[off + i] = it[k][i];
+      if (out) std::memcpy(out + 8 * (off + i), o.data() + 8 * (off + i), 8 * sizeof(double));
+    }
+    off += (long)e->ctx[k]->in.size();
+  }
+}
+
+cfft_scft_ensemble* ens_create(cfft_scft* const* ctx, int32_t n) {
+  require(ctx != nullptr && n >= 1 && n <= kEnsMax, CFFT_ERR_INVALID, "an ensemble holds 1 to 64 contexts");
+  std::unique_ptr<cfft_scft_ensemble> e(new cfft_scft_ensemble());
+  for (int k = 0; k < n; ++k) {
+    cfft_scft* c = ctx[k];
+    require(c != nullptr, CFFT_ERR_INVALID, "null context");
+    for (int j = 0; j < k; ++j) require(ctx[j] != c, CFFT_ERR_INVALID, "a context appears twice");
+    if (k == 0) e->prec = c->prec;
+    require(c->prec == e->prec && c->ns == ctx[0]->ns, CFFT_ERR_INVALID,
+            "the contexts of an ensemble share precision and contour length");
+    e->ctx.push_back(c);
+  }
+  // the sweep's factor pointers and switch steps, as sweep_prepare sets them
+  e->parts.resize(n);
+  for (int k = 0; k < n; ++k) {
+    cfft_scft* c = ctx[k];
+    const int ni = (int)c->in.size();
+    EnsPart& p = e->parts[k];
+    p.plan = c->plan;
+    p.hist = c->hist.p;
+    p.E1.assign(2 * ni, c->E.p);
+    p.E2.assign(2 * ni, c->E.p);
+    p.sw.assign(2 * ni, 0);
+  }
+  e->sweep = ens_sweep_create(e->parts, ctx[0]->ns);
+  return e.release();
+}
+
+}  // namespace
+
 extern "C" {
 
 int cfft_scft_create(cfft_plan* plan, double f, double chiN, int32_t ns, int32_t m_hist, cfft_scft** out) {
@@ -670,6 +832,48 @@ int cfft_scft_propagators(cfft_scft* c, void** hist_out) {
     require(c && hist_out, CFFT_ERR_INVALID, "null argument");
     hip_check(hipStreamSynchronize(c->stream), "sync");  // the last iteration's work is complete
     *hist_out = c->hist.p;
+  });
+}
+
+int cfft_scft_ensemble_create(cfft_scft* const* ctx, int32_t n_ctx, cfft_scft_ensemble** out) {
+  return guarded([&] {
+    require(out != nullptr, CFFT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    *out = ens_create(ctx, n_ctx);
+  });
+}
+
+int cfft_scft_ensemble_iterate(cfft_scft_ensemble* e, double* out) {
+  return guarded([&] {
+    require(e != nullptr, CFFT_ERR_INVALID, "null ensemble");
+    ens_enter(e);
+    std::vector<std::vector<char>> act(e->ctx.size());
+    bool any = false;
+    for (size_t k = 0; k < e->ctx.size(); ++k) {
+      const cfft_scft* c = e->ctx[k];
+      act[k].assign(c->in.size(), 0);
+      for (size_t i = 0; i < c->in.size(); ++i) any = (act[k][i] = !c->in[i]->failed) || any;
+    }
+    require(any, CFFT_ERR_INVALID, "every instance of the ensemble has failed; set new fields");
+    ens_iterate(e, act, out);
+    ens_sweep_sync(e->sweep);
+  });
+}
+
+int cfft_scft_ensemble_solve(cfft_scft_ensemble* e, double tol, int32_t max_iter, int32_t* iters_out, double* out) {
+  return guarded([&] {
+    require(e != nullptr && max_iter >= 0, CFFT_ERR_INVALID, "bad argument");
+    ens_enter(e);
+    ens_solve(e, tol, max_iter, iters_out, out);
+    ens_sweep_sync(e->sweep);
+  });
+}
+
+int cfft_scft_ensemble_destroy(cfft_scft_ensemble* e) {
+  return guarded([&] {
+    if (!e) return;
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
+    delete e;
   });
 }
 

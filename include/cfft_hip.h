@@ -415,6 +415,31 @@ int cfft_scft_iterate_multi(cfft_scft* ctx, double* out);
 int cfft_scft_solve_multi(cfft_scft* ctx, double tol, int32_t max_iter, int32_t* iters_out, double* out);
 int cfft_scft_get_inst(cfft_scft* ctx, int32_t inst, int32_t which, double* out);
 
+/* Several contexts advanced together (no counterpart in the reference; the
+ * reference's only parallelism is threads over blocks, src/execution.jl:20,
+ * 50).  The contexts may differ in grid, space group, cell and instance
+ * count; they share precision and ns, and each plan must be on path QS with
+ * a cubic channel grid of length H in {8, 12, 16, 24, 32} (grids 32^3 to
+ * 128^3 at L = 2) and FP64 twiddle arithmetic (else CFFT_ERR_UNSUPPORTED).
+ * The contexts fall into classes by H; per iteration and class, one launch
+ * per chain stage and contour step covers the plane pairs (or tiles) of
+ * every context of the class with an active instance, from a work list.
+ * Each class replays its sweep from one hipGraph on a stream of its own, so
+ * the classes run concurrently, and each context's density, update and
+ * mixing work follows on its class's stream.  Each instance's
+ * results equal those of its own context bit for bit.  The ensemble borrows
+ * the contexts: they must outlive it, and while an ensemble call runs no
+ * other call may use them.  out and iters_out hold one entry block per
+ * instance, contexts in the order given, instances in context order
+ * (out[8] per instance as cfft_scft_iterate_multi). */
+typedef struct cfft_scft_ensemble cfft_scft_ensemble;
+int cfft_scft_ensemble_create(cfft_scft* const* ctx, int32_t n_ctx, cfft_scft_ensemble** out);
+int cfft_scft_ensemble_iterate(cfft_scft_ensemble* e, double* out);
+/* As cfft_scft_solve_multi over every instance of every context. */
+int cfft_scft_ensemble_solve(cfft_scft_ensemble* e, double tol, int32_t max_iter, int32_t* iters_out,
+                             double* out);
+int cfft_scft_ensemble_destroy(cfft_scft_ensemble* e);
+
 /* ---- warm-start network (multilayer perceptron) ---------------------------- */
 /* No counterpart in the reference.  dims[0..n_layers]: input width, hidden
  * widths, output width.  SiLU hidden activations, linear output.  Parameters:
