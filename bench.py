@@ -375,43 +375,42 @@ def batched_fields(cf, torch, ops_s, wA, prec, e, batch, steps, path):
 def batched_instances(cf, n, seed, workers):
     """64 (default) independent instances, sizes {32,48,64,96}^3 and groups
     {47,70,223,229,230} drawn from a seed, f ~ U(.25,.45), chiN ~ U(12,30),
-    Ns 100, each started from its group's lowest non-extinct star: solved
-    concurrently in mixed precision (FP32 propagators, FP64 reductions) to
-    residual 1e-5, then finished in FP64 to 1e-8; plain FP64 to 1e-8 from the
-    same starts for the per-instance errors and the like-for-like time.
-    Instances of one grid and space group share a context (one sweep for all
-    of them); FP64 with a context per instance beside it."""
+    Ns 100, each started from its group's lowest non-extinct star: solved in
+    mixed precision (FP32 propagators, FP64 reductions) to residual 1e-5,
+    then finished in FP64 to 1e-8; plain FP64 to 1e-8 from the same starts
+    for the per-instance errors and the like-for-like time.  Instances of one
+    grid and space group share a context; the contexts run as one ensemble
+    (per transform-length class one launch per chain stage and contour step
+    over the active fields, cfft_hip.scft.SCFTEnsemble).  Beside it: the same
+    contexts each on its own stream (grouped), a context per instance, and
+    the tuned launch schedules."""
     from cfft_hip import batch
     insts = batch.draw_instances(n, seed)
-    ru, pu, su = batch.solve_batch(insts, precision=cf.FP64, tol=1e-8, max_iter=400, workers=workers,
-                                   group=False)
-    rt, pt, st = batch.solve_batch(insts, precision=cf.FP64, tol=1e-8, max_iter=400, workers=workers,
-                                   tune=True)
+    strip = lambda rs: [{k: v for k, v in r.items() if k not in ("ensemble", "schedule")} for r in rs]
+    solve = lambda prec, tol, **kw: batch.solve_batch(insts, precision=prec, tol=tol, max_iter=400,
+                                                      workers=workers, **kw)
+    ru, pu, su = solve(cf.FP64, 1e-8, group=False)
+    rg, pg, sg = solve(cf.FP64, 1e-8)
+    r32g, p32g, s32g = solve(cf.MIXED, 1e-5)
+    rt, pt, st = solve(cf.FP64, 1e-8, tune=True, ensemble=True)
     sched = {}
     for r in rt:
         sched[r["schedule"]] = sched.get(r["schedule"], 0) + 1
-    rt_plain = [{k: v for k, v in r.items() if k != "schedule"} for r in rt]
-    r32, p32, s32 = batch.solve_batch(insts, precision=cf.MIXED, tol=1e-5, max_iter=400, workers=workers)
-    rmf, pmf, smf = batch.solve_batch(insts, precision=cf.MIXED, tol=1e-5, max_iter=400, workers=workers,
-                                      finish_tol=1e-8)
-    r64, p64, s64 = batch.solve_batch(insts, precision=cf.FP64, tol=1e-8, max_iter=400, workers=workers)
-    # the same solves with the groups advanced together (one multi-plan sweep
-    # per iteration, cfft_hip.scft.SCFTEnsemble)
-    re32, pe32, se32 = batch.solve_batch(insts, precision=cf.MIXED, tol=1e-5, max_iter=400, workers=workers,
-                                         ensemble=True)
-    re64, pe64, se64 = batch.solve_batch(insts, precision=cf.FP64, tol=1e-8, max_iter=400, workers=workers,
-                                         ensemble=True)
-    in_ens = sum(bool(r.get("ensemble")) for r in re64)
-    strip = lambda rs: [{k: v for k, v in r.items() if k != "ensemble"} for r in rs]
+    r32, p32, s32 = solve(cf.MIXED, 1e-5, ensemble=True)
+    rmf, pmf, smf = solve(cf.MIXED, 1e-5, finish_tol=1e-8, ensemble=True)
+    r64, p64, s64 = solve(cf.FP64, 1e-8, ensemble=True)
+    in_ens = sum(bool(r.get("ensemble")) for r in r64)
+    r32, rmf, r64, rt = strip(r32), strip(rmf), strip(r64), strip(rt)
     dF = np.array([abs(a["F"] - b["F"]) for a, b in zip(r32, r64)])
     dFf = np.array([abs(a["F"] - b["F"]) for a, b in zip(rmf, r64)])
     return {
-        "instances": n, "workers": workers,
+        "instances": n, "workers": workers, "groups": len(batch.groups_of(insts)),
+        "instances_in_ensemble": in_ens, "mixing": dict(batch.BATCH_MIXING),
         "mixed": {"tol": 1e-5, "instances_per_s": n / s32, "solve_s": s32, "plan_s": p32,
                   "converged": int(sum(a["res"] < 1e-5 for a in r32)),
                   "iters_mean": float(np.mean([a["iters"] for a in r32])),
                   "abs_dF_vs_fp64": {"max": float(dF.max()), "median": float(np.median(dF))},
-                  "other_solution": int(np.sum(dF > 1e-3))},
+                  "other_solution": int(np.sum(dF > 1e-3)), "speedup_vs_fp64": s64 / s32},
         "mixed_then_fp64": {"tol": 1e-8, "instances_per_s": n / smf, "solve_s": smf, "plan_s": pmf,
                             "converged": int(sum(a["res"] < 1e-8 for a in rmf)),
                             "iters_mean": float(np.mean([a["iters"] + a["iters_finish"] for a in rmf])),
@@ -419,19 +418,19 @@ def batched_instances(cf, n, seed, workers):
                             "speedup_vs_fp64": s64 / smf},
         "fp64": {"tol": 1e-8, "instances_per_s": n / s64, "solve_s": s64, "plan_s": p64,
                  "converged": int(sum(a["res"] < 1e-8 for a in r64)),
-                 "iters_mean": float(np.mean([a["iters"] for a in r64]))},
-        "groups": len(batch.groups_of(insts)),
-        "fp64_ensemble": {"instances_per_s": n / se64, "solve_s": se64, "plan_s": pe64,
-                          "instances_in_ensemble": in_ens, "same_results": strip(re64) == r64,
-                          "speedup_vs_grouped": s64 / se64},
-        "mixed_ensemble": {"instances_per_s": n / se32, "solve_s": se32, "plan_s": pe32,
-                           "same_results": strip(re32) == r32, "speedup_vs_grouped": s32 / se32},
+                 "iters_mean": float(np.mean([a["iters"] for a in r64])),
+                 "iters_max": int(max(a["iters"] for a in r64))},
+        # the same contexts each on its own stream, host threads driving them
+        "fp64_grouped": {"instances_per_s": n / sg, "solve_s": sg, "plan_s": pg,
+                         "same_results": strip(rg) == r64, "ensemble_speedup": sg / s64},
+        "mixed_grouped": {"instances_per_s": n / s32g, "solve_s": s32g, "same_results": strip(r32g) == r32,
+                          "ensemble_speedup": s32g / s32},
         "fp64_context_per_instance": {"instances_per_s": n / su, "solve_s": su, "plan_s": pu,
-                                      "same_results": ru == r64, "grouped_speedup": su / s64},
+                                      "same_results": strip(ru) == r64, "ensemble_speedup": su / s64},
         # each group's launch schedule (tile rows, graph replay) timed on its
         # own workload and the fastest applied (cfft_hip.tune)
         "fp64_tuned": {"instances_per_s": n / st, "solve_s": st, "plan_and_tune_s": pt,
-                       "instances_by_schedule": sched, "same_results": rt_plain == r64,
+                       "instances_by_schedule": sched, "same_results": rt == r64,
                        "speedup_vs_untuned": s64 / st},
     }
 

@@ -215,8 +215,18 @@ def ensemble_ok(plan):
             and plan.H[0] in ENSEMBLE_LENS)
 
 
+# Mixing of the batch's contexts: Anderson steps from residual 0.1 down
+# (simple mixing, lam 0.1, above).  On the 64-instance batch of sizes
+# 32-96^3 and groups 47/70/223/229/230 every instance converges to 1e-8 in
+# 84.5 iterations on average (max 138), against 153 (one instance, Fddd 96^3
+# near its order-disorder transition, creeping under simple mixing and
+# unconverged after 400) with Anderson from 1e-2, the single solver's
+# default; the converged solutions are the same.
+BATCH_MIXING = dict(lam=1.0, lam_simple=0.1, simple_above=0.1)
+
+
 def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8, m_hist=10,
-                finish_tol=None, group=True, tune=False, ensemble=False):
+                finish_tol=None, group=True, tune=False, ensemble=False, mixing=BATCH_MIXING):
     """Plan every instance (or group of instances), then solve all of them
     concurrently.  With finish_tol, each instance continues in FP64 from its
     `precision` solution until the residual is below finish_tol (its FP64
@@ -228,7 +238,8 @@ def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8,
     (grouped only), the groups whose plans qualify (``ensemble_ok``) run as
     one ``SCFTEnsemble`` per stage (and one for the FP64 finish), the others
     as before, concurrently with it; tile-height schedules from the tuner
-    apply to both.  Returns (results, plan_seconds, solve_seconds);
+    apply to both.  mixing: keyword arguments of ``set_mixing`` for every
+    context (None: the contexts' own default).  Returns (results, plan_seconds, solve_seconds);
     results[i] = dict(iters, F, res, Q[, iters_finish]) in the order of
     `instances`."""
     finish = finish_tol is not None
@@ -253,6 +264,11 @@ def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8,
     else:
         with ThreadPoolExecutor(max_workers=workers) as ex:
             prepared = list(ex.map(lambda inst: _prepare(inst, precision, m_hist, finish, tol), instances))
+    if mixing:
+        for sv, fin, _ in prepared:
+            for x in (sv, fin):
+                if x is not None:
+                    x.set_mixing(**mixing)
     t1 = time.perf_counter()
 
     def run(item):
