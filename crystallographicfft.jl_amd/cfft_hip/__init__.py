@@ -22,8 +22,8 @@ from .plans import (ASUPoint, CenteredKRFFTBackwardPlan, CenteredKRFFTPlan, Cent
                     plan_centered_scft, plan_krfft, plan_krfft_centered, plan_m2_backward,
                     plan_m2_q, plan_m2_scft, plan_m7_scft, select_rep_ops, subgrid_to_fullgrid,
                     update_kernel, update_m2_kernel)
-from .asu import (ASUBlock, CFFTPlan, CrystallographicASU, calc_asu, ldiv, mul, pack_asu,
-                  plan_cfft)
+from .asu import (ASUBlock, CFFTPlan, CrystallographicASU, InterleavedOrbit, analyze_interleaved_orbits,
+                  calc_asu, ldiv, mul, pack_asu, pack_asu_interleaved, plan_cfft)
 from . import scft  # noqa: F401
 from .families import (FractalCFFTPlan, G0ASUBackwardPlan, G0ASUPlan, GeneralG0ASUPlan,
                        RecursiveKRFFTPlan, SelectiveG0Plan, SparseKRFFTPlan, StagedPm3mPlan,

@@ -4,6 +4,8 @@ Reference counterparts:
   calc_asu(N, ops), calc_asu(sg_num, dim, N)   src/asu.jl:51-55, 216-325
   ASUBlock / CrystallographicASU                src/asu.jl:29-44
   pack_asu                                      src/pack_asu.jl:6-102
+  InterleavedOrbit / analyze_interleaved_orbits src/asu.jl:16-21, 331-431
+  pack_asu_interleaved (Mode B)                 src/pack_asu.jl:113-202
   CFFTPlan / plan_cfft                          src/cfft_plan.jl:4-40
   mul! / ldiv!  (here mul / ldiv)               src/cfft_plan.jl:43-66,
                                                 src/execution.jl:10-64
@@ -22,11 +24,12 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib as L
+from . import _lib as L_
 from .plans import ASUPoint, _N3, _pack_any, find_optimal_shift
 from .symmetry_ops import get_ops
 
 __all__ = ["calc_asu", "ASUBlock", "CrystallographicASU", "pack_asu", "CFFTPlan", "plan_cfft",
-           "mul", "ldiv"]
+           "mul", "ldiv", "InterleavedOrbit", "analyze_interleaved_orbits", "pack_asu_interleaved"]
 
 
 def _calc_asu_ops(N, ops):
@@ -122,6 +125,79 @@ def pack_asu(points, N, dtype=np.float64, shift=None):
         order.append((eff, len(dim_blocks[eff]) - 1))
     sh = tuple(shift) if shift is not None else (0.0,) * dim
     return CrystallographicASU(dim_blocks, sh, order)
+
+
+@dataclass
+class InterleavedOrbit:
+    """src/asu.jl:16-21: the coset shifts x_s in [0, L) of one orbit under the
+    ops taken mod L; ops[i] lists the ops g with g(representative) = members[i]."""
+    representative: np.ndarray
+    members: list
+    ops: list
+    multiplicity: int
+
+
+def analyze_interleaved_orbits(N, ops, L=None):
+    """src/asu.jl:331-431: orbits of the stride-L subgrids (coset shifts
+    p in [0, L)^D, first axis fastest) under x -> (R x + t) mod L, t in grid
+    units; each orbit is found by closing the representative under the ops,
+    then every member lists the ops that send the representative there.
+    Members are sorted lexicographically (first component most significant)."""
+    D = len(tuple(N))
+    Lv = np.array([2] * D if L is None else [int(v) for v in L], dtype=np.int64)
+    Rs = [op.Rm for op in ops]
+    ts = [op.tv for op in ops]
+    # coset shifts in the reference's product order (first axis fastest)
+    coarse = [np.array(p[::-1], dtype=np.int64) for p in np.ndindex(*Lv[::-1])]
+    visited = set()
+    orbits = []
+    for p in coarse:
+        if tuple(p) in visited:
+            continue
+        members = {tuple(p)}
+        queue = [p]
+        while queue:
+            cur = queue.pop(0)
+            for R, t in zip(Rs, ts):
+                x = tuple(int(v) for v in np.mod(R @ cur + t, Lv))
+                if x not in members:
+                    members.add(x)
+                    queue.append(np.array(x, dtype=np.int64))
+        final = sorted(members)
+        mops = [[op for op, R, t in zip(ops, Rs, ts) if tuple(int(v) for v in np.mod(R @ p + t, Lv)) == m]
+                for m in final]
+        orbits.append(InterleavedOrbit(p.copy(), [np.array(m, dtype=np.int64) for m in final], mops,
+                                       len(final)))
+        visited |= members
+    return orbits
+
+
+def pack_asu_interleaved(u, N, ops, L=None, asu_only=True):
+    """src/pack_asu.jl:113-202 (Mode B): the stride-L subgrids of a full grid
+    as blocks.  asu_only (default): the one subgrid of the orbit whose
+    representative is the zero shift (Gamma_0; the first orbit if none is),
+    without orbit information; otherwise one block per orbit representative,
+    each carrying its orbit.  Block data are copies of u[r_0::L_0, ...]
+    (host arrays), ranges the 0-based strided index ranges, depths zero;
+    all blocks sit under dim_blocks[D] and the shift is zero."""
+    u = np.asarray(u)
+    N = tuple(int(n) for n in N)
+    D = len(N)
+    if u.shape != N:
+        raise L_.CFFTError(L_.CFFT_ERR_INVALID, "u must have the grid's shape")
+    orbits = analyze_interleaved_orbits(N, ops, L)
+    Lv = [2] * D if L is None else [int(v) for v in L]
+    if asu_only:
+        zero = [o for o in orbits if not np.any(o.representative)]
+        orbits = zero[:1] if zero else orbits[:1]
+    blocks = []
+    for o in orbits:
+        rep = [int(v) for v in o.representative]
+        sl = tuple(slice(rep[d], None, Lv[d]) for d in range(D))
+        rng = [range(rep[d], N[d], Lv[d]) for d in range(D)]
+        blocks.append(ASUBlock(np.array(u[sl], copy=True), rng, np.zeros(D, dtype=np.int64),
+                               None if asu_only else o))
+    return CrystallographicASU({D: blocks}, (0.0,) * D, [(D, i) for i in range(len(blocks))])
 
 
 class CFFTPlan:

@@ -412,8 +412,29 @@ class GeneralCFFTPlan:
         self.output_buffer = np.zeros(self.dev.n_spec, dtype=np.complex128)
 
 
-def plan_krfft(spec, ops_shifted) -> GeneralCFFTPlan:
-    return GeneralCFFTPlan(spec, ops_shifted)
+def plan_krfft(*args) -> GeneralCFFTPlan:
+    """``plan_krfft(spec, ops_shifted)`` (src/krfft.jl:301-389), or
+    ``plan_krfft(real_asu, spec, ops)`` (src/krfft.jl:119-...) with the real
+    ASU of ``pack_asu_interleaved``: its one block is the stride-L subgrid,
+    and the plan is the same as the first form's when that L is the one
+    ``auto_L`` finds for the ops (otherwise CFFT_ERR_UNSUPPORTED)."""
+    if len(args) == 2:
+        return GeneralCFFTPlan(args[0], args[1])
+    if len(args) != 3:
+        raise TypeError("plan_krfft(spec, ops_shifted) or plan_krfft(real_asu, spec, ops)")
+    real_asu, spec, ops = args
+    blocks = [b for d in sorted(real_asu.dim_blocks) for b in real_asu.dim_blocks[d]]
+    N = tuple(int(n) for n in spec.N)
+    if len(blocks) != 1:
+        raise L.CFFTError(L.CFFT_ERR_UNSUPPORTED, "the real ASU must be the one stride-L subgrid block")
+    dims = blocks[0].data.shape
+    if any(N[d] % dims[d] for d in range(len(N))):
+        raise L.CFFTError(L.CFFT_ERR_INVALID, "Block size must divide Global size for Cooley-Tukey decomposition")
+    Lb = tuple(N[d] // dims[d] for d in range(len(N)))
+    if Lb != tuple(int(v) for v in auto_L(ops))[:len(N)]:
+        raise L.CFFTError(L.CFFT_ERR_UNSUPPORTED,
+                          f"subgrid stride {Lb} differs from auto_L of the ops {tuple(auto_L(ops))}")
+    return GeneralCFFTPlan(spec, ops)
 
 
 def fft_reconstruct(plan: GeneralCFFTPlan):

@@ -43,6 +43,8 @@ HALL_3D = {
     71: "-I 2 2",
     72: "-I 2 2c",
     74: "-I 2b 2",
+    75: "P 4",
+    99: "P 4 -2",
     123: "-P 4 2",
     136: "-P 4n 2n",
     139: "-I 4 2",
@@ -61,7 +63,7 @@ HALL_3D = {
 # point-group order x number of centring vectors.
 GROUP_ORDER_3D = {
     1: 1, 2: 2, 10: 4, 12: 8, 14: 4, 25: 4, 35: 8, 47: 8,
-    62: 8, 63: 16, 65: 16, 69: 32, 70: 32, 71: 16, 72: 16, 74: 16,
+    62: 8, 63: 16, 65: 16, 69: 32, 70: 32, 71: 16, 72: 16, 74: 16, 75: 4, 99: 8,
     123: 16, 136: 16, 139: 32, 141: 32, 200: 24, 221: 48, 223: 48, 224: 48,
     225: 192, 227: 192, 229: 96, 230: 96,
 }
