@@ -178,6 +178,47 @@ def test_ia3d_mixed_scft_iterations_vs_oracle(path):
         assert _rel(phiB, out["phiB"]) < 2e-5
 
 
+def test_ia3d_ordered_gyroid_mixed_vs_fp64_solve():
+    """The Ia-3d melt (f 0.35, chiN 18, Ns 200) in a cell near the gyroid's
+    own period (9.0; at 3.8 it relaxes to the disordered state), 64^3: the
+    FP64 solve reaches an ordered gyroid (F well below chiN f (1 - f), phi_A
+    from < 0.1 to > 0.9); the mixed-precision solve to residual 1e-6 lands on
+    the same solution (|dF| < 2e-5, max |dphi| < 1e-4; measured 8e-7 and
+    2e-6), and a mixed stage to 1e-5 followed by an FP64 finish to 1e-8
+    matches the FP64 solve to 2e-9 (two FP64 solves stopped at residual 1e-8
+    from different fields: measured 1.8e-10)."""
+    from cfft_hip.scft import QS_MIXED_TOL, SCFTSolver, default_path
+    N = (64, 64, 64)
+    cell = 9.0
+    ops_s, R, t, wA, wB = _gyroid(N, cell=cell)
+    lat = cell * np.eye(3)
+    mk = lambda prec, path=None: SCFTSolver(N, ops_s, lat, 0.35, 18.0, 200, prec, path=path)
+    s64 = mk(cf.FP64)
+    s64.set_fields(wA, wB)
+    i64, o64 = s64.solve(tol=1e-8, max_iter=600)
+    p64 = s64.densities()
+    assert o64["res"] < 1e-8
+    assert o64["F"] < 18.0 * 0.35 * 0.65 - 0.2
+    assert p64[0].min() < 0.1 and p64[0].max() > 0.9
+    s32 = mk(cf.MIXED)
+    s32.set_fields(wA, wB)
+    i32, o32 = s32.solve(tol=1e-6, max_iter=600)
+    dphi = max(float(np.max(np.abs(a - b))) for a, b in zip(s32.densities(), p64))
+    print(f"  FP64 {i64} it F {o64['F']:.10f}; mixed {i32} it res {o32['res']:.1e} "
+          f"dF {abs(o32['F'] - o64['F']):.2e} dphi {dphi:.2e}", flush=True)
+    assert o32["res"] < 1e-6
+    assert abs(o32["F"] - o64["F"]) < 2e-5
+    assert dphi < 1e-4
+    hand = mk(cf.MIXED, default_path(N, ops_s, lat, cf.MIXED, QS_MIXED_TOL))
+    hand.set_fields(wA, wB)
+    hand.solve(tol=QS_MIXED_TOL, max_iter=600)
+    fin = mk(cf.FP64)
+    fin.set_fields(*hand.fields())
+    _, of = fin.solve(tol=1e-8, max_iter=600)
+    print(f"  mixed then FP64: dF {abs(of['F'] - o64['F']):.2e}", flush=True)
+    assert of["res"] < 1e-8 and abs(of["F"] - o64["F"]) < 2e-9
+
+
 # ---------------------------------------------------------------------------
 # the 64-instance shape: every grid size with every group
 # ---------------------------------------------------------------------------
