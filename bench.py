@@ -142,7 +142,8 @@ def profile_stages(sw, steps, torch):
     return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(L.STAGES) if cnt[i]}
 
 
-STRATEGY_PATH = {"m7_chain": 0, "m7_persistent": 0, "qr_chain": 5, "qs_chain": 6, "qs_chain_eager": 6}
+STRATEGY_PATH = {"m7_chain": 0, "m7_persistent": 0, "qr_chain": 5, "qs_chain": 6, "qs_chain_eager": 6,
+                 "qs_chain_tuned": 6}
 # the FETCH_SIZE / WRITE_SIZE record of each pipeline path's chain kernels
 # (eager launches and graph replays run the same kernels)
 TRAFFIC_RECORD = {0: "m7_chain", 5: "qr_chain", 6: "qs_chain"}
@@ -582,13 +583,20 @@ def main():
         # orbit-reduced Q chain; the fastest one is used for the timed region
         trial = {}
         sweeps = {}
+        # the plane-pair chain's schedule (tile rows, graph replay, pair
+        # workgroup size, lane split) tuned on this workload (cfft_hip.tune)
+        from cfft_hip.tune import tune_schedule
+        tuned, tuned_times = tune_schedule(N, ops_s, lat, NS, prec, 2, cf.PATH_QS, lanes=True)
         for label, path, persist, graph in (("m7_chain", cf.PATH_AUTO, 0, 1), ("m7_persistent", cf.PATH_AUTO, 1, 1),
                                             ("qr_chain", cf.PATH_QR, 0, 1), ("qs_chain", cf.PATH_QS, 0, 1),
-                                            ("qs_chain_eager", cf.PATH_QS, 0, 0)):
+                                            ("qs_chain_eager", cf.PATH_QS, 0, 0), ("qs_chain_tuned", cf.PATH_QS, 0, 1)):
             sw = cf.scft.PropagatorSweep(N, ops_s, lat, F_A, NS, prec, path=path)
             sw.set_fields(wA, wB)
             sw.plan.set_option(cf._lib.OPT_PERSISTENT, persist)
             sw.plan.set_option(cf._lib.OPT_GRAPH, graph)
+            if label == "qs_chain_tuned":
+                for k, v in tuned.items():
+                    sw.plan.set_option(k, v)
             trial[label] = time_sweeps(sw, 3, 1, torch)
             sweeps[label] = sw
         best = min(trial, key=trial.get)
@@ -598,6 +606,8 @@ def main():
                 del sweeps[label]
         variants[name] = {k + "_ms_per_step": v / (3 * NS) * 1e3 for k, v in trial.items()}
         variants[name]["used"] = best
+        variants[name]["tuned_schedule"] = {"options": {str(k): int(v) for k, v in tuned.items()},
+                                            "us_per_step": {k: round(v, 3) for k, v in tuned_times.items()}}
         # full sweep once for Q
         sw.reset()
         sw.run()

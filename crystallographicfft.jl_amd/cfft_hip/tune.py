@@ -1,15 +1,21 @@
-"""Launch-schedule tuning for a workload: the fold/unfold tile height and
-hipGraph replay of a plan, timed on the plan's own grid, group, precision and
-number of fields per launch.  No counterpart in the reference.
+"""Launch-schedule tuning for a workload: the fold/unfold tile height,
+hipGraph replay and, on path QS, the plane-pair workgroup size and the
+radix schedule (lane split) of the transforms, timed on the plan's own grid,
+group, precision and number of fields per launch.  No counterpart in the
+reference.
 
-The candidates do not change results (every schedule computes each field with
-the same arithmetic), only the decomposition of the phase kernels and how the
-step sequence is launched, so the choice is made by timing alone.  A timing
-is one contour sweep of `steps` Strang steps over `batch` fields with all
+Tile height, graph replay and workgroup size do not change results (every
+schedule computes each field with the same arithmetic), only the
+decomposition of the phase kernels and how the step sequence is launched.
+The lane split R1 x R2 of a length-H transform (in-lane R2-point passes,
+then a cross-lane radix-R1 pass) reorders the butterflies, so results move
+by rounding only; it is searched only when `lanes` is set.  A timing is one
+contour sweep of `steps` Strang steps over `batch` fields with all
 interaction factors 1 (a plain multi-step sweep, the kernels of the stored
 sweeps apart from the q store), after one untimed sweep that captures the
-graph; the best of `reps` sweeps counts.  Results are cached per workload in
-this process.
+graph; the best of `reps` sweeps counts.  The grid of tile heights x graph
+replay is timed in full, then each further dimension from the best so far.
+Results are cached per workload in this process.
 """
 from __future__ import annotations
 
@@ -29,6 +35,25 @@ def schedule_candidates(H1):
     return [{L.OPT_TILE_ROWS: r, L.OPT_GRAPH: g} for r in rows for g in (1, 0)]
 
 
+def extra_dimensions(plan, lanes):
+    """Further schedule dimensions of a plan on path QS: the plane-pair
+    workgroup size (512 threads for H = 16, 24, 32) and, with `lanes`, the
+    lane split R1 = 2 of the transforms (H = 16, 32)."""
+    out = []
+    if plan.path == L.PATH_QS:
+        if plan.H[1] in (16, 24, 32):
+            out.append((L.OPT_PAIR_THREADS, 512))
+        if lanes and plan.H[0] in (16, 32):
+            out.append((L.OPT_FFT_LANES, 2))
+    return out
+
+
+def _label(opt):
+    return (f"rows{opt[L.OPT_TILE_ROWS]}_graph{opt[L.OPT_GRAPH]}"
+            + (f"_pt{opt[L.OPT_PAIR_THREADS]}" if opt.get(L.OPT_PAIR_THREADS) else "")
+            + (f"_lanes{opt[L.OPT_FFT_LANES]}" if opt.get(L.OPT_FFT_LANES) else ""))
+
+
 def _key(N, ops_shifted, lattice, ns, precision, batch, path):
     ops = tuple((tuple(np.asarray(o.Rm).ravel().tolist()), tuple(np.asarray(o.tv).ravel().tolist()))
                 for o in ops_shifted)
@@ -36,11 +61,12 @@ def _key(N, ops_shifted, lattice, ns, precision, batch, path):
             int(precision), int(batch), int(path))
 
 
-def tune_schedule(N, ops_shifted, lattice, ns, precision, batch, path, steps=None, reps=3):
+def tune_schedule(N, ops_shifted, lattice, ns, precision, batch, path, steps=None, reps=3, lanes=False):
     """(best options, {label: microseconds per step}) for a plan of `batch`
-    fields with ds = 1/ns; options are {OPT_TILE_ROWS: r, OPT_GRAPH: g}."""
+    fields with ds = 1/ns; options are {OPT_TILE_ROWS: r, OPT_GRAPH: g,
+    OPT_PAIR_THREADS: t, OPT_FFT_LANES: l} (the last two 0 = default)."""
     import torch
-    key = _key(N, ops_shifted, lattice, ns, precision, batch, path)
+    key = _key(N, ops_shifted, lattice, ns, precision, batch, path) + (bool(lanes),)
     if key in _CACHE:
         return _CACHE[key]
     steps = int(ns if steps is None else steps)
@@ -48,11 +74,11 @@ def tune_schedule(N, ops_shifted, lattice, ns, precision, batch, path, steps=Non
     M3 = int(np.prod(plan.M))
     dt = torch.float64 if precision == L.FP64 else torch.float32
     q = torch.ones(batch * M3, dtype=dt, device="cuda")
-    E = torch.ones(batch * M3, dtype=dt, device="cuda")
+    E = torch.ones(batch * M3, dtype=torch.float64, device="cuda")
     s = torch.cuda.current_stream().cuda_stream
     times = {}
-    best, best_t = None, None
-    for opt in schedule_candidates(plan.H[1]):
+
+    def timed(opt):
         for k, v in opt.items():
             plan.set_option(k, v)
         plan.step_dev(q.data_ptr(), E.data_ptr(), steps, s)  # captures the graph when enabled
@@ -64,10 +90,22 @@ def tune_schedule(N, ops_shifted, lattice, ns, precision, batch, path, steps=Non
             torch.cuda.synchronize()
             el = (time.perf_counter() - t0) / steps * 1e6
             t = el if t is None else min(t, el)
-        label = f"rows{opt[L.OPT_TILE_ROWS]}_graph{opt[L.OPT_GRAPH]}"
-        times[label] = t
+        times[_label(opt)] = t
+        return t
+
+    best, best_t = None, None
+    for opt in schedule_candidates(plan.H[1]):
+        opt = dict(opt)
+        opt.update({L.OPT_PAIR_THREADS: 0, L.OPT_FFT_LANES: 0})
+        t = timed(opt)
         if best_t is None or t < best_t:
-            best, best_t = dict(opt), t
+            best, best_t = opt, t
+    for k, v in extra_dimensions(plan, lanes):
+        opt = dict(best)
+        opt[k] = v
+        t = timed(opt)
+        if t < best_t:
+            best, best_t = opt, t
     del plan
     _CACHE[key] = (best, times)
     return best, times

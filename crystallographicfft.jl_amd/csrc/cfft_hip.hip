@@ -63,6 +63,8 @@ struct cfft_plan {
   int graph = 1;       // replay multi-step chains from captured hipGraphs
   int tile_rows = 0;   // y rows per fold/unfold tile (0: the default rule)
   int arith32 = 0;     // mixed precision on path QS: FP32 twiddle and fold-factor arithmetic (option)
+  int lanes = 0;       // path QS: lane split of the plane-pair and tile transforms (0: the default)
+  int pair_threads = 0;  // path QS: threads per plane-pair workgroup (0: 256)
   hipStream_t cap_stream = nullptr;  // capture stream of the chains
   struct Cached {
     std::string key;
@@ -809,6 +811,22 @@ void launch_p2q(cfft_plan* p, const PipeP& pp, const PipeBufs& B, hipStream_t s)
       f32 = true;
     }
   }
+  if (!f32 && (p->lanes || p->pair_threads)) {
+    // a tuned schedule (set_option checked that it is instantiated)
+    const int nt = p->pair_threads ? p->pair_threads : kQNT;
+#define CFFT_P2Q_SCHED(L, NT, LN)                                                                  \
+  if (p->H[1] == L && nt == NT && p->lanes == LN) {                                                \
+    if (pp.qs_sep) hipLaunchKernelGGL((k_p2q<T, double, L, true, NT, LN>), dim3(g), dim3(NT), 0, s, pp, B); \
+    else hipLaunchKernelGGL((k_p2q<T, double, L, false, NT, LN>), dim3(g), dim3(NT), 0, s, pp, B);          \
+    launch_check<T>("k_p2q");                                                                      \
+    return;                                                                                        \
+  }
+    CFFT_P2Q_SCHED(32, 256, 2) CFFT_P2Q_SCHED(32, 512, 0) CFFT_P2Q_SCHED(32, 512, 2)
+    CFFT_P2Q_SCHED(16, 256, 2) CFFT_P2Q_SCHED(16, 512, 0) CFFT_P2Q_SCHED(16, 512, 2)
+    CFFT_P2Q_SCHED(24, 512, 0)
+#undef CFFT_P2Q_SCHED
+    throw Error{CFFT_ERR_UNSUPPORTED, "no plane-pair kernel for this schedule"};
+  }
   if (!f32) {
 #define CFFT_P2Q(L)                                                                           \
   case L:                                                                                     \
@@ -837,6 +855,19 @@ void launch_p51(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipS
 #undef CFFT_P51
       f32 = true;
     }
+  }
+  if (!f32 && p->lanes) {
+    // a tuned lane split (set_option checked that it is instantiated)
+#define CFFT_P51_SCHED(L, LN)                                                                              \
+  if (p->H[0] == L && p->lanes == LN) {                                                                    \
+    if (B.f0_out) hipLaunchKernelGGL((k_p51<T, double, L, true, LN>), dim3(g), dim3(kNT), 0, s, pp, B, step); \
+    else hipLaunchKernelGGL((k_p51<T, double, L, false, LN>), dim3(g), dim3(kNT), 0, s, pp, B, step);        \
+    launch_check<T>("k_p51");                                                                              \
+    return;                                                                                                \
+  }
+    CFFT_P51_SCHED(32, 2) CFFT_P51_SCHED(16, 2)
+#undef CFFT_P51_SCHED
+    throw Error{CFFT_ERR_UNSUPPORTED, "no unfold/fold kernel for this lane split"};
   }
   if (!f32) {
 #define CFFT_P51(L)                                                                              \
@@ -1137,7 +1168,8 @@ constexpr size_t kMaxGraphs = 16;
 template <typename T>
 void run_steps_graph(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int nsteps, hipStream_t s) {
   std::string key(reinterpret_cast<const char*>(&pp), sizeof(PipeP));
-  const long hdr[6] = {reinterpret_cast<long>(q), stride, step0, nsteps, p->fuse, p->arith32};
+  const long hdr[8] = {reinterpret_cast<long>(q), stride, step0, nsteps, p->fuse, p->arith32, p->lanes,
+                       p->pair_threads};
   key.append(reinterpret_cast<const char*>(hdr), sizeof(hdr));
   for (size_t i = 0; i < p->graphs.size(); ++i)
     if (p->graphs[i].key == key) {
@@ -1272,6 +1304,7 @@ EnsPlanInfo ens_plan_info(cfft_plan* p) {
   i.cubic = cubic(p) ? 1 : 0;
   i.batch = p->batch;
   i.sep = p->qs_sep ? 1 : 0;
+  i.lanes = p->lanes;
   i.pair_items = (long)p->qs_items * p->batch;
   i.tile_items = n_tiles(p, pipe_params(p, ESel{}));
   i.M3 = p->M3;
@@ -1795,6 +1828,22 @@ int cfft_plan_set_option(cfft_plan* p, int32_t key, int64_t value) {
         p->tile_rows = (int)value;
         break;
       case CFFT_OPT_FP32_TWIDDLES: p->arith32 = value != 0; break;
+      case CFFT_OPT_FFT_LANES: {
+        // 0 or the default split: the default kernels; 2 on H = 16, 32 (path QS)
+        const int def = (p->H[0] % 16 == 0) ? 4 : ((p->H[0] % 4 == 0) ? 2 : 1);
+        require(value == 0 || value == def || (value == 2 && is_qs(p) && (p->H[0] == 16 || p->H[0] == 32)),
+                CFFT_ERR_UNSUPPORTED, "lane split not available for this plan");
+        p->lanes = (value == 0 || value == def) ? 0 : (int)value;
+        break;
+      }
+      case CFFT_OPT_PAIR_THREADS:
+        require(value == 0 || value == 256 || (value == 512 && is_qs(p) && (p->H[1] == 16 || p->H[1] == 24 ||
+                                                                            p->H[1] == 32)),
+                CFFT_ERR_UNSUPPORTED, "plane-pair workgroup size not available for this plan");
+        require(value != 512 || !fp32_arith(p), CFFT_ERR_UNSUPPORTED,
+                "the FP32-twiddle plane-pair kernel runs 256 threads");
+        p->pair_threads = value == 512 ? 512 : 0;
+        break;
       default: throw Error{CFFT_ERR_INVALID, "unknown option " + std::to_string(key)};
     }
   });

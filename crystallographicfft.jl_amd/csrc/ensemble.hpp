@@ -60,6 +60,7 @@ struct EnsPart {
 struct EnsPlanInfo {
   int path, prec, arith32, device, L, cubic, batch;
   int sep;                      // path QS: the separable Q multiply
+  int lanes;                    // path QS: a lane split other than the default (0: none)
   long pair_items, tile_items;  // plane-pair and tile work items of one launch
   long M3;
 };

@@ -181,11 +181,20 @@ __device__ __forceinline__ void lane_twiddles(cx<T> (&v)[R2], int r) {
 }
 
 // Split of a length-LEN transform over R1 cooperating lanes with R2 = LEN/R1
-// values each (R2 divisible by R1).
-template <int LEN>
+// values each (R2 divisible by R1): the in-lane R2-point Stockham passes
+// (radices 4, 2, 3 in that order) followed by one cross-lane radix-R1 pass.
+// LANES = 0 picks R1 = 4 for LEN divisible by 16, 2 for LEN divisible by 4,
+// else 1; LANES > 0 sets R1 (a radix schedule the tuner may choose).
+__host__ __device__ constexpr int ml_r1(int len, int lanes) {
+  return lanes ? lanes : ((len % 16 == 0) ? 4 : ((len % 4 == 0) ? 2 : 1));
+}
+__host__ __device__ constexpr int ml_r2(int len, int lanes) { return len / ml_r1(len, lanes); }
+
+template <int LEN, int LANES = 0>
 struct MLS {
-  static constexpr int R1 = (LEN % 16 == 0) ? 4 : ((LEN % 4 == 0) ? 2 : 1);
-  static constexpr int R2 = LEN / R1;
+  static constexpr int R1 = ml_r1(LEN, LANES);
+  static constexpr int R2 = ml_r2(LEN, LANES);
+  static_assert(R1 * R2 == LEN && R2 % R1 == 0, "lane split must divide the length, R2 divisible by R1");
 };
 
 // Multi-lane FFT.  A group of R1 lanes (lane index r) owns one transform; on
@@ -222,9 +231,9 @@ __device__ __forceinline__ void fft_ml(cx<T> (&v)[MLS<LEN>::R2], int r, int g, c
 // the transform's own slots.  The R1 lanes are consecutive lanes of one
 // wavefront, so wavefront-scope fences order the exchange; no workgroup
 // barrier and no separate exchange buffer.  Same entry/exit layout as fft_ml.
-template <typename T, typename TW, int LEN, int DIR>
-__device__ __forceinline__ void fft_ml_lds(cx<T> (&v)[MLS<LEN>::R2], int r, cx<T>* slot, int es) {
-  constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2;
+template <typename T, typename TW, int LEN, int DIR, int LANES = 0>
+__device__ __forceinline__ void fft_ml_lds(cx<T> (&v)[ml_r2(LEN, LANES)], int r, cx<T>* slot, int es) {
+  constexpr int R1 = MLS<LEN, LANES>::R1, R2 = MLS<LEN, LANES>::R2;
   stockham<T, TW, R2, DIR, 1>(v);
   if constexpr (R1 > 1) {
     lane_twiddles<T, TW, LEN, DIR, R1, R2>(v, r);
@@ -255,9 +264,9 @@ __device__ __forceinline__ void fft_ml_lds(cx<T> (&v)[MLS<LEN>::R2], int r, cx<T
 }
 
 // Output position of register p after fft_ml.
-template <int LEN>
+template <int LEN, int LANES = 0>
 __device__ __forceinline__ int ml_out_pos(int r, int p) {
-  constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2;
+  constexpr int R1 = MLS<LEN, LANES>::R1, R2 = MLS<LEN, LANES>::R2;
   return (r + R1 * (p / R1)) + R2 * (p % R1);
 }
 

@@ -59,9 +59,9 @@ struct PipeBufs {
   const void* qsep;      // path QS: axis factors F[d][s][q] (cx<double>, 3 x 8 x M0)
 };
 
-template <int LEN, int NT = kNT>
+template <int LEN, int NT = kNT, int LANES = 0>
 struct PipeG {
-  static constexpr int R1 = MLS<LEN>::R1;
+  static constexpr int R1 = MLS<LEN, LANES>::R1;
   static constexpr int G = NT / R1;  // transforms per pass of a workgroup of NT threads
 };
 
@@ -76,10 +76,10 @@ constexpr int kRow = (L % 16 == 0) ? (sizeof(T) == 4 ? L + 4 : L + 2) : L + 1;
 // FFT of n1 * n2 length-LEN transforms held in LDS: element j of transform
 // t = t1 + n1 t2 is buf[t1 * ts1 + t2 * ts2 + j * es].  Every thread of the
 // workgroup must call.
-template <typename T, typename TW, int LEN, int DIR, int NT = kNT>
+template <typename T, typename TW, int LEN, int DIR, int NT = kNT, int LANES = 0>
 __device__ __forceinline__ void lds_fft2(cx<T>* buf, int n1, int ts1, int n2, int ts2, int es) {
   if constexpr (LEN > 1) {
-    constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2, G = PipeG<LEN, NT>::G;
+    constexpr int R1 = MLS<LEN, LANES>::R1, R2 = MLS<LEN, LANES>::R2, G = PipeG<LEN, NT, LANES>::G;
     const int r = threadIdx.x % R1, g = threadIdx.x / R1;
     const int n_tr = n1 * n2;
     for (int t0 = 0; t0 < n_tr; t0 += G) {
@@ -94,9 +94,9 @@ __device__ __forceinline__ void lds_fft2(cx<T>* buf, int n1, int ts1, int n2, in
 #pragma unroll
       for (int m = 0; m < R2; ++m) v[m] = slot[(r + R1 * m) * es];
       if (ok) {
-        fft_ml_lds<T, TW, LEN, DIR>(v, r, slot, es);
+        fft_ml_lds<T, TW, LEN, DIR, LANES>(v, r, slot, es);
 #pragma unroll
-        for (int q = 0; q < R2; ++q) slot[ml_out_pos<LEN>(r, q) * es] = v[q];
+        for (int q = 0; q < R2; ++q) slot[ml_out_pos<LEN, LANES>(r, q) * es] = v[q];
       }
     }
   }
@@ -341,10 +341,10 @@ __device__ void ph_fold_fftx(const PipeP& p, const PipeBufs& B, long item, int s
 // through the transform's own LDS row (stride L + 1, plane stride NR (L + 1))
 // and leave the result there (from_global) or store it (to_global).  Groups
 // past the last row work on row 0 and store nothing.
-template <typename T, typename TW, int L, int DIR, int NT, int NR = L, int RS = L + 1>
+template <typename T, typename TW, int L, int DIR, int NT, int NR = L, int RS = L + 1, int LANES = 0>
 __device__ __forceinline__ void rows_fft_from_global(const cx<T>* __restrict__ p0, const cx<T>* __restrict__ p1,
                                                      int n_rows, cx<T>* lds) {
-  constexpr int R1 = MLS<L>::R1, R2 = MLS<L>::R2, G = PipeG<L, NT>::G, rs = RS;
+  constexpr int R1 = MLS<L, LANES>::R1, R2 = MLS<L, LANES>::R2, G = PipeG<L, NT, LANES>::G, rs = RS;
   const int r = threadIdx.x % R1, g = threadIdx.x / R1;
   for (int t0 = 0; t0 < n_rows; t0 += G) {
     const int t = t0 + g;
@@ -357,17 +357,17 @@ __device__ __forceinline__ void rows_fft_from_global(const cx<T>* __restrict__ p
     for (int m = 0; m < R2; ++m) v[m] = src[r + R1 * m];
     if (ok) {
       cx<T>* slot = lds + pl * (NR * rs) + z * rs;
-      fft_ml_lds<T, TW, L, DIR>(v, r, slot, 1);
+      fft_ml_lds<T, TW, L, DIR, LANES>(v, r, slot, 1);
 #pragma unroll
-      for (int q = 0; q < R2; ++q) slot[ml_out_pos<L>(r, q)] = v[q];
+      for (int q = 0; q < R2; ++q) slot[ml_out_pos<L, LANES>(r, q)] = v[q];
     }
   }
 }
 
-template <typename T, typename TW, int L, int DIR, int NT, int NR = L, int RS = L + 1>
+template <typename T, typename TW, int L, int DIR, int NT, int NR = L, int RS = L + 1, int LANES = 0>
 __device__ __forceinline__ void rows_fft_to_global(cx<T>* lds, cx<T>* __restrict__ p0, cx<T>* __restrict__ p1,
                                                    int n_rows) {
-  constexpr int R1 = MLS<L>::R1, R2 = MLS<L>::R2, G = PipeG<L, NT>::G, rs = RS;
+  constexpr int R1 = MLS<L, LANES>::R1, R2 = MLS<L, LANES>::R2, G = PipeG<L, NT, LANES>::G, rs = RS;
   const int r = threadIdx.x % R1, g = threadIdx.x / R1;
   for (int t0 = 0; t0 < n_rows; t0 += G) {
     const int t = t0 + g;
@@ -379,10 +379,10 @@ __device__ __forceinline__ void rows_fft_to_global(cx<T>* lds, cx<T>* __restrict
 #pragma unroll
     for (int m = 0; m < R2; ++m) v[m] = slot[r + R1 * m];
     if (ok) {
-      fft_ml_lds<T, TW, L, DIR>(v, r, slot, 1);
+      fft_ml_lds<T, TW, L, DIR, LANES>(v, r, slot, 1);
       cx<T>* dst = (pl ? p1 : p0) + z * L;
 #pragma unroll
-      for (int q = 0; q < R2; ++q) dst[ml_out_pos<L>(r, q)] = v[q];
+      for (int q = 0; q < R2; ++q) dst[ml_out_pos<L, LANES>(r, q)] = v[q];
     }
   }
 }
@@ -562,7 +562,7 @@ __device__ __forceinline__ void unit_lines(int u, int off, int& a, int& b, bool&
 // contour step (about 1e-7 per step at the k = 0 mode, so 1e-5 in Q after
 // 100 steps); in FP64 only the rounding of each result to the data type
 // remains, which changes from step to step.
-template <typename T, typename TW, int L, int NT, bool SEP>
+template <typename T, typename TW, int L, int NT, bool SEP, int LANES = 0>
 __device__ __forceinline__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& B, long item, cx<T>* lds, cx<double>* fq) {
   using TQ = double;  // arithmetic of the Q multiply
   constexpr int rs = kRow<T, L>, NP = L * L, SL = L * rs, NF = 2 + 2 * L;
@@ -609,9 +609,9 @@ __device__ __forceinline__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& 
     __builtin_amdgcn_global_load_lds((const void*)(F + (d * 8 + sf) * Ms + q), dst, 16, 0, 0);
   }
   // FFT y: the rows of both planes, loaded straight into the transform lanes
-  rows_fft_from_global<T, TW, L, -1, NT, L, rs>(chan + (long)x0 * NP, chan + (long)x1 * NP, ns * L, lds);
+  rows_fft_from_global<T, TW, L, -1, NT, L, rs, LANES>(chan + (long)x0 * NP, chan + (long)x1 * NP, ns * L, lds);
   __syncthreads();
-  lds_fft2<T, TW, L, -1, NT>(lds, L, 1, ns, SL, rs);      // FFT z: columns of both planes
+  lds_fft2<T, TW, L, -1, NT, LANES>(lds, L, 1, ns, SL, rs);      // FFT z: columns of both planes
   if constexpr (SEP) {
     // t_{s,d} depends on bit d of s only: Q_s(q) = prod_d g_d(s_d, q_d) with
     // g_d(s_d, .) = F_d[s_d << d][.], and the octet sum factors into three
@@ -710,9 +710,9 @@ __device__ __forceinline__ void ph_plane_pair_q(const PipeP& p, const PipeBufs& 
     }
   }
   __syncthreads();
-  lds_fft2<T, TW, L, +1, NT>(lds, L, 1, ns, SL, rs);  // IFFT z
+  lds_fft2<T, TW, L, +1, NT, LANES>(lds, L, 1, ns, SL, rs);  // IFFT z
   // IFFT y, each row stored straight from its transform lanes
-  rows_fft_to_global<T, TW, L, +1, NT, L, rs>(lds, chan + (long)x0 * NP, chan + (long)x1 * NP, ns * L);
+  rows_fft_to_global<T, TW, L, +1, NT, L, rs, LANES>(lds, chan + (long)x0 * NP, chan + (long)x1 * NP, ns * L);
   __syncthreads();
 }
 
@@ -796,10 +796,10 @@ __device__ __forceinline__ void wave_sync() {
 // whole tile; WAVE = true, the rows of the wave's own tile points (wave w holds
 // the points of rows RPW w .. RPW w + RPW - 1, RPW = 64 / L0): its group gi
 // takes channel gi mod n_ch of row RPW w + gi / n_ch.
-template <int L0, bool WAVE>
+template <int L0, bool WAVE, int LANES = 0>
 __device__ __forceinline__ bool tile_row_of(const PipeP& p, int t0, int g, int& tt) {
   if constexpr (WAVE) {
-    constexpr int GW = 64 / MLS<L0>::R1, RPW = 64 / L0;  // groups and tile rows per wave
+    constexpr int GW = 64 / MLS<L0, LANES>::R1, RPW = 64 / L0;  // groups and tile rows per wave
     const int w = g / GW, gi = g - w * GW;
     const int c = gi % p.n_ch, yy = RPW * w + gi / p.n_ch;
     const bool ok = gi < RPW * p.n_ch && yy < p.YB;
@@ -813,16 +813,16 @@ __device__ __forceinline__ bool tile_row_of(const PipeP& p, int t0, int g, int& 
   }
 }
 
-template <typename T, typename TW, int L0, int DIR, bool WAVE = false, int RS = L0 + 1>
+template <typename T, typename TW, int L0, int DIR, bool WAVE = false, int RS = L0 + 1, int LANES = 0>
 __device__ __forceinline__ void tile_rows_fft_from_global(const PipeP& p, const cx<T>* __restrict__ ch, int b, int z,
                                                           int y0, cx<T>* lds) {
-  constexpr int R1 = MLS<L0>::R1, R2 = MLS<L0>::R2, G = PipeG<L0, kNT>::G, rs = RS;
+  constexpr int R1 = MLS<L0, LANES>::R1, R2 = MLS<L0, LANES>::R2, G = PipeG<L0, kNT, LANES>::G, rs = RS;
   const int n_rows = WAVE ? 1 : p.n_ch * p.YB;
   const long HH = (long)p.H[2] * p.H[1];
   const int r = threadIdx.x % R1, g = threadIdx.x / R1;
   for (int t0 = 0; t0 < n_rows; t0 += G) {
     int tt;
-    const bool ok = tile_row_of<L0, WAVE>(p, t0, g, tt);
+    const bool ok = tile_row_of<L0, WAVE, LANES>(p, t0, g, tt);
     const int c = tt >> p.ybs, yy = tt & (p.YB - 1);
     const cx<T>* src = ch + ((long)b * p.n_ch + c) * L0 * HH + (long)z * p.H[1] + y0 + yy;
     cx<T> v[R2];
@@ -830,33 +830,33 @@ __device__ __forceinline__ void tile_rows_fft_from_global(const PipeP& p, const 
     for (int m = 0; m < R2; ++m) v[m] = src[(long)(r + R1 * m) * HH];
     if (ok) {
       cx<T>* slot = lds + tt * rs;
-      fft_ml_lds<T, TW, L0, DIR>(v, r, slot, 1);
+      fft_ml_lds<T, TW, L0, DIR, LANES>(v, r, slot, 1);
 #pragma unroll
-      for (int q = 0; q < R2; ++q) slot[ml_out_pos<L0>(r, q)] = v[q];
+      for (int q = 0; q < R2; ++q) slot[ml_out_pos<L0, LANES>(r, q)] = v[q];
     }
   }
 }
 
-template <typename T, typename TW, int L0, int DIR, bool WAVE = false, int RS = L0 + 1>
+template <typename T, typename TW, int L0, int DIR, bool WAVE = false, int RS = L0 + 1, int LANES = 0>
 __device__ __forceinline__ void tile_rows_fft_to_global(const PipeP& p, cx<T>* lds, cx<T>* __restrict__ ch, int b,
                                                         int z, int y0) {
-  constexpr int R1 = MLS<L0>::R1, R2 = MLS<L0>::R2, G = PipeG<L0, kNT>::G, rs = RS;
+  constexpr int R1 = MLS<L0, LANES>::R1, R2 = MLS<L0, LANES>::R2, G = PipeG<L0, kNT, LANES>::G, rs = RS;
   const int n_rows = WAVE ? 1 : p.n_ch * p.YB;
   const long HH = (long)p.H[2] * p.H[1];
   const int r = threadIdx.x % R1, g = threadIdx.x / R1;
   for (int t0 = 0; t0 < n_rows; t0 += G) {
     int tt;
-    const bool ok = tile_row_of<L0, WAVE>(p, t0, g, tt);
+    const bool ok = tile_row_of<L0, WAVE, LANES>(p, t0, g, tt);
     cx<T>* slot = lds + tt * rs;
     cx<T> v[R2];
 #pragma unroll
     for (int m = 0; m < R2; ++m) v[m] = slot[r + R1 * m];
     if (ok) {
-      fft_ml_lds<T, TW, L0, DIR>(v, r, slot, 1);
+      fft_ml_lds<T, TW, L0, DIR, LANES>(v, r, slot, 1);
       const int c = tt >> p.ybs, yy = tt & (p.YB - 1);
       cx<T>* dst = ch + ((long)b * p.n_ch + c) * L0 * HH + (long)z * p.H[1] + y0 + yy;
 #pragma unroll
-      for (int q = 0; q < R2; ++q) dst[(long)ml_out_pos<L0>(r, q) * HH] = v[q];
+      for (int q = 0; q < R2; ++q) dst[(long)ml_out_pos<L0, LANES>(r, q) * HH] = v[q];
     }
   }
 }
@@ -867,7 +867,7 @@ __device__ __forceinline__ void tile_rows_fft_to_global(const PipeP& p, cx<T>* l
 // plain multi-step sweep q(s + 1) has no reader and stays in registers); the
 // same thread then folds E_{s+1} q(s + 1) at the same tile points, so q(s + 1)
 // is never read back.
-template <typename T, typename TW, int L0, bool STORE>
+template <typename T, typename TW, int L0, bool STORE, int LANES = 0>
 __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds) {
   const int H0 = p.H[0], H1 = p.H[1], H2 = p.H[2], YB = p.YB;
   const int nyb = H1 / YB;
@@ -903,13 +903,13 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
   // those rows (every channel), the wave only ever touches its own LDS rows
   // and wavefront-scope ordering replaces the workgroup barriers.
   constexpr bool ROWS_PER_WAVE = (64 % L0 == 0) && (L0 * 8 <= kNT);
-  const bool wave = ROWS_PER_WAVE && (64 / L0) * p.n_ch <= 64 / MLS<L0>::R1;
+  const bool wave = ROWS_PER_WAVE && (64 / L0) * p.n_ch <= 64 / MLS<L0, LANES>::R1;
   // IFFT x of the tile rows, loaded straight into the transform lanes
   if (wave) {
-    tile_rows_fft_from_global<T, TW, L0, +1, true, rs>(p, ch, b, z, y0, lds);
+    tile_rows_fft_from_global<T, TW, L0, +1, true, rs, LANES>(p, ch, b, z, y0, lds);
     wave_sync();
   } else {
-    tile_rows_fft_from_global<T, TW, L0, +1, false, rs>(p, ch, b, z, y0, lds);
+    tile_rows_fft_from_global<T, TW, L0, +1, false, rs, LANES>(p, ch, b, z, y0, lds);
     __syncthreads();
   }
 #pragma unroll
@@ -936,10 +936,10 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
   // FFT x of the tile rows, each stored straight from its transform lanes
   if (wave) {
     wave_sync();
-    tile_rows_fft_to_global<T, TW, L0, -1, true, rs>(p, lds, ch, b, z, y0);
+    tile_rows_fft_to_global<T, TW, L0, -1, true, rs, LANES>(p, lds, ch, b, z, y0);
   } else {
     __syncthreads();
-    tile_rows_fft_to_global<T, TW, L0, -1, false, rs>(p, lds, ch, b, z, y0);
+    tile_rows_fft_to_global<T, TW, L0, -1, false, rs, LANES>(p, lds, ch, b, z, y0);
     __syncthreads();
   }
 }
@@ -1031,18 +1031,20 @@ __global__ void __launch_bounds__(256) k_expand(const cx<T>* __restrict__ grep, 
   ch[tid] = twmul<T, double, -1>(grep[(long)b * n_rep + (ent & 0xFFFFFu)], W[ent >> 20]);
 }
 
-// Threads per workgroup of the plane-pair kernel (512 gains 3 % at 2 fields
-// per launch and loses a third at 16: one workgroup per CU instead of two).
+// Threads per workgroup of the plane-pair kernel by default (512, a schedule
+// the tuner may choose, gained 3 % at 2 fields per launch and lost a third
+// at 16: one workgroup per CU instead of two).
 constexpr int kQNT = 256;
 
 // SEP: the separable multiply (p.qs_sep) or the general one, compiled apart so
-// that each gets its own register allocation.
-template <typename T, typename TW, int L, bool SEP>
-__global__ void __launch_bounds__(kQNT) k_p2q(PipeP p, PipeBufs B) {
-  constexpr int NFQ = (8 * (2 + 2 * L) + kQNT - 1) / kQNT * kQNT;  // whole loads of kQNT factors
+// that each gets its own register allocation.  NT threads per workgroup;
+// LANES: the lane split of the row and column transforms (0: the default).
+template <typename T, typename TW, int L, bool SEP, int NT = kQNT, int LANES = 0>
+__global__ void __launch_bounds__(NT) k_p2q(PipeP p, PipeBufs B) {
+  constexpr int NFQ = (8 * (2 + 2 * L) + NT - 1) / NT * NT;  // whole loads of NT factors
   __shared__ cx<T> lds[2 * L * kRow<T, L>];
   __shared__ cx<double> fq[NFQ];  // axis factors, FP64 in both precisions
-  ph_plane_pair_q<T, TW, L, kQNT, SEP>(p, B, blockIdx.x, lds, fq);
+  ph_plane_pair_q<T, TW, L, NT, SEP, LANES>(p, B, blockIdx.x, lds, fq);
 }
 
 // Tile item of workgroup `blk` in a grid of `nblk` (one workgroup per item).
@@ -1062,11 +1064,11 @@ __device__ inline long p51_item(long blk, long nblk, int nyb, bool half_rows) {
 
 // STORE: q(s + 1) written to B.f0_out, or (between two steps of a plain
 // sweep) not written at all.
-template <typename T, typename TW, int L0, bool STORE>
+template <typename T, typename TW, int L0, bool STORE, int LANES = 0>
 __global__ void __launch_bounds__(kNT) k_p51(PipeP p, PipeBufs B, int step) {
   __shared__ cx<T> lds[64 * kRow<T, L0>];
   const long item = p51_item(blockIdx.x, gridDim.x, p.H[1] / p.YB, p.YB * (int)sizeof(cx<T>) < 128);
-  ph_unfold_fold<T, TW, L0, STORE>(p, B, item, step, lds);
+  ph_unfold_fold<T, TW, L0, STORE, LANES>(p, B, item, step, lds);
 }
 
 // ---- persistent sweep: nsteps Strang steps in one cooperative launch ---------

@@ -281,6 +281,15 @@ int cfft_sweep_hist_dev(cfft_plan* plan, void* hist, const void* const* E1, cons
  * repeat on every contour step and bias F by about 1e-6).  The plane-pair Q
  * multiply and the interaction factors are FP64 either way. */
 #define CFFT_OPT_FP32_TWIDDLES 5
+/* Radix schedule of the path-QS plane-pair and tile transforms: the lane
+ * split R1 x R2 of a length-H transform (in-lane R2-point passes, then one
+ * cross-lane radix-R1 pass).  0 (default): R1 = 4 for H divisible by 16,
+ * else 2; 2 selects R1 = 2 for H = 16 or 32 (results differ from the default
+ * by rounding only).  Other values: CFFT_ERR_UNSUPPORTED. */
+#define CFFT_OPT_FFT_LANES 6
+/* Threads per plane-pair workgroup on path QS: 0 or 256 (default), or 512
+ * for H = 16, 24, 32 (same results bit for bit). */
+#define CFFT_OPT_PAIR_THREADS 7
 /* No counterpart in the reference. */
 int cfft_plan_set_option(cfft_plan* plan, int32_t key, int64_t value);
 
