@@ -23,7 +23,7 @@ FLAGS = [
 ]
 
 SOURCES = [os.path.join(CSRC, f) for f in ("cfft_hip.hip", "blocks.hip", "scft.hip", "mlp.hip", "dense.hip",
-                                            "ensemble.hip", "plan.cpp")]
+                                            "ensemble.hip", "p3c.hip", "plan.cpp")]
 
 
 def _run(cmd, verbose):

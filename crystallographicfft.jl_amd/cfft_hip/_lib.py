@@ -50,7 +50,8 @@ EXPORTED = [
     "cfft_scft_propagators", "cfft_scft_destroy", "cfft_scft_create_multi", "cfft_scft_set_params_inst",
     "cfft_scft_set_fields_inst", "cfft_scft_iterate_multi", "cfft_scft_solve_multi", "cfft_scft_get_inst",
     "cfft_scft_ensemble_create", "cfft_scft_ensemble_iterate", "cfft_scft_ensemble_solve",
-    "cfft_scft_ensemble_destroy",
+    "cfft_scft_ensemble_destroy", "cfft_p3c_create", "cfft_p3c_forward", "cfft_p3c_forward_dev",
+    "cfft_p3c_destroy",
     "cfft_mlp_create", "cfft_mlp_info",
     "cfft_mlp_get_params", "cfft_mlp_set_params", "cfft_mlp_forward", "cfft_mlp_gradient",
     "cfft_mlp_train", "cfft_mlp_destroy",
@@ -165,6 +166,11 @@ def _declare(lib):
         "cfft_scft_ensemble_iterate": (C.c_int, [P, PD]),
         "cfft_scft_ensemble_solve": (C.c_int, [P, D, I32, PI32, PD]),
         "cfft_scft_ensemble_destroy": (C.c_int, [P]),
+        "cfft_p3c_create": (C.c_int, [I32, I32, PI32, I32, PI64, C.POINTER(P), C.POINTER(P), C.POINTER(P),
+                                      C.POINTER(P)]),
+        "cfft_p3c_forward": (C.c_int, [P, PD, PD]),
+        "cfft_p3c_forward_dev": (C.c_int, [P, P, P, P]),
+        "cfft_p3c_destroy": (C.c_int, [P]),
         "cfft_mlp_create": (C.c_int, [I32, PI32, C.c_uint64, C.POINTER(P)]),
         "cfft_mlp_info": (C.c_int, [P, PI64]),
         "cfft_mlp_get_params": (C.c_int, [P, PF]),
@@ -211,6 +217,11 @@ def last_error() -> str:
 def check(status: int):
     if status != CFFT_OK:
         raise CFFTError(status, last_error())
+
+
+def i64p(a: np.ndarray):
+    assert a.dtype == np.int64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(PI64)
 
 
 def i32p(a: np.ndarray):

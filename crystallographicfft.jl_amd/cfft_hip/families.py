@@ -8,16 +8,20 @@ the general stride-L G0-ASU (src/recursive_blocks.jl:1378-1588), the fractal
 trees (src/fractal_krfft.jl:678-1279, src/fractal_krfft_v3.jl:205-452) and the
 staged Pm-3m decomposition (src/staged_krfft.jl:115-900).  Every one of them
 computes the spectral-ASU values F(h) of a symmetric real field u on the full
-N^3 grid (or, backward, u from them); they differ only in how the CPU work is
-ordered.  Here they share the device pipeline of `DevicePlan` (stride-L
-subgrid, centring fold, batched LDS FFTs, reconstruction): the plan builders
-keep the reference's preconditions and error behaviour, the executes return
-the same arrays.  The staged Pm-3m plan returns F000 = FFT of u(2x) on the
-(N/2)^3 grid, the quantity its reference counterpart returns.
-
-Unlike the P3c factorizations, whose twiddle formula hard-codes the cubic
-3-fold axis (src/recursive_blocks.jl:958-972), the device pipeline is exact
-for every group, so only the assertions the reference itself makes are kept.
+N^3 grid (or, backward, u from them); they differ in how the work is ordered.
+The four P3c forward families (recursive, sparse, selective, G0-ASU) run
+their own algorithm on the device when the shifted ops hold the
+body-diagonal 3-fold with an even translation (`cfft_hip.p3c`: FFTs of the
+stride-4 sub-sub-grids of one offset per 3-fold orbit, then the family's
+own tables as sparse stages, `cfft_p3c_*`); other groups, where the
+reference's P3c formula does not hold (src/recursive_blocks.jl:958-972),
+take the device pipeline of `DevicePlan` (stride-L subgrid, centring fold,
+batched LDS FFTs, reconstruction), which is exact for every group.  The
+remaining families (general G0-ASU, G0-ASU backward, fractal, staged Pm-3m)
+run on the device pipeline.  The plan builders keep the reference's
+preconditions and error behaviour, the executes return the same arrays.  The
+staged Pm-3m plan returns F000 = FFT of u(2x) on the (N/2)^3 grid, the
+quantity its reference counterpart returns.
 """
 from __future__ import annotations
 
@@ -34,6 +38,18 @@ def has_cubic_p3c_symmetry(ops) -> bool:
     3-fold rotation [0 0 1; 1 0 0; 0 1 0]."""
     target = np.array([[0, 0, 1], [1, 0, 0], [0, 1, 0]])
     return any(np.array_equal(np.rint(np.asarray(op.Rm)).astype(int), target) for op in ops)
+
+
+# sub-sub-grid FFT lengths of the P3c engine (csrc/p3c.hip)
+_P3C_LENS = (2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64)
+
+
+def p3c_applies(ops_shifted, n) -> bool:
+    """The 3-fold rotation [0 0 1; 1 0 0; 0 1 0] with an even translation
+    (mod n): the condition under which the P3c sub-sub-grid relation holds."""
+    target = np.array([[0, 0, 1], [1, 0, 0], [0, 1, 0]])
+    return any(np.array_equal(np.rint(np.asarray(op.Rm)).astype(int), target)
+               and np.all(np.mod(np.asarray(op.tv), n) % 2 == 0) for op in ops_shifted)
 
 
 def _check_p3c(spec: SpectralIndexing, what: str):
@@ -67,13 +83,38 @@ class _ForwardFamilyPlan:
         u = np.asarray(u, dtype=np.float64)
         if u.shape != self.grid_N:
             raise L.CFFTError(L.CFFT_ERR_SIZE, f"field shape {u.shape} does not match N = {self.grid_N}")
+        if getattr(self, "p3c", None) is not None:
+            self.p3c.forward(u, self.output_buffer)
+            return self.output_buffer
         f0 = self.dev.full_to_sub(u)  # stride-L extraction on the device (k_full_to_sub)
         self.output_buffer[:] = self.dev.forward(f0)
         return self.output_buffer
 
+    def _attach_p3c(self, kind, ops_shifted):
+        """The family's own P3c tables and device engine when the ops allow
+        them (cubic grid, the body-diagonal 3-fold with an even translation,
+        an FFT length N/4 the device supports); otherwise the device pipeline
+        stays in charge and `p3c` is None."""
+        from . import p3c
+        self.p3c = None
+        self.p3c_tables = None
+        n = self.grid_N[0]
+        if not (self.grid_N == (n, n, n) and n % 4 == 0 and p3c_applies(ops_shifted, n)):
+            return
+        if n // 4 not in _P3C_LENS:
+            return
+        cls, stages, info = p3c.family_tables(kind, self.spec, ops_shifted)
+        self.p3c = p3c.P3cEngine(n, cls, stages)
+        self.p3c_tables = dict(info, representatives=len(cls[3]), nnz=self.p3c.nnz)
+
 
 class RecursiveKRFFTPlan(_ForwardFamilyPlan):
-    """src/recursive_blocks.jl:35-56."""
+    """src/recursive_blocks.jl:35-56: G0 at every M-grid point from the
+    sub-sub-grid FFTs, then the A8 sum (`cfft_hip.p3c`)."""
+
+    def __init__(self, spec, ops_shifted):
+        super().__init__(spec, ops_shifted)
+        self._attach_p3c("recursive", ops_shifted)
 
 
 def plan_krfft_recursive(spec: SpectralIndexing, ops_shifted) -> RecursiveKRFFTPlan:
@@ -88,11 +129,15 @@ def execute_recursive_krfft(plan: RecursiveKRFFTPlan, spec: SpectralIndexing, u)
 
 
 class SparseKRFFTPlan(_ForwardFamilyPlan):
-    """src/recursive_blocks.jl:447-512."""
+    """src/recursive_blocks.jl:447-512: one CSR table from the sub-sub-grid
+    FFT outputs straight to the spectral ASU (`cfft_hip.p3c`).  `centering`
+    is recorded; the merged table keeps the entries the reference drops as
+    reading identically-zero FFT outputs (they multiply zeros)."""
 
     def __init__(self, spec, ops_shifted, centering):
         super().__init__(spec, ops_shifted)
         self.centering = centering
+        self._attach_p3c("sparse", ops_shifted)
 
 
 def plan_krfft_sparse(spec: SpectralIndexing, ops_shifted, centering: str = "auto") -> SparseKRFFTPlan:
@@ -110,7 +155,12 @@ def execute_sparse_krfft(plan: SparseKRFFTPlan, spec: SpectralIndexing, u):
 
 
 class SelectiveG0Plan(_ForwardFamilyPlan):
-    """src/recursive_blocks.jl:725-754."""
+    """src/recursive_blocks.jl:725-754: G0 only at the positions the A8 sum
+    reads (`cfft_hip.p3c`)."""
+
+    def __init__(self, spec, ops_shifted):
+        super().__init__(spec, ops_shifted)
+        self._attach_p3c("selective", ops_shifted)
 
 
 def plan_krfft_selective(spec: SpectralIndexing, ops_shifted) -> SelectiveG0Plan:
@@ -125,7 +175,13 @@ def execute_selective_krfft(plan: SelectiveG0Plan, spec: SpectralIndexing, u):
 
 
 class G0ASUPlan(_ForwardFamilyPlan):
-    """src/recursive_blocks.jl:1002-1032."""
+    """src/recursive_blocks.jl:1002-1032: G0 at the orbit representatives of
+    the selective positions under the ops with even translations, the
+    symmetry phases merged into the A8 weights (`cfft_hip.p3c`)."""
+
+    def __init__(self, spec, ops_shifted):
+        super().__init__(spec, ops_shifted)
+        self._attach_p3c("g0asu", ops_shifted)
 
 
 class GeneralG0ASUPlan(_ForwardFamilyPlan):

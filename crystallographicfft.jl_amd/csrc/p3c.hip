@@ -1,0 +1,202 @@
+// P3c KRFFT families on the device (the reference's recursive, sparse,
+// selective-G0 and G0-ASU forward plans, src/recursive_blocks.jl:64-1240):
+// for a cubic group whose shifted ops hold the body-diagonal 3-fold C
+// (x, y, z) -> (z, x, y) with an even translation, the stride-2 subgrid's
+// spectrum G0 follows from the FFTs of the stride-4 sub-sub-grids of one
+// offset per orbit of the 8 offsets under C, and the spectral-ASU values
+// from G0 through the 8 representative ops.  The host builds the family's
+// tables as CSR matrices (one stage: FFT outputs -> spectral ASU; two
+// stages: FFT outputs -> G0 at the family's positions -> spectral ASU); the
+// device packs the sub-sub-grids, runs their FFTs and the sparse products.
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "../../include/cfft_hip.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+using namespace cfft;
+
+struct cfft_p3c {
+  int N = 0, M2 = 0, n_rep = 0, n_stage = 0;
+  int off[8][3] = {};          // sub-sub-grid offsets e of the representatives (u at 4 k + 2 e)
+  long n_out[2] = {0, 0};      // rows of each stage; the last stage's rows are the spectral points
+  long n_in0 = 0;              // n_rep M2^3
+  DevBuf rowptr[2], cols[2], vals[2];
+  DevBuf u, g, mid, F;         // full grid, FFT outputs, G0 values, spectral values
+  hipStream_t stream = nullptr;
+  ~cfft_p3c() {
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+  }
+};
+
+namespace {
+
+struct Offs {
+  int e[8][3];
+};
+
+// g[r][k] = u(4 k + 2 e_r), x fastest, as complex
+__global__ void k_p3c_pack(const double* __restrict__ u, cx<double>* __restrict__ g, int N, int M2, int n_rep,
+                           Offs o) {
+  const long n = (long)M2 * M2 * M2;
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n * n_rep) return;
+  const int r = (int)(i / n);
+  const long k = i - (long)r * n;
+  const int kx = (int)(k % M2), ky = (int)((k / M2) % M2), kz = (int)(k / ((long)M2 * M2));
+  const long x = 4 * kx + 2 * o.e[r][0], y = 4 * ky + 2 * o.e[r][1], z = 4 * kz + 2 * o.e[r][2];
+  g[i] = {u[x + (long)N * (y + (long)N * z)], 0.0};
+}
+
+// y = A x (complex CSR): eight lanes per row, partial sums reduced across the lane octet
+__global__ void k_p3c_spmv(const long* __restrict__ rowptr, const int* __restrict__ cols,
+                           const cx<double>* __restrict__ vals, const cx<double>* __restrict__ x,
+                           cx<double>* __restrict__ y, long n_rows) {
+  const long row = (blockIdx.x * (long)blockDim.x + threadIdx.x) >> 3;
+  const int lane = threadIdx.x & 7;
+  const bool ok = row < n_rows;
+  const long b = ok ? rowptr[row] : 0, e = ok ? rowptr[row + 1] : 0;
+  cx<double> s = {0.0, 0.0};
+  for (long j = b + lane; j < e; j += 8) s = cadd(s, cmul(vals[j], x[cols[j]]));
+  s = octet_sum(s);
+  if (ok && lane == 0) y[row] = s;
+}
+
+template <int LEN>
+void fft3(cx<double>* g, int n_rep, hipStream_t s) {
+  const long M2 = LEN, n = M2 * M2 * M2;
+  const long n_tr = n_rep * M2 * M2;
+  const dim3 grid(nblk(n_tr, MLGeom<LEN>::G)), block(MLGeom<LEN>::WG);
+  // x (stride 1), y (stride M2), z (stride M2^2); transform i starts at (i % inner) + (i / inner) outer
+  hipLaunchKernelGGL((k_fft_ml<double, double, LEN, -1>), grid, block, 0, s, g, n_tr, 1L, M2, 1L);
+  hipLaunchKernelGGL((k_fft_ml<double, double, LEN, -1>), grid, block, 0, s, g, n_tr, M2, M2 * M2, M2);
+  hipLaunchKernelGGL((k_fft_ml<double, double, LEN, -1>), grid, block, 0, s, g, n_tr, M2 * M2, n, M2 * M2);
+  (void)n;
+}
+
+void run(cfft_p3c* h, const double* u, double* F, hipStream_t s) {
+  cx<double>* g = static_cast<cx<double>*>(h->g.p);
+  Offs o{};
+  for (int r = 0; r < 8; ++r)
+    for (int d = 0; d < 3; ++d) o.e[r][d] = h->off[r][d];
+  hipLaunchKernelGGL(k_p3c_pack, dim3(nblk(h->n_in0, 256)), dim3(256), 0, s, u, g, h->N, h->M2, h->n_rep, o);
+  hip_check(hipGetLastError(), "k_p3c_pack");
+  switch (h->M2) {
+    case 2: fft3<2>(g, h->n_rep, s); break;
+    case 3: fft3<3>(g, h->n_rep, s); break;
+    case 4: fft3<4>(g, h->n_rep, s); break;
+    case 6: fft3<6>(g, h->n_rep, s); break;
+    case 8: fft3<8>(g, h->n_rep, s); break;
+    case 12: fft3<12>(g, h->n_rep, s); break;
+    case 16: fft3<16>(g, h->n_rep, s); break;
+    case 24: fft3<24>(g, h->n_rep, s); break;
+    case 32: fft3<32>(g, h->n_rep, s); break;
+    case 48: fft3<48>(g, h->n_rep, s); break;
+    case 64: fft3<64>(g, h->n_rep, s); break;
+    default: throw Error{CFFT_ERR_UNSUPPORTED, "sub-sub-grid FFT length not supported"};
+  }
+  hip_check(hipGetLastError(), "k_fft_ml");
+  const cx<double>* x = g;
+  for (int st = 0; st < h->n_stage; ++st) {
+    cx<double>* y = st + 1 == h->n_stage ? reinterpret_cast<cx<double>*>(F) : static_cast<cx<double>*>(h->mid.p);
+    hipLaunchKernelGGL(k_p3c_spmv, dim3(nblk(8 * h->n_out[st], 256)), dim3(256), 0, s,
+                       static_cast<const long*>(h->rowptr[st].p), static_cast<const int*>(h->cols[st].p),
+                       static_cast<const cx<double>*>(h->vals[st].p), x, y, h->n_out[st]);
+    hip_check(hipGetLastError(), "k_p3c_spmv");
+    x = y;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfft_p3c_create(int32_t N, int32_t n_rep, const int32_t* offsets, int32_t n_stage, const int64_t* n_rows,
+                    const int64_t* const* rowptr, const int32_t* const* cols, const double* const* vals,
+                    cfft_p3c** out) {
+  return guarded([&] {
+    require(out && offsets && n_rows && rowptr && cols && vals, CFFT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    require(N > 0 && N % 4 == 0, CFFT_ERR_NOT_DIVISIBLE, "M must be divisible by 2 for P3c");
+    require(fft_supported(N / 4), CFFT_ERR_UNSUPPORTED, "sub-sub-grid FFT length not supported");
+    require(n_rep >= 1 && n_rep <= 8, CFFT_ERR_INVALID, "1 to 8 sub-sub-grid representatives");
+    require(n_stage == 1 || n_stage == 2, CFFT_ERR_INVALID, "one or two sparse stages");
+    std::unique_ptr<cfft_p3c> h(new cfft_p3c());
+    h->N = N;
+    h->M2 = N / 4;
+    h->n_rep = n_rep;
+    h->n_stage = n_stage;
+    for (int r = 0; r < n_rep; ++r)
+      for (int d = 0; d < 3; ++d) {
+        require(offsets[3 * r + d] == 0 || offsets[3 * r + d] == 1, CFFT_ERR_INVALID, "offsets are 0 or 1");
+        h->off[r][d] = offsets[3 * r + d];
+      }
+    h->n_in0 = (long)n_rep * h->M2 * h->M2 * h->M2;
+    long n_in = h->n_in0;
+    for (int st = 0; st < n_stage; ++st) {
+      const long nr = n_rows[st];
+      require(nr >= 1 && rowptr[st] && cols[st] && vals[st], CFFT_ERR_INVALID, "bad stage");
+      require(rowptr[st][0] == 0, CFFT_ERR_INVALID, "row pointers start at 0");
+      for (long i = 0; i < nr; ++i)
+        require(rowptr[st][i + 1] >= rowptr[st][i], CFFT_ERR_INVALID, "row pointers must not decrease");
+      const long nnz = rowptr[st][nr];
+      for (long j = 0; j < nnz; ++j)
+        require(cols[st][j] >= 0 && cols[st][j] < n_in, CFFT_ERR_BOUNDS, "column index out of range");
+      h->n_out[st] = nr;
+      h->rowptr[st].alloc(sizeof(int64_t) * (nr + 1));
+      h->cols[st].alloc(sizeof(int32_t) * std::max<long>(nnz, 1));
+      h->vals[st].alloc(16 * std::max<long>(nnz, 1));
+      std::lock_guard<std::recursive_mutex> guard(setup_mutex());
+      hip_check(hipMemcpy(h->rowptr[st].p, rowptr[st], sizeof(int64_t) * (nr + 1), hipMemcpyHostToDevice), "upload");
+      if (nnz > 0) {
+        hip_check(hipMemcpy(h->cols[st].p, cols[st], sizeof(int32_t) * nnz, hipMemcpyHostToDevice), "upload");
+        hip_check(hipMemcpy(h->vals[st].p, vals[st], 16 * nnz, hipMemcpyHostToDevice), "upload");
+      }
+      n_in = nr;
+    }
+    h->u.alloc(sizeof(double) * (size_t)N * N * N);
+    h->g.alloc(16 * (size_t)h->n_in0);
+    if (n_stage == 2) h->mid.alloc(16 * (size_t)h->n_out[0]);
+    h->F.alloc(16 * (size_t)h->n_out[n_stage - 1]);
+    {
+      std::lock_guard<std::recursive_mutex> guard(setup_mutex());
+      hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+    }
+    *out = h.release();
+  });
+}
+
+int cfft_p3c_forward(cfft_p3c* h, const double* u, double* F) {
+  return guarded([&] {
+    require(h && u && F, CFFT_ERR_INVALID, "null argument");
+    const size_t nu = sizeof(double) * (size_t)h->N * h->N * h->N;
+    const size_t nf = 16 * (size_t)h->n_out[h->n_stage - 1];
+    hip_check(hipMemcpyAsync(h->u.p, u, nu, hipMemcpyHostToDevice, h->stream), "H2D");
+    run(h, static_cast<const double*>(h->u.p), static_cast<double*>(h->F.p), h->stream);
+    hip_check(hipMemcpyAsync(F, h->F.p, nf, hipMemcpyDeviceToHost, h->stream), "D2H");
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+  });
+}
+
+int cfft_p3c_forward_dev(cfft_p3c* h, const double* u, double* F, void* stream) {
+  return guarded([&] {
+    require(h && u && F, CFFT_ERR_INVALID, "null argument");
+    run(h, u, F, static_cast<hipStream_t>(stream));
+  });
+}
+
+int cfft_p3c_destroy(cfft_p3c* h) {
+  return guarded([&] {
+    if (!h) return;
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
+    delete h;
+  });
+}
+
+}  // extern "C"

@@ -449,6 +449,31 @@ int cfft_scft_ensemble_solve(cfft_scft_ensemble* e, double tol, int32_t max_iter
                              double* out);
 int cfft_scft_ensemble_destroy(cfft_scft_ensemble* e);
 
+/* ---- P3c KRFFT families (src/recursive_blocks.jl:64-1323) ------------------
+ * The recursive, sparse, selective-G0 and G0-ASU forward plans of the
+ * reference for cubic groups: the spectral-ASU values from the FFTs of the
+ * stride-4 sub-sub-grids u(4 k + 2 e) of one offset e per orbit of the 8
+ * offsets under the body-diagonal 3-fold.  The host builds the family's
+ * tables (src/recursive_blocks.jl:514-629 sparse, :64-144 recursive, :756-888
+ * selective, :1034-1244 G0-ASU) as CSR matrices of complex (re, im) values:
+ * stage 0 maps the n_rep M2^3 FFT outputs (representative r's block at
+ * r M2^3, x fastest; M2 = N/4) to n_rows[0] values, stage 1 (when n_stage =
+ * 2) maps those to n_rows[1]; the last stage's rows are the spectral points.
+ * offsets: n_rep x 3 entries 0 or 1.  rowptr[s]: n_rows[s] + 1 int64
+ * entries from 0; cols[s] (int32) and vals[s] (2 doubles each): rowptr[s]
+ * [n_rows[s]] entries.  Columns out of range: CFFT_ERR_BOUNDS. */
+typedef struct cfft_p3c cfft_p3c;
+int cfft_p3c_create(int32_t N, int32_t n_rep, const int32_t* offsets, int32_t n_stage, const int64_t* n_rows,
+                    const int64_t* const* rowptr, const int32_t* const* cols, const double* const* vals,
+                    cfft_p3c** out);
+/* u: N^3 real full grid (x fastest); F: 2 n_spec doubles.  Host pointers,
+ * blocking (execute_*_krfft!, src/recursive_blocks.jl:323-330, 631-670,
+ * 890-972, 1246-1323). */
+int cfft_p3c_forward(cfft_p3c* plan, const double* u, double* F);
+/* Device pointers, on `stream`, without synchronising. */
+int cfft_p3c_forward_dev(cfft_p3c* plan, const double* u, double* F, void* stream);
+int cfft_p3c_destroy(cfft_p3c* plan);
+
 /* ---- warm-start network (multilayer perceptron) ---------------------------- */
 /* No counterpart in the reference.  dims[0..n_layers]: input width, hidden
  * widths, output width.  SiLU hidden activations, linear output.  Parameters:

@@ -1,6 +1,7 @@
 """The reference's other KRFFT families (recursive P3c, sparse, selective,
 G0-ASU forward/backward, general G0-ASU, fractal, staged Pm-3m) on the device
-pipeline: the plan builders' preconditions on the CPU (test/test_g0_asu.jl,
+(the four P3c forward families through their own tables on cubic groups,
+the rest on the device pipeline): the plan builders' preconditions on the CPU (test/test_g0_asu.jl,
 test/test_g0_asu_backward.jl, test/test_staged_pm3m.jl check the same
 assertions), and on the GPU the outputs against the full-grid FFT of the
 symmetrised field at the spectral-ASU points (the check of
@@ -57,16 +58,27 @@ FORWARD = [(cf.plan_krfft_recursive, cf.execute_recursive_krfft),
            (cf.plan_krfft_g0asu_general, cf.execute_general_g0asu_krfft)]
 
 
+P3C_FAMILIES = (cf.plan_krfft_recursive, cf.plan_krfft_sparse, cf.plan_krfft_selective, cf.plan_krfft_g0asu)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("sg,N", [(229, (32, 32, 32)), (221, (16, 16, 16)), (47, (16, 16, 16)),
-                                  (230, (24, 24, 24)), (70, (16, 16, 16))])
+                                  (230, (24, 24, 24)), (70, (16, 16, 16)), (225, (64, 64, 64)),
+                                  (227, (32, 32, 32)), (223, (48, 48, 48))])
 def test_forward_families_match_full_fft(sg, N):
+    """Every forward family against the full FFT at the spectral ASU (1e-12
+    relative); on cubic groups the four P3c families run their own tables on
+    the device (cfft_p3c_*), on the others the device pipeline."""
+    print(f"families sg={sg} N={N}", flush=True)
     ops_s, R, t, spec = _spec(sg, N)
     u = O.make_symmetric(R, t, N, seed=21)
     ref = np.fft.fftn(u)[tuple(spec.idx[:, d] for d in range(3))]
     scale = np.max(np.abs(ref))
+    cubic = cf.has_cubic_p3c_symmetry(ops_s)
     for planner, execute in FORWARD:
         plan = planner(spec, ops_s)
+        if planner in P3C_FAMILIES:
+            assert (getattr(plan, "p3c", None) is not None) == cubic, planner.__name__
         out = execute(plan, spec, u)
         assert out is plan.output_buffer
         assert np.max(np.abs(out - ref)) / scale < 1e-12, planner.__name__
