@@ -310,7 +310,7 @@ def scft_ia3d(cf, tol64, tol32, max_iter, cell=3.8):
     t0 = time.perf_counter()
     ia, oa = s32.solve(tol=hand, max_iter=max_iter)
     fin = cf.scft.SCFTSolver(n3, ops_s, lat, 0.35, 18.0, 200, cf.FP64, m_hist=10)
-    fin.set_fields(*s32.fields())
+    fin.continue_from(s32)  # fields and Anderson history
     ib, ob = fin.solve(tol=tol64, max_iter=max_iter)
     e_mf = time.perf_counter() - t0
     del s32, fin

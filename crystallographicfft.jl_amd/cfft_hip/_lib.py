@@ -49,7 +49,8 @@ EXPORTED = [
     "cfft_scft_set_fields", "cfft_scft_set_params", "cfft_scft_iterate", "cfft_scft_solve", "cfft_scft_get",
     "cfft_scft_propagators", "cfft_scft_destroy", "cfft_scft_create_multi", "cfft_scft_set_params_inst",
     "cfft_scft_set_fields_inst", "cfft_scft_iterate_multi", "cfft_scft_solve_multi", "cfft_scft_get_inst",
-    "cfft_scft_ensemble_create", "cfft_scft_ensemble_iterate", "cfft_scft_ensemble_solve",
+    "cfft_scft_copy_state", "cfft_scft_ensemble_create", "cfft_scft_ensemble_iterate",
+    "cfft_scft_ensemble_solve",
     "cfft_scft_ensemble_destroy", "cfft_p3c_create", "cfft_p3c_forward", "cfft_p3c_forward_dev",
     "cfft_p3c_destroy",
     "cfft_mlp_create", "cfft_mlp_info",
@@ -162,6 +163,7 @@ def _declare(lib):
         "cfft_scft_iterate_multi": (C.c_int, [P, PD]),
         "cfft_scft_solve_multi": (C.c_int, [P, D, I32, PI32, PD]),
         "cfft_scft_get_inst": (C.c_int, [P, I32, I32, PD]),
+        "cfft_scft_copy_state": (C.c_int, [P, P]),
         "cfft_scft_ensemble_create": (C.c_int, [C.POINTER(P), I32, C.POINTER(P)]),
         "cfft_scft_ensemble_iterate": (C.c_int, [P, PD]),
         "cfft_scft_ensemble_solve": (C.c_int, [P, D, I32, PI32, PD]),

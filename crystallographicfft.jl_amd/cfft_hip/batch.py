@@ -229,7 +229,8 @@ def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8,
                 finish_tol=None, group=True, tune=False, ensemble=False, mixing=BATCH_MIXING):
     """Plan every instance (or group of instances), then solve all of them
     concurrently.  With finish_tol, each instance continues in FP64 from its
-    `precision` solution until the residual is below finish_tol (its FP64
+    `precision` solution, fields and Anderson history handed over
+    (``continue_from``), until the residual is below finish_tol (its FP64
     plan is made in the planning phase).  The path follows
     ``scft.default_path`` for the stage's tolerance.  With tune (grouped
     only), each group's launch schedule is timed on its own workload
@@ -277,7 +278,7 @@ def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8,
         it, out = solver.solve(tol=tol, max_iter=max_iter)
         r = _result(it, out)
         if fin is not None:
-            fin.set_fields(*solver.fields())
+            fin.continue_from(solver)
             it2, o2 = fin.solve(tol=finish_tol, max_iter=max_iter)
             _finish(r, it2, o2, out)
         return r
@@ -289,8 +290,7 @@ def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8,
         its, outs = solver.solve(tol=tol, max_iter=max_iter)
         rs = [_result(it, out) for it, out in zip(its, outs)]
         if fin is not None:
-            for i in range(solver.n):
-                fin.set_fields(i, *solver.fields(i))
+            fin.continue_from(solver)
             its2, outs2 = fin.solve(tol=finish_tol, max_iter=max_iter)
             for r, it2, o2, out in zip(rs, its2, outs2, outs):
                 _finish(r, it2, o2, out)
@@ -308,8 +308,7 @@ def solve_batch(instances, precision=L.MIXED, tol=1e-6, max_iter=400, workers=8,
             fins = [fin for _, fin, _ in items]
             ens2 = SCFTEnsemble(fins)
             for sv, fin, _ in items:
-                for i in range(sv.n):
-                    fin.set_fields(i, *sv.fields(i))
+                fin.continue_from(sv)
             res2 = ens2.solve(tol=finish_tol, max_iter=max_iter)
             for rs, (its2, outs2), (_, outs) in zip(rss, res2, res):
                 for r, it2, o2, out in zip(rs, its2, outs2, outs):

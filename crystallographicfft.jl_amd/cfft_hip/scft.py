@@ -200,6 +200,13 @@ class SCFTSolver:
     def set_mixing(self, lam=1.0, lam_simple=0.1, simple_above=1e-2):
         L.check(L.lib().cfft_scft_set_mixing(self._h, float(lam), float(lam_simple), float(simple_above)))
 
+    def continue_from(self, other):
+        """Take over another solver's fields and Anderson history (same
+        subgrid, instance count and history length; cfft_scft_copy_state):
+        a mixed-precision stage handed to an FP64 finish keeps mixing
+        instead of restarting from simple mixing."""
+        L.check(L.lib().cfft_scft_copy_state(self._h, other._h))
+
     def set_fields(self, wA, wB):
         a = np.ascontiguousarray(np.asarray(wA, dtype=np.float64).ravel(order="F"))
         b = np.ascontiguousarray(np.asarray(wB, dtype=np.float64).ravel(order="F"))
@@ -287,6 +294,13 @@ class SCFTBatch:
 
     def set_mixing(self, lam=1.0, lam_simple=0.1, simple_above=1e-2):
         L.check(L.lib().cfft_scft_set_mixing(self._h, float(lam), float(lam_simple), float(simple_above)))
+
+    def continue_from(self, other):
+        """Take over another solver's fields and Anderson history (same
+        subgrid, instance count and history length; cfft_scft_copy_state):
+        a mixed-precision stage handed to an FP64 finish keeps mixing
+        instead of restarting from simple mixing."""
+        L.check(L.lib().cfft_scft_copy_state(self._h, other._h))
 
     def set_fields(self, i, wA, wB):
         a = np.ascontiguousarray(np.asarray(wA, dtype=np.float64).ravel(order="F"))

@@ -827,6 +827,31 @@ int cfft_scft_get_inst(cfft_scft* c, int32_t inst, int32_t which, double* out) {
   return guarded([&] { get_inst(c, instance(c, inst), which, out); });
 }
 
+int cfft_scft_copy_state(cfft_scft* dst, cfft_scft* src) {
+  return guarded([&] {
+    require(dst && src && dst != src, CFFT_ERR_INVALID, "two distinct contexts");
+    require(dst->M3 == src->M3 && dst->in.size() == src->in.size() && dst->m == src->m, CFFT_ERR_INVALID,
+            "the contexts must share the subgrid, the instance count and the history length");
+    // the fields and the Anderson history (FP64 in both precisions): the
+    // destination's next iteration continues the source's mixing
+    hip_check(hipStreamSynchronize(src->stream), "sync");
+    hip_check(hipStreamSynchronize(dst->stream), "sync");
+    const size_t n2 = 2 * (size_t)dst->M3;
+    for (size_t i = 0; i < dst->in.size(); ++i) {
+      ScftInst& d = *dst->in[i];
+      const ScftInst& s = *src->in[i];
+      hip_check(hipMemcpyAsync(d.w.p, s.w.p, 8 * n2, hipMemcpyDeviceToDevice, dst->stream), "copy w");
+      hip_check(hipMemcpyAsync(d.Wr.p, s.Wr.p, d.Wr.bytes, hipMemcpyDeviceToDevice, dst->stream), "copy W");
+      hip_check(hipMemcpyAsync(d.Dr.p, s.Dr.p, d.Dr.bytes, hipMemcpyDeviceToDevice, dst->stream), "copy D");
+      d.order = s.order;
+      d.G = s.G;
+      d.next_slot = s.next_slot;
+      d.failed = s.failed;
+    }
+    hip_check(hipStreamSynchronize(dst->stream), "sync");
+  });
+}
+
 int cfft_scft_propagators(cfft_scft* c, void** hist_out) {
   return guarded([&] {
     require(c && hist_out, CFFT_ERR_INVALID, "null argument");

@@ -423,6 +423,13 @@ int cfft_scft_iterate_multi(cfft_scft* ctx, double* out);
  * iters_out[n_inst], out[8 n_inst]. */
 int cfft_scft_solve_multi(cfft_scft* ctx, double tol, int32_t max_iter, int32_t* iters_out, double* out);
 int cfft_scft_get_inst(cfft_scft* ctx, int32_t inst, int32_t which, double* out);
+/* Hand a solve over between contexts of the same subgrid, instance count and
+ * Anderson history length (typically a mixed-precision stage and its FP64
+ * finish; no counterpart in the reference): each instance's fields, history
+ * slots (w and deviations, FP64 in both precisions) and Gram matrix are
+ * copied from src to dst, so dst's next iteration continues src's mixing
+ * instead of restarting it from simple mixing. */
+int cfft_scft_copy_state(cfft_scft* dst, cfft_scft* src);
 
 /* Several contexts advanced together (no counterpart in the reference; the
  * reference's only parallelism is threads over blocks, src/execution.jl:20,

@@ -213,7 +213,7 @@ def test_ia3d_ordered_gyroid_mixed_vs_fp64_solve():
     hand.set_fields(wA, wB)
     hand.solve(tol=QS_MIXED_TOL, max_iter=600)
     fin = mk(cf.FP64)
-    fin.set_fields(*hand.fields())
+    fin.continue_from(hand)
     _, of = fin.solve(tol=1e-8, max_iter=600)
     print(f"  mixed then FP64: dF {abs(of['F'] - o64['F']):.2e}", flush=True)
     assert of["res"] < 1e-8 and abs(of["F"] - o64["F"]) < 2e-9

@@ -493,3 +493,49 @@ def test_trained_warm_start_gains_over_cold_start():
     assert summ["max_abs_dF"] < 1e-6
     assert summ["iter_ratio"] < 0.6
     assert summ["time_ratio"] <= 0.5
+
+
+def test_continue_from_hands_over_the_mixing():
+    """cfft_scft_copy_state: a context that takes over another's fields and
+    Anderson history continues its iterations bit for bit (FP64 to FP64,
+    one- and two-instance contexts); a mixed-precision stage handed to an
+    FP64 finish this way needs fewer FP64 iterations than restarting the
+    mixing from its fields, and reaches the same solution."""
+    from cfft_hip import batch
+    from cfft_hip.scft import SCFTBatch, SCFTSolver
+    N = (32, 32, 32)
+    shift, ops_s = cf.find_optimal_shift(cf.get_ops(229, 3, N), N)
+    lat = 4.0 * np.eye(3)
+    insts = [batch.Instance(N, 229, 4.0, 0.3, 20.0, 40, s) for s in (31, 32)]
+    starts = [batch.start_fields(i, ops_s, shift) for i in insts]
+    a = SCFTSolver(N, ops_s, lat, 0.3, 20.0, 40, cf.FP64)
+    a.set_fields(*starts[0])
+    for _ in range(12):
+        a.iterate()
+    b = SCFTSolver(N, ops_s, lat, 0.3, 20.0, 40, cf.FP64)
+    b.continue_from(a)
+    for _ in range(6):
+        assert a.iterate() == b.iterate()
+    m1 = SCFTBatch(N, ops_s, lat, [0.3, 0.3], [20.0, 20.0], 40, cf.FP64)
+    for i, w in enumerate(starts):
+        m1.set_fields(i, *w)
+    for _ in range(12):
+        m1.iterate()
+    m2 = SCFTBatch(N, ops_s, lat, [0.3, 0.3], [20.0, 20.0], 40, cf.FP64)
+    m2.continue_from(m1)
+    for _ in range(6):
+        assert m1.iterate() == m2.iterate()
+    # mixed stage to 1e-5, then FP64 to 1e-9: hand-off against restart
+    s32 = SCFTSolver(N, ops_s, lat, 0.3, 20.0, 40, cf.MIXED, path=cf.PATH_QS)
+    s32.set_fields(*starts[0])
+    s32.solve(tol=1e-5, max_iter=300)
+    f1 = SCFTSolver(N, ops_s, lat, 0.3, 20.0, 40, cf.FP64)
+    f1.continue_from(s32)
+    i1, o1 = f1.solve(tol=1e-9, max_iter=300)
+    f2 = SCFTSolver(N, ops_s, lat, 0.3, 20.0, 40, cf.FP64)
+    f2.set_fields(*s32.fields())
+    i2, o2 = f2.solve(tol=1e-9, max_iter=300)
+    print(f"  FP64 finish: {i1} iterations with the history, {i2} restarting", flush=True)
+    assert o1["res"] < 1e-9 and o2["res"] < 1e-9
+    assert i1 < i2
+    assert abs(o1["F"] - o2["F"]) < 1e-9
