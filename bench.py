@@ -439,6 +439,31 @@ def batched_instances(cf, n, seed, workers):
     }
 
 
+def ensemble_iterations(cf, n, seed, iters):
+    """The batched-instances workload built as one ensemble (contexts of one
+    grid and group each, planned one after another on this thread) and
+    advanced `iters` iterations in FP64, then in mixed precision: the launch
+    sequence of every solve_batch(..., ensemble=True) iteration, for a
+    kernel trace.  Returns the wall time per iteration."""
+    from cfft_hip import batch
+    from cfft_hip.scft import SCFTEnsemble
+    insts = batch.draw_instances(n, seed)
+    out = {}
+    for name, prec, tol in (("fp64", cf.FP64, 1e-8), ("mixed", cf.MIXED, 1e-5)):
+        prep = [batch._prepare_group([insts[k] for k in g], prec, 10, False, tol) for g in batch.groups_of(insts)]
+        ens = SCFTEnsemble([p[0] for p in prep])
+        for sv, _, starts in prep:
+            sv.set_mixing(**batch.BATCH_MIXING)
+            for i, (wA, wB) in enumerate(starts):
+                sv.set_fields(i, wA, wB)
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            ens.iterate()
+        out[name + "_ms_per_iteration"] = (time.perf_counter() - t0) / iters * 1e3
+        del ens, prep
+    return out
+
+
 def config_pmmm_64(cf, torch):
     """Pmmm (#47) 64^3, cell 4.0, ds 0.01, one field (the reference's
     CPU-runnable configuration): device-resident forward and backward KRFFT
@@ -552,6 +577,9 @@ def main():
     ap.add_argument("--warm-eval", type=int, default=12, help="held-out sets for the warm start")
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--wide", type=int, default=16, help="fields per launch of the batched measurement")
+    ap.add_argument("--ensemble-iters", type=int, default=0,
+                    help="profiling aid: only build the 64-instance batch as one ensemble and run this many "
+                         "iterations in FP64, then in mixed precision, from one host thread")
     args = ap.parse_args()
     if args.gpus > 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -563,6 +591,10 @@ def main():
     import cfft_hip as cf
 
     torch.cuda.set_device(0)
+    if args.ensemble_iters > 0:
+        print(json.dumps({"ensemble_iterations": ensemble_iterations(cf, args.batch, 4, args.ensemble_iters)}),
+              flush=True)
+        return
     # one plain 64 MiB device copy: the reference point that scales the HBM
     # byte counters of a profiler run of this script (traffic per launch)
     src = torch.empty(8 << 20, dtype=torch.float64, device="cuda")
