@@ -439,6 +439,50 @@ def batched_instances(cf, n, seed, workers):
     }
 
 
+def families_fm3m(cf, torch, n=128, reps=100):
+    """The sparse-stage KRFFT families (P3c: recursive, sparse, selective-G0,
+    G0-ASU; the fractal decimation tree) on Fm-3m (#225) n^3, device-resident
+    forward u -> F at the spectral ASU (us per call, back to back after one
+    warm-up), beside the device pipeline's forward of the stride-2 subgrid of
+    the same field; plan time and table sizes per family."""
+    from cfft_hip import families as FAM
+    N3 = (n, n, n)
+    _, ops_s = cf.find_optimal_shift(cf.get_ops(225, 3, N3), N3)
+    spec = cf.calc_spectral_asu(ops_s, 3, N3)
+    rng = np.random.default_rng(225)
+    u = torch.from_numpy(rng.standard_normal(n ** 3)).to("cuda")  # timing only: any field
+    F = torch.empty(2 * len(spec.points), dtype=torch.float64, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+
+    def timed(call):
+        call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e6
+
+    out = {"grid": list(N3), "n_spec": len(spec.points)}
+    for name, planner in (("recursive", FAM.plan_krfft_recursive), ("sparse", FAM.plan_krfft_sparse),
+                          ("selective", FAM.plan_krfft_selective), ("g0asu", FAM.plan_krfft_g0asu),
+                          ("fractal", FAM.plan_fractal_krfft)):
+        t0 = time.perf_counter()
+        plan = planner(spec, ops_s)
+        plan_s = time.perf_counter() - t0
+        eng = plan.p3c
+        out[name] = {"us_per_forward": timed(lambda: eng.forward_dev(u.data_ptr(), F.data_ptr(), s)),
+                     "plan_s": plan_s,
+                     "tables": plan.fractal if name == "fractal" else plan.p3c_tables}
+        del plan, eng
+    dev = cf.DevicePlan(N3, ops_s, 0.0, None, cf.PATH_AUTO)
+    Lv = tuple(int(x) for x in dev.L)
+    f0 = u.view(n, n, n)[::Lv[2], ::Lv[1], ::Lv[0]].contiguous().view(-1)
+    out["device_pipeline_subgrid_forward_us"] = timed(lambda: dev.forward_dev(f0.data_ptr(), F.data_ptr(), s))
+    out["subgrid"] = [int(x) for x in dev.M]
+    return out
+
+
 def ensemble_iterations(cf, n, seed, iters):
     """The batched-instances workload built as one ensemble (contexts of one
     grid and group each, planned one after another on this thread) and
@@ -684,6 +728,8 @@ def main():
 
     # ---- the Pmmm 64^3 configuration (one field) ----
     pmmm = config_pmmm_64(cf, torch)
+    # ---- the P3c forward families on Fm-3m 128^3 ----
+    fams = families_fm3m(cf, torch)
 
     # ---- SCFT: Im-3m 128^3 plain cold start, Ia-3d 128^3 mixed vs FP64 ----
     scft = None
@@ -774,6 +820,7 @@ def main():
         "cpu_baseline": cpu,
         "many_fields_per_launch": wide,
         "config_pmmm_64": pmmm,
+        "p3c_families_fm3m_128": fams,
         "scft": scft,
     }
     print(json.dumps(line), flush=True)

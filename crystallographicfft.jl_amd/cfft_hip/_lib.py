@@ -52,7 +52,7 @@ EXPORTED = [
     "cfft_scft_copy_state", "cfft_scft_ensemble_create", "cfft_scft_ensemble_iterate",
     "cfft_scft_ensemble_solve",
     "cfft_scft_ensemble_destroy", "cfft_p3c_create", "cfft_p3c_forward", "cfft_p3c_forward_dev",
-    "cfft_p3c_destroy",
+    "cfft_p3c_destroy", "cfft_p3c_create_strided",
     "cfft_mlp_create", "cfft_mlp_info",
     "cfft_mlp_get_params", "cfft_mlp_set_params", "cfft_mlp_forward", "cfft_mlp_gradient",
     "cfft_mlp_train", "cfft_mlp_destroy",
@@ -170,6 +170,8 @@ def _declare(lib):
         "cfft_scft_ensemble_destroy": (C.c_int, [P]),
         "cfft_p3c_create": (C.c_int, [I32, I32, PI32, I32, PI64, C.POINTER(P), C.POINTER(P), C.POINTER(P),
                                       C.POINTER(P)]),
+        "cfft_p3c_create_strided": (C.c_int, [I32, I32, I32, PI32, I32, PI64, C.POINTER(P), C.POINTER(P),
+                                              C.POINTER(P), C.POINTER(P)]),
         "cfft_p3c_forward": (C.c_int, [P, PD, PD]),
         "cfft_p3c_forward_dev": (C.c_int, [P, P, P, P]),
         "cfft_p3c_destroy": (C.c_int, [P]),

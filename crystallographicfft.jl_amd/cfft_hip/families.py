@@ -263,12 +263,35 @@ def execute_g0asu_ikrfft(plan: G0ASUBackwardPlan, spec: SpectralIndexing, F_spec
 
 
 class FractalCFFTPlan(_ForwardFamilyPlan):
-    """src/fractal_krfft.jl:670-676 (also the v2 and v3 plans)."""
+    """src/fractal_krfft.jl:670-676 (also the v2 and v3 plans): on a cubic
+    grid the decimation tree of `cfft_hip.p3c.fractal_tables` — leaves at
+    stride 2^depth, one per orbit of the offsets under the group, split by
+    parity while that still reduces the leaves, and only the values the
+    spectral ASU needs formed at each level (the v2 plan's needed sets,
+    src/fractal_krfft.jl:1172-1208) — run on the device (cfft_p3c_*);
+    `fractal` then summarises the tree.  With no level that reduces, or a
+    leaf length the device does not transform, the device pipeline."""
+
+    def __init__(self, spec, ops_shifted, depth=None):
+        super().__init__(spec, ops_shifted)
+        from . import p3c
+        self.p3c = None
+        self.fractal = None
+        n = self.grid_N[0]
+        if self.grid_N != (n, n, n):
+            return
+        if depth is None:
+            depth = p3c.fractal_depth(ops_shifted, n, lens=_P3C_LENS)
+        if depth < 1:
+            return
+        stride, offs, stages, info = p3c.fractal_tables(spec, ops_shifted, depth)
+        self.p3c = p3c.P3cEngine(n, None, stages, stride=stride, offsets=offs)
+        self.fractal = dict(info, stride=stride, nnz=self.p3c.nnz)
 
 
-def plan_fractal_krfft(spec: SpectralIndexing, ops_shifted) -> FractalCFFTPlan:
-    """src/fractal_krfft.jl:678-722."""
-    return FractalCFFTPlan(spec, ops_shifted)
+def plan_fractal_krfft(spec: SpectralIndexing, ops_shifted, depth=None) -> FractalCFFTPlan:
+    """src/fractal_krfft.jl:678-722; `depth` overrides the tree depth."""
+    return FractalCFFTPlan(spec, ops_shifted, depth)
 
 
 def execute_fractal_krfft(plan: FractalCFFTPlan, u):
@@ -298,6 +321,15 @@ class StagedPm3mPlan:
         if tuple(self.dev.L) != (2, 2, 2):
             raise L.CFFTError(L.CFFT_ERR_UNSUPPORTED, "Pm-3m subgrid stride is not 2")
         self.a8_fft = np.zeros(self.N_half, dtype=np.complex128)
+        # F000 from the P3c relation: 4 sub-sub-grid FFTs of (N/4)^3 and a
+        # sparse stage of 8 entries per M-grid point, when N/4 is a length
+        # the device transforms; otherwise the device pipeline's fold
+        self.p3c = None
+        n = self.N_full[0]
+        if n % 4 == 0 and n // 4 in _P3C_LENS and p3c_applies(ops_s, n):
+            from . import p3c
+            cls, stages = p3c.g0_table(ops_s, n)
+            self.p3c = p3c.P3cEngine(n, cls, stages)
 
 
 def plan_staged_pm3m(N_full, use_yx: bool = False) -> StagedPm3mPlan:
@@ -312,11 +344,15 @@ def plan_staged_pm3m(N_full, use_yx: bool = False) -> StagedPm3mPlan:
 
 def execute_staged(plan: StagedPm3mPlan, u):
     """src/staged_krfft.jl:562-567 -> F000, the FFT of u(2x) on the (N/2)^3
-    grid: the 8-channel fold + channel FFTs + assembly of the device pipeline
-    (exact for any input on a primitive lattice)."""
+    grid: through the P3c engine (4 sub-sub-grid FFTs and one sparse stage)
+    when N/4 is a supported length, else the 8-channel fold + channel FFTs +
+    assembly of the device pipeline."""
     u = np.asarray(u, dtype=np.float64)
     if u.shape != plan.N_full:
         raise L.CFFTError(L.CFFT_ERR_SIZE, f"field shape {u.shape} does not match N = {plan.N_full}")
+    if plan.p3c is not None:
+        plan.a8_fft[...] = plan.p3c.forward(u).reshape(plan.N_half, order="F")
+        return plan.a8_fft
     f0 = plan.dev.full_to_sub(u)
     plan.a8_fft[...] = plan.dev.forward_g0(f0)
     return plan.a8_fft

@@ -1,8 +1,11 @@
-"""Tables of the P3c KRFFT families and their device engine (cfft_p3c_*).
+"""Tables of the sparse-stage KRFFT families (P3c and fractal) and their
+device engine (cfft_p3c_*).
 
 Reference: src/recursive_blocks.jl — plan_krfft_recursive (:64-144),
 plan_krfft_sparse (:514-629), plan_krfft_selective (:756-888),
-plan_krfft_g0asu (:1034-1244) and their execute functions.
+plan_krfft_g0asu (:1034-1244) and their execute functions;
+src/fractal_krfft.jl — build_recursive_tree (:300-600), plan_fractal_krfft
+(:678-722), execute_fractal_krfft! (:869-901).
 
 For a cubic grid N (M = N/2, M2 = N/4) and shifted ops holding the
 body-diagonal 3-fold C: (x, y, z) -> (z, x, y) with an even translation
@@ -121,6 +124,17 @@ def _lin(q, M):
     return q[..., 0] + M * (q[..., 1] + M * q[..., 2])
 
 
+def g0_table(ops_shifted, n):
+    """(classes, [stage]): G0 at every M-grid point (x fastest), the F000 =
+    FFT of u(2 x) of the staged Pm-3m plan (src/staged_krfft.jl:562-567),
+    from the representatives' sub-sub-grid FFTs."""
+    cls = p3c_classes(ops_shifted, n)
+    M = n // 2
+    g = np.arange(M ** 3)
+    allq = np.stack([g % M, (g // M) % M, g // (M * M)], 1)
+    return cls, [csr(*g0_rows(allq, n, cls), M ** 3)]
+
+
 def family_tables(kind, spec: SpectralIndexing, ops_shifted):
     """(classes, stages): the CSR stages of one family ("sparse",
     "recursive", "selective", "g0asu") and a summary of its table sizes."""
@@ -172,15 +186,112 @@ def family_tables(kind, spec: SpectralIndexing, ops_shifted):
     return cls, [B, A], {"g0_positions": len(S), "g0_reps": len(reps)}
 
 
-class P3cEngine:
-    """Device side of a P3c family plan: the representatives' sub-sub-grid
-    pack, their FFTs and the family's sparse stages (cfft_p3c_*)."""
+def _offset_orbits(ops_shifted, n, S):
+    """Orbits of the stride-S offsets e in [0, S)^3 (x fastest) under the
+    group's action e -> (R e + t) mod S: per offset its representative (the
+    smallest key in the orbit, which is the orbit {g e}) and the (R, d) of an
+    op that maps it there, R e + t = rep + S d, so g_e(k) = g_rep(R k + d)
+    for g_e(k) = u(S k + e)."""
+    E = np.array([[a, b, c] for c in range(S) for b in range(S) for a in range(S)], dtype=np.int32)
+    Rm = np.stack([np.rint(np.asarray(op.Rm)).astype(np.int32) for op in ops_shifted])
+    tv = np.stack([np.mod(np.rint(np.asarray(op.tv)).astype(np.int64), n) % S for op in ops_shifted]).astype(np.int32)
+    rep = np.full(len(E), np.iinfo(np.int32).max, dtype=np.int32)
+    best = np.zeros(len(E), dtype=np.int32)
+    for g in range(len(Rm)):  # one op at a time: memory stays O(S^3)
+        W = (E @ Rm[g].T + tv[g]) % S
+        k = W[:, 0] + S * (W[:, 1] + S * W[:, 2])
+        m = k < rep
+        rep[m], best[m] = k[m], g
+    tfull = np.stack([np.mod(np.rint(np.asarray(op.tv)).astype(np.int64), n) for op in ops_shifted])
+    R = Rm[best].astype(np.int64)
+    V = np.einsum("eij,ej->ei", R, E.astype(np.int64)) + tfull[best]
+    Erep = np.stack([rep % S, (rep // S) % S, rep // (S * S)], 1).astype(np.int64)
+    return E.astype(np.int64), rep.astype(np.int64), R, (V - Erep) // S
 
-    def __init__(self, n, cls, stages):
-        eps, orbit_rep, chain, reps = cls
+
+def fractal_depth(ops_shifted, n, lens=None, max_depth=6):
+    """Depth of the decimation tree: split by parity while it still reduces
+    the leaves (fewer than 8x the orbits one level down, the point where the
+    reference's recursion finds no symmetry left to use), with leaf length
+    n / 2^depth >= 2 and, if given, in `lens`."""
+    d, orbits = 0, 1
+    while d < max_depth and n % (2 << d) == 0 and n // (2 << d) >= 2:
+        S = 2 << d
+        if lens is not None and n // S not in lens:
+            break
+        _, rep, _, _ = _offset_orbits(ops_shifted, n, S)
+        k = int(np.sum(rep == np.arange(len(rep))))
+        if k >= 8 * orbits:
+            break
+        d, orbits = d + 1, k
+    return d
+
+
+def fractal_tables(spec: SpectralIndexing, ops_shifted, depth):
+    """(stride, leaf offsets, stages, summary) of the fractal family for a
+    cubic grid: leaves are the stride-2^depth sub-grids g_e(k) = u(2^depth k
+    + e) of one offset per orbit; level l's values G^l_e(q) (stride 2^l,
+    q mod N/2^l) come from level l+1 by the parity split
+        G^l_e(q) = sum_b exp(-2 pi i q.b / M_l) G^{l+1}_{e + 2^l b}(q mod M_{l+1}),
+    each child through its orbit representative,
+        G_c(p) = exp(2 pi i (R^-T p).d / M) G_rep(R^-T p mod M).
+    Only the values the spectral ASU needs are formed at each level, top
+    down; the stages run bottom up (leaf FFTs -> level depth-1 -> ... ->
+    spectral ASU), 8 entries per row before merging."""
+    n = int(spec.N[0])
+    if depth < 1:
+        raise ValueError("depth >= 1")
+    H = np.array([get_k_vector(spec, i) for i in range(len(spec.points))], dtype=np.int64).reshape(-1, 3)
+    B = np.array([[(b >> 0) & 1, (b >> 1) & 1, (b >> 2) & 1] for b in range(8)], dtype=np.int64)
+    # rows of level 0: (offset 0, h)
+    rows_e = np.zeros((len(H), 3), dtype=np.int64)
+    rows_q = H % n
+    stages, sizes = [], [len(H)]
+    for lev in range(depth):
+        S1, M0, M1 = 2 << lev, n >> lev, n >> (lev + 1)
+        E, rep, Rs, ds = _offset_orbits(ops_shifted, n, S1)
+        reps = np.nonzero(rep == np.arange(len(rep)))[0]
+        slot = np.full(len(rep), -1, dtype=np.int64)
+        slot[reps] = np.arange(len(reps))
+        r_idx, c_key, vals = [], [], []
+        for b in B:
+            c = rows_e + (1 << lev) * b
+            ci = c[:, 0] + S1 * (c[:, 1] + S1 * c[:, 2])
+            p = rows_q % M1
+            Rit = np.rint(np.linalg.inv(Rs[ci]).transpose(0, 2, 1)).astype(np.int64)
+            pp = np.einsum("rij,rj->ri", Rit, p) % M1
+            v = np.exp(-2j * np.pi * (rows_q @ b) / M0) * np.exp(2j * np.pi * np.sum(pp * ds[ci], 1) / M1)
+            r_idx.append(np.arange(len(rows_q)))
+            c_key.append(slot[rep[ci]] * M1 ** 3 + pp[:, 0] + M1 * (pp[:, 1] + M1 * pp[:, 2]))
+            vals.append(v)
+        r_idx, c_key, vals = np.concatenate(r_idx), np.concatenate(c_key), np.concatenate(vals)
+        if lev + 1 < depth:
+            uk, cols = np.unique(c_key, return_inverse=True)
+            sl, q = uk // M1 ** 3, uk % M1 ** 3
+            rows_e = E[reps[sl]]
+            rows_q = np.stack([q % M1, (q // M1) % M1, q // (M1 * M1)], 1)
+            sizes.append(len(uk))
+        else:
+            cols, leaf_offsets = c_key, E[reps]
+        stages.append(csr(r_idx, cols, vals, len(r_idx) // 8))
+    return 1 << depth, leaf_offsets, stages[::-1], {"depth": depth, "leaves": len(leaf_offsets),
+                                                     "level_rows": sizes}
+
+
+class P3cEngine:
+    """Device side of a sparse-stage family plan (cfft_p3c_*): the leaf pack,
+    the leaf FFTs and the family's sparse stages.  P3c plans pass the class
+    structure `cls` (stride-4 sub-sub-grids at 2 e); fractal plans pass
+    `stride` and the raw leaf offsets in [0, stride)."""
+
+    def __init__(self, n, cls, stages, stride=None, offsets=None):
         self.n = int(n)
         self.n_out = int(len(stages[-1][0]) - 1)
-        off = np.ascontiguousarray(eps[reps].astype(np.int32).ravel())
+        if stride is None:
+            eps, orbit_rep, chain, reps = cls
+            off = np.ascontiguousarray(eps[reps].astype(np.int32).ravel())
+        else:
+            off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int32).ravel())
         self._keep = []
         rp = (C.c_void_p * len(stages))()
         cs = (C.c_void_p * len(stages))()
@@ -193,8 +304,12 @@ class P3cEngine:
             self._keep += [rowptr, cols, vals]
             rp[i], cs[i], vs[i] = rowptr.ctypes.data, cols.ctypes.data, vals.ctypes.data
         h = C.c_void_p()
-        L.check(L.lib().cfft_p3c_create(self.n, len(reps), L.i32p(off), len(stages), L.i64p(nr),
-                                        rp, cs, vs, C.byref(h)))
+        if stride is None:
+            L.check(L.lib().cfft_p3c_create(self.n, len(off) // 3, L.i32p(off), len(stages), L.i64p(nr),
+                                            rp, cs, vs, C.byref(h)))
+        else:
+            L.check(L.lib().cfft_p3c_create_strided(self.n, int(stride), len(off) // 3, L.i32p(off), len(stages),
+                                                    L.i64p(nr), rp, cs, vs, C.byref(h)))
         self._keep = []
         self._h = h
         self.nnz = [int(s[0][-1]) for s in stages]
@@ -207,6 +322,11 @@ class P3cEngine:
             except Exception:
                 pass
             self._h = None
+
+    def forward_dev(self, u_ptr, F_ptr, stream=None):
+        """Device-resident forward: u (N^3 FP64, x fastest) -> F (n_spec
+        complex FP64), on `stream`, without synchronising."""
+        L.check(L.lib().cfft_p3c_forward_dev(self._h, C.c_void_p(u_ptr), C.c_void_p(F_ptr), C.c_void_p(stream)))
 
     def forward(self, u, out=None):
         u = np.ascontiguousarray(np.asarray(u, dtype=np.float64).ravel(order="F"))

@@ -1,13 +1,15 @@
-// P3c KRFFT families on the device (the reference's recursive, sparse,
-// selective-G0 and G0-ASU forward plans, src/recursive_blocks.jl:64-1240):
-// for a cubic group whose shifted ops hold the body-diagonal 3-fold C
-// (x, y, z) -> (z, x, y) with an even translation, the stride-2 subgrid's
-// spectrum G0 follows from the FFTs of the stride-4 sub-sub-grids of one
-// offset per orbit of the 8 offsets under C, and the spectral-ASU values
-// from G0 through the 8 representative ops.  The host builds the family's
-// tables as CSR matrices (one stage: FFT outputs -> spectral ASU; two
-// stages: FFT outputs -> G0 at the family's positions -> spectral ASU); the
-// device packs the sub-sub-grids, runs their FFTs and the sparse products.
+// Sparse-stage KRFFT families on the device.  P3c (the reference's
+// recursive, sparse, selective-G0 and G0-ASU forward plans,
+// src/recursive_blocks.jl:64-1240): for a cubic group whose shifted ops hold
+// the body-diagonal 3-fold C (x, y, z) -> (z, x, y) with an even
+// translation, the stride-2 subgrid's spectrum G0 follows from the FFTs of
+// the stride-4 sub-sub-grids of one offset per orbit of the 8 offsets under
+// C, and the spectral-ASU values from G0 through the 8 representative ops.
+// Fractal (src/fractal_krfft.jl:670-901): leaves are the stride-2^d
+// sub-grids of one offset per orbit under the whole group, combined level by
+// level through the parity split.  The host builds each family's tables as
+// CSR matrices (leaf FFT outputs -> ... -> spectral ASU); the device packs
+// the leaves, runs their FFTs and the sparse products.
 #include <hip/hip_runtime.h>
 
 #include <memory>
@@ -19,13 +21,15 @@
 
 using namespace cfft;
 
+constexpr int kMaxStages = 8;
+
 struct cfft_p3c {
-  int N = 0, M2 = 0, n_rep = 0, n_stage = 0;
-  int off[8][3] = {};          // sub-sub-grid offsets e of the representatives (u at 4 k + 2 e)
-  long n_out[2] = {0, 0};      // rows of each stage; the last stage's rows are the spectral points
-  long n_in0 = 0;              // n_rep M2^3
-  DevBuf rowptr[2], cols[2], vals[2];
-  DevBuf u, g, mid, F;         // full grid, FFT outputs, G0 values, spectral values
+  int N = 0, S = 0, Ml = 0, n_rep = 0, n_stage = 0;  // grid, leaf stride, leaf length N / S
+  long n_out[kMaxStages] = {};  // rows of each stage; the last stage's rows are the spectral points
+  long n_in0 = 0;               // n_rep Ml^3
+  DevBuf offs;                  // n_rep x 3 leaf offsets (u at S k + e)
+  DevBuf rowptr[kMaxStages], cols[kMaxStages], vals[kMaxStages];
+  DevBuf u, g, mid[2], F;       // full grid, leaf FFT outputs, intermediate levels, spectral values
   hipStream_t stream = nullptr;
   ~cfft_p3c() {
     if (stream) {
@@ -37,20 +41,16 @@ struct cfft_p3c {
 
 namespace {
 
-struct Offs {
-  int e[8][3];
-};
-
-// g[r][k] = u(4 k + 2 e_r), x fastest, as complex
-__global__ void k_p3c_pack(const double* __restrict__ u, cx<double>* __restrict__ g, int N, int M2, int n_rep,
-                           Offs o) {
-  const long n = (long)M2 * M2 * M2;
+// g[r][k] = u(S k + e_r), x fastest, as complex
+__global__ void k_p3c_pack(const double* __restrict__ u, cx<double>* __restrict__ g, int N, int S, int Ml,
+                           int n_rep, const int* __restrict__ offs) {
+  const long n = (long)Ml * Ml * Ml;
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= n * n_rep) return;
   const int r = (int)(i / n);
   const long k = i - (long)r * n;
-  const int kx = (int)(k % M2), ky = (int)((k / M2) % M2), kz = (int)(k / ((long)M2 * M2));
-  const long x = 4 * kx + 2 * o.e[r][0], y = 4 * ky + 2 * o.e[r][1], z = 4 * kz + 2 * o.e[r][2];
+  const int kx = (int)(k % Ml), ky = (int)((k / Ml) % Ml), kz = (int)(k / ((long)Ml * Ml));
+  const long x = (long)S * kx + offs[3 * r], y = (long)S * ky + offs[3 * r + 1], z = (long)S * kz + offs[3 * r + 2];
   g[i] = {u[x + (long)N * (y + (long)N * z)], 0.0};
 }
 
@@ -82,12 +82,10 @@ void fft3(cx<double>* g, int n_rep, hipStream_t s) {
 
 void run(cfft_p3c* h, const double* u, double* F, hipStream_t s) {
   cx<double>* g = static_cast<cx<double>*>(h->g.p);
-  Offs o{};
-  for (int r = 0; r < 8; ++r)
-    for (int d = 0; d < 3; ++d) o.e[r][d] = h->off[r][d];
-  hipLaunchKernelGGL(k_p3c_pack, dim3(nblk(h->n_in0, 256)), dim3(256), 0, s, u, g, h->N, h->M2, h->n_rep, o);
+  hipLaunchKernelGGL(k_p3c_pack, dim3(nblk(h->n_in0, 256)), dim3(256), 0, s, u, g, h->N, h->S, h->Ml, h->n_rep,
+                     static_cast<const int*>(h->offs.p));
   hip_check(hipGetLastError(), "k_p3c_pack");
-  switch (h->M2) {
+  switch (h->Ml) {
     case 2: fft3<2>(g, h->n_rep, s); break;
     case 3: fft3<3>(g, h->n_rep, s); break;
     case 4: fft3<4>(g, h->n_rep, s); break;
@@ -99,18 +97,73 @@ void run(cfft_p3c* h, const double* u, double* F, hipStream_t s) {
     case 32: fft3<32>(g, h->n_rep, s); break;
     case 48: fft3<48>(g, h->n_rep, s); break;
     case 64: fft3<64>(g, h->n_rep, s); break;
-    default: throw Error{CFFT_ERR_UNSUPPORTED, "sub-sub-grid FFT length not supported"};
+    default: throw Error{CFFT_ERR_UNSUPPORTED, "leaf FFT length not supported"};
   }
   hip_check(hipGetLastError(), "k_fft_ml");
   const cx<double>* x = g;
   for (int st = 0; st < h->n_stage; ++st) {
-    cx<double>* y = st + 1 == h->n_stage ? reinterpret_cast<cx<double>*>(F) : static_cast<cx<double>*>(h->mid.p);
+    cx<double>* y = st + 1 == h->n_stage ? reinterpret_cast<cx<double>*>(F)
+                                         : static_cast<cx<double>*>(h->mid[st & 1].p);
     hipLaunchKernelGGL(k_p3c_spmv, dim3(nblk(8 * h->n_out[st], 256)), dim3(256), 0, s,
                        static_cast<const long*>(h->rowptr[st].p), static_cast<const int*>(h->cols[st].p),
                        static_cast<const cx<double>*>(h->vals[st].p), x, y, h->n_out[st]);
     hip_check(hipGetLastError(), "k_p3c_spmv");
     x = y;
   }
+}
+
+// validated build shared by the P3c and the strided (fractal) creators
+cfft_p3c* create(int N, int S, int n_rep, const int32_t* offsets, int n_stage, const int64_t* n_rows,
+                 const int64_t* const* rowptr, const int32_t* const* cols, const double* const* vals) {
+  require(N > 0 && S >= 1 && N % S == 0, CFFT_ERR_NOT_DIVISIBLE, "grid size not divisible by the leaf stride");
+  const int Ml = N / S;
+  require(Ml >= 2 && Ml <= 64 && fft_supported(Ml), CFFT_ERR_UNSUPPORTED, "leaf FFT length not supported");
+  require(n_rep >= 1 && (long)n_rep <= (long)S * S * S, CFFT_ERR_INVALID, "1 to S^3 leaves");
+  require(n_stage >= 1 && n_stage <= kMaxStages, CFFT_ERR_INVALID, "one to eight sparse stages");
+  for (int i = 0; i < 3 * n_rep; ++i)
+    require(offsets[i] >= 0 && offsets[i] < S, CFFT_ERR_INVALID, "leaf offsets lie in [0, S)");
+  std::unique_ptr<cfft_p3c> h(new cfft_p3c());
+  h->N = N;
+  h->S = S;
+  h->Ml = Ml;
+  h->n_rep = n_rep;
+  h->n_stage = n_stage;
+  h->n_in0 = (long)n_rep * Ml * Ml * Ml;
+  long n_in = h->n_in0, n_mid = 1;
+  for (int st = 0; st < n_stage; ++st) {
+    const long nr = n_rows[st];
+    require(nr >= 1 && rowptr[st] && cols[st] && vals[st], CFFT_ERR_INVALID, "bad stage");
+    require(rowptr[st][0] == 0, CFFT_ERR_INVALID, "row pointers start at 0");
+    for (long i = 0; i < nr; ++i)
+      require(rowptr[st][i + 1] >= rowptr[st][i], CFFT_ERR_INVALID, "row pointers must not decrease");
+    const long nnz = rowptr[st][nr];
+    for (long j = 0; j < nnz; ++j)
+      require(cols[st][j] >= 0 && cols[st][j] < n_in, CFFT_ERR_BOUNDS, "column index out of range");
+    h->n_out[st] = nr;
+    if (st + 1 < n_stage) n_mid = std::max(n_mid, nr);
+    h->rowptr[st].alloc(sizeof(int64_t) * (nr + 1));
+    h->cols[st].alloc(sizeof(int32_t) * std::max<long>(nnz, 1));
+    h->vals[st].alloc(16 * std::max<long>(nnz, 1));
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
+    hip_check(hipMemcpy(h->rowptr[st].p, rowptr[st], sizeof(int64_t) * (nr + 1), hipMemcpyHostToDevice), "upload");
+    if (nnz > 0) {
+      hip_check(hipMemcpy(h->cols[st].p, cols[st], sizeof(int32_t) * nnz, hipMemcpyHostToDevice), "upload");
+      hip_check(hipMemcpy(h->vals[st].p, vals[st], 16 * nnz, hipMemcpyHostToDevice), "upload");
+    }
+    n_in = nr;
+  }
+  h->offs.alloc(sizeof(int32_t) * 3 * n_rep);
+  h->u.alloc(sizeof(double) * (size_t)N * N * N);
+  h->g.alloc(16 * (size_t)h->n_in0);
+  if (n_stage >= 2) h->mid[0].alloc(16 * (size_t)n_mid);
+  if (n_stage >= 3) h->mid[1].alloc(16 * (size_t)n_mid);
+  h->F.alloc(16 * (size_t)h->n_out[n_stage - 1]);
+  {
+    std::lock_guard<std::recursive_mutex> guard(setup_mutex());
+    hip_check(hipMemcpy(h->offs.p, offsets, sizeof(int32_t) * 3 * n_rep, hipMemcpyHostToDevice), "upload");
+    hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+  }
+  return h.release();
 }
 
 }  // namespace
@@ -124,51 +177,24 @@ int cfft_p3c_create(int32_t N, int32_t n_rep, const int32_t* offsets, int32_t n_
     require(out && offsets && n_rows && rowptr && cols && vals, CFFT_ERR_INVALID, "null argument");
     *out = nullptr;
     require(N > 0 && N % 4 == 0, CFFT_ERR_NOT_DIVISIBLE, "M must be divisible by 2 for P3c");
-    require(fft_supported(N / 4), CFFT_ERR_UNSUPPORTED, "sub-sub-grid FFT length not supported");
     require(n_rep >= 1 && n_rep <= 8, CFFT_ERR_INVALID, "1 to 8 sub-sub-grid representatives");
     require(n_stage == 1 || n_stage == 2, CFFT_ERR_INVALID, "one or two sparse stages");
-    std::unique_ptr<cfft_p3c> h(new cfft_p3c());
-    h->N = N;
-    h->M2 = N / 4;
-    h->n_rep = n_rep;
-    h->n_stage = n_stage;
-    for (int r = 0; r < n_rep; ++r)
-      for (int d = 0; d < 3; ++d) {
-        require(offsets[3 * r + d] == 0 || offsets[3 * r + d] == 1, CFFT_ERR_INVALID, "offsets are 0 or 1");
-        h->off[r][d] = offsets[3 * r + d];
-      }
-    h->n_in0 = (long)n_rep * h->M2 * h->M2 * h->M2;
-    long n_in = h->n_in0;
-    for (int st = 0; st < n_stage; ++st) {
-      const long nr = n_rows[st];
-      require(nr >= 1 && rowptr[st] && cols[st] && vals[st], CFFT_ERR_INVALID, "bad stage");
-      require(rowptr[st][0] == 0, CFFT_ERR_INVALID, "row pointers start at 0");
-      for (long i = 0; i < nr; ++i)
-        require(rowptr[st][i + 1] >= rowptr[st][i], CFFT_ERR_INVALID, "row pointers must not decrease");
-      const long nnz = rowptr[st][nr];
-      for (long j = 0; j < nnz; ++j)
-        require(cols[st][j] >= 0 && cols[st][j] < n_in, CFFT_ERR_BOUNDS, "column index out of range");
-      h->n_out[st] = nr;
-      h->rowptr[st].alloc(sizeof(int64_t) * (nr + 1));
-      h->cols[st].alloc(sizeof(int32_t) * std::max<long>(nnz, 1));
-      h->vals[st].alloc(16 * std::max<long>(nnz, 1));
-      std::lock_guard<std::recursive_mutex> guard(setup_mutex());
-      hip_check(hipMemcpy(h->rowptr[st].p, rowptr[st], sizeof(int64_t) * (nr + 1), hipMemcpyHostToDevice), "upload");
-      if (nnz > 0) {
-        hip_check(hipMemcpy(h->cols[st].p, cols[st], sizeof(int32_t) * nnz, hipMemcpyHostToDevice), "upload");
-        hip_check(hipMemcpy(h->vals[st].p, vals[st], 16 * nnz, hipMemcpyHostToDevice), "upload");
-      }
-      n_in = nr;
+    std::vector<int32_t> raw(3 * n_rep);
+    for (int i = 0; i < 3 * n_rep; ++i) {
+      require(offsets[i] == 0 || offsets[i] == 1, CFFT_ERR_INVALID, "offsets are 0 or 1");
+      raw[i] = 2 * offsets[i];
     }
-    h->u.alloc(sizeof(double) * (size_t)N * N * N);
-    h->g.alloc(16 * (size_t)h->n_in0);
-    if (n_stage == 2) h->mid.alloc(16 * (size_t)h->n_out[0]);
-    h->F.alloc(16 * (size_t)h->n_out[n_stage - 1]);
-    {
-      std::lock_guard<std::recursive_mutex> guard(setup_mutex());
-      hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
-    }
-    *out = h.release();
+    *out = create(N, 4, n_rep, raw.data(), n_stage, n_rows, rowptr, cols, vals);
+  });
+}
+
+int cfft_p3c_create_strided(int32_t N, int32_t stride, int32_t n_leaf, const int32_t* offsets, int32_t n_stage,
+                            const int64_t* n_rows, const int64_t* const* rowptr, const int32_t* const* cols,
+                            const double* const* vals, cfft_p3c** out) {
+  return guarded([&] {
+    require(out && offsets && n_rows && rowptr && cols && vals, CFFT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    *out = create(N, stride, n_leaf, offsets, n_stage, n_rows, rowptr, cols, vals);
   });
 }
 

@@ -473,9 +473,20 @@ typedef struct cfft_p3c cfft_p3c;
 int cfft_p3c_create(int32_t N, int32_t n_rep, const int32_t* offsets, int32_t n_stage, const int64_t* n_rows,
                     const int64_t* const* rowptr, const int32_t* const* cols, const double* const* vals,
                     cfft_p3c** out);
+/* Fractal KRFFT (plan_fractal_krfft, src/fractal_krfft.jl:678-722, and its
+ * v2/v3 variants, src/fractal_krfft.jl:1021-1170, src/fractal_krfft_v3.jl:
+ * 205-379): the leaves are the stride-`stride` sub-grids u(stride k + e) of
+ * n_leaf offsets e (n_leaf x 3 entries in [0, stride)), leaf length
+ * N / stride (2 to 64, a supported FFT length); 1 to 8 stages, stage 0
+ * reading the n_leaf (N/stride)^3 leaf FFT outputs, each later stage the
+ * previous stage's rows, the last stage's rows the spectral points.  The
+ * handle is run and destroyed by the cfft_p3c_* calls below. */
+int cfft_p3c_create_strided(int32_t N, int32_t stride, int32_t n_leaf, const int32_t* offsets, int32_t n_stage,
+                            const int64_t* n_rows, const int64_t* const* rowptr, const int32_t* const* cols,
+                            const double* const* vals, cfft_p3c** out);
 /* u: N^3 real full grid (x fastest); F: 2 n_spec doubles.  Host pointers,
  * blocking (execute_*_krfft!, src/recursive_blocks.jl:323-330, 631-670,
- * 890-972, 1246-1323). */
+ * 890-972, 1246-1323; execute_fractal_krfft!, src/fractal_krfft.jl:869-901). */
 int cfft_p3c_forward(cfft_p3c* plan, const double* u, double* F);
 /* Device pointers, on `stream`, without synchronising. */
 int cfft_p3c_forward_dev(cfft_p3c* plan, const double* u, double* F, void* stream);

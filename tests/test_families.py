@@ -83,7 +83,27 @@ def test_forward_families_match_full_fft(sg, N):
         assert out is plan.output_buffer
         assert np.max(np.abs(out - ref)) / scale < 1e-12, planner.__name__
     fp = cf.plan_fractal_krfft(spec, ops_s)
+    assert fp.p3c is not None  # the decimation tree on the device (cfft_p3c_create_strided)
     assert np.max(np.abs(cf.execute_fractal_krfft(fp, u) - ref)) / scale < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sg,n,depths", [(229, 32, (1, 2, 3, 4)), (230, 24, (1, 2, 3)), (139, 32, (1, 2, 3, 4)),
+                                         (2, 16, (1,)), (229, 128, (None,))])
+def test_fractal_depths_on_device(sg, n, depths):
+    """The fractal plan at each depth of its tree (None: the depth it picks,
+    6 for Im-3m 128^3 with 5984 leaves of 2^3) against the full FFT at the
+    spectral ASU, 1e-12 relative."""
+    N = (n, n, n)
+    ops_s, R, t, spec = _spec(sg, N)
+    u = O.make_symmetric(R, t, N, seed=24)
+    ref = np.fft.fftn(u)[tuple(spec.idx[:, d] for d in range(3))]
+    for depth in depths:
+        print(f"fractal sg={sg} N={N} depth={depth}", flush=True)
+        fp = cf.plan_fractal_krfft(spec, ops_s, depth)
+        assert fp.p3c is not None
+        out = cf.execute_fractal_krfft(fp, u)
+        assert np.max(np.abs(out - ref)) / np.max(np.abs(ref)) < 1e-12, fp.fractal
 
 
 @pytest.mark.gpu
@@ -104,12 +124,14 @@ def test_g0asu_backward_round_trip(sg, N):
 @pytest.mark.gpu
 @pytest.mark.parametrize("n", [16, 32])
 def test_staged_pm3m(n):
-    """F000 = FFT of u(2x); the full spectrum at random frequencies and at the
-    spectral ASU from F000 (test/test_staged_pm3m.jl)."""
+    """F000 = FFT of u(2x), through the P3c engine; the full spectrum at
+    random frequencies and at the spectral ASU from F000
+    (test/test_staged_pm3m.jl)."""
     N = (n, n, n)
     ops_s, R, t, spec = _spec(221, N)
     u = O.make_symmetric(R, t, N, seed=23)
     plan = cf.plan_staged_pm3m(N)
+    assert plan.p3c is not None  # F000 from the P3c engine (cfft_p3c_*)
     F000 = cf.execute_staged(plan, u)
     ref000 = np.fft.fftn(u[::2, ::2, ::2])
     assert np.max(np.abs(F000 - ref000)) / np.max(np.abs(ref000)) < 1e-12

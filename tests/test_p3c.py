@@ -81,3 +81,70 @@ def test_non_cubic_groups_refused():
         with pytest.raises(cf.CFFTError) as e:
             p3c.p3c_classes(ops_s, 16)
         assert e.value.code == cf._lib.CFFT_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("n", [16, 24, 32])
+def test_staged_g0_table(n):
+    """The staged Pm-3m table (G0 at every M-grid point) gives the FFT of the
+    stride-2 subgrid, fftn(u[::2, ::2, ::2]), for a Pm-3m-symmetric field."""
+    N = (n, n, n)
+    ops, ops_s, shift, R, t = group_setup(221, N)
+    u = O.make_symmetric(R, t, N, seed=n)
+    cls, (stage,) = p3c.g0_table(ops_s, n)
+    got = _spmv(stage, _rep_ffts(u, cls))
+    ref = np.fft.fftn(u[::2, ::2, ::2]).ravel(order="F")
+    assert np.diff(stage[0]).max() <= 8
+    assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 1e-12
+
+
+@pytest.mark.parametrize("sg,n", [(229, 32), (221, 16), (225, 32), (230, 24), (47, 16), (70, 32),
+                                  (139, 32), (2, 16), (227, 32), (200, 16), (123, 16)])
+def test_fractal_tables_give_full_fft(sg, n):
+    """The fractal decimation tree (cfft_hip.p3c.fractal_tables), at every
+    depth up to the one the plan picks: NumPy FFTs of the leaves, run through
+    the stages, give the full-grid FFT at every spectral-ASU point (the check
+    of test/test_fractal_krfft.jl), for cubic and non-cubic groups."""
+    N = (n, n, n)
+    ops, ops_s, shift, R, t = group_setup(sg, N)
+    u = O.make_symmetric(R, t, N, seed=sg + 2 * n)
+    spec = cf.calc_spectral_asu(ops_s, 3, N)
+    ref = np.fft.fftn(u)[tuple(spec.idx[:, d] for d in range(3))]
+    D = p3c.fractal_depth(ops_s, n)
+    assert D >= 1
+    leaves = []
+    for depth in range(1, D + 1):
+        S, offs, stages, info = p3c.fractal_tables(spec, ops_s, depth)
+        assert S == 2 ** depth and len(stages) == depth and info["leaves"] == len(offs)
+        x = np.concatenate([np.fft.fftn(u[o[0]::S, o[1]::S, o[2]::S]).ravel(order="F") for o in offs])
+        for st in stages:
+            assert st[1].min() >= 0 and st[1].max() < len(x)
+            assert np.diff(st[0]).max() <= 8
+            x = _spmv(st, x)
+        assert np.max(np.abs(x - ref)) / np.max(np.abs(ref)) < 1e-12, depth
+        leaves.append(len(offs) * (n // S) ** 3)
+    # every level the plan takes shrinks the leaves
+    assert all(b < a for a, b in zip([n ** 3] + leaves, leaves))
+
+
+def test_fractal_depth_and_orbits():
+    """Split depth: the magic-shifted groups tested here relate all 8 parity
+    classes at the first split; Pmmm and P-1 stop there (the next split
+    gives 8x the leaves), Im-3m goes on while the leaves shrink; every
+    offset's (R, d) maps it onto its representative."""
+    for sg, n, want in ((47, 16, 1), (2, 16, 1), (229, 32, 4), (230, 24, 3)):
+        N = (n, n, n)
+        _, ops_s = cf.find_optimal_shift(cf.get_ops(sg, 3, N), N)
+        assert p3c.fractal_depth(ops_s, n) == want, sg
+        S = 4
+        E, rep, Rs, ds = p3c._offset_orbits(ops_s, n, S)
+        key = E[:, 0] + S * (E[:, 1] + S * E[:, 2])
+        img = np.einsum("eij,ej->ei", Rs, E)
+        # R e + t = rep + S d for the op chosen per offset (t from that op)
+        for i in range(len(E)):
+            t = [np.mod(np.rint(op.tv).astype(np.int64), n) for op in ops_s
+                 if np.array_equal(np.rint(op.Rm).astype(np.int64), Rs[i])]
+            assert any(np.array_equal(img[i] + tt, E[key == rep[i]][0] + S * ds[i]) for tt in t)
+        assert np.all(rep <= key)
+    # P1: nothing to split
+    N = (16, 16, 16)
+    assert p3c.fractal_depth(cf.get_ops(1, 3, N), 16) == 0
