@@ -624,6 +624,8 @@ def main():
     ap.add_argument("--ensemble-iters", type=int, default=0,
                     help="profiling aid: only build the 64-instance batch as one ensemble and run this many "
                          "iterations in FP64, then in mixed precision, from one host thread")
+    ap.add_argument("--only-families", action="store_true",
+                    help="profiling aid: only time the sparse-stage KRFFT families on Fm-3m 128^3")
     args = ap.parse_args()
     if args.gpus > 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -638,6 +640,9 @@ def main():
     if args.ensemble_iters > 0:
         print(json.dumps({"ensemble_iterations": ensemble_iterations(cf, args.batch, 4, args.ensemble_iters)}),
               flush=True)
+        return
+    if args.only_families:
+        print(json.dumps({"p3c_families_fm3m_128": families_fm3m(cf, torch)}), flush=True)
         return
     # one plain 64 MiB device copy: the reference point that scales the HBM
     # byte counters of a profiler run of this script (traffic per launch)

@@ -41,19 +41,6 @@ struct cfft_p3c {
 
 namespace {
 
-// g[r][k] = u(S k + e_r), x fastest, as complex
-__global__ void k_p3c_pack(const double* __restrict__ u, cx<double>* __restrict__ g, int N, int S, int Ml,
-                           int n_rep, const int* __restrict__ offs) {
-  const long n = (long)Ml * Ml * Ml;
-  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i >= n * n_rep) return;
-  const int r = (int)(i / n);
-  const long k = i - (long)r * n;
-  const int kx = (int)(k % Ml), ky = (int)((k / Ml) % Ml), kz = (int)(k / ((long)Ml * Ml));
-  const long x = (long)S * kx + offs[3 * r], y = (long)S * ky + offs[3 * r + 1], z = (long)S * kz + offs[3 * r + 2];
-  g[i] = {u[x + (long)N * (y + (long)N * z)], 0.0};
-}
-
 // y = A x (complex CSR): eight lanes per row, partial sums reduced across the lane octet
 __global__ void k_p3c_spmv(const long* __restrict__ rowptr, const int* __restrict__ cols,
                            const cx<double>* __restrict__ vals, const cx<double>* __restrict__ x,
@@ -68,38 +55,86 @@ __global__ void k_p3c_spmv(const long* __restrict__ rowptr, const int* __restric
   if (ok && lane == 0) y[row] = s;
 }
 
+// Batched multi-lane FFT of WG-thread workgroups (k_fft_ml's transform
+// addressing): the leaf batches are small (4 leaves of 32^3 at 128^3 are
+// 4096 transforms per axis), so 64-thread workgroups spread one pass over
+// four times as many CUs as k_fft_ml's 256 threads.
+template <typename T, typename TW, int LEN, int DIR, int WG>
+__global__ void __launch_bounds__(WG) k_leaf_fft(cx<T>* __restrict__ data, long n_tr, long inner, long outer,
+                                                 long stride) {
+  constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2, G = WG / R1, GP = G + 1;
+  __shared__ cx<T> xch[R1 > 1 ? R2 * R1 * GP : 1];
+  const int r = threadIdx.x % R1, g = threadIdx.x / R1;
+  const long gid = (long)blockIdx.x * G + g;
+  const bool valid = gid < n_tr;
+  const long base = valid ? (gid % inner) + (gid / inner) * outer : 0;
+  cx<T> a[R2];
+#pragma unroll
+  for (int m = 0; m < R2; ++m) a[m] = valid ? data[base + (long)(r + R1 * m) * stride] : cx<T>{T(0), T(0)};
+  fft_ml<T, TW, LEN, DIR, GP>(a, r, g, xch);
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < R2; ++q) data[base + (long)ml_out_pos<LEN>(r, q) * stride] = a[q];
+  }
+}
+
+// The leaf pack fused with the x pass: row i = (leaf, ky, kz) of the leaves
+// (x fastest) read from the full grid, g_r(k) = u(S k + e_r), transformed and
+// stored as complex rows of g.
+template <int LEN, int WG>
+__global__ void __launch_bounds__(WG) k_leaf_pack_fftx(const double* __restrict__ u, cx<double>* __restrict__ g,
+                                                       long n_tr, int N, int S, const int* __restrict__ offs) {
+  constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2, G = WG / R1, GP = G + 1;
+  __shared__ cx<double> xch[R1 > 1 ? R2 * R1 * GP : 1];
+  const int r = threadIdx.x % R1, gl = threadIdx.x / R1;
+  const long gid = (long)blockIdx.x * G + gl;
+  const bool valid = gid < n_tr;
+  const long row = valid ? gid : 0;
+  const int ky = (int)(row % LEN), kz = (int)((row / LEN) % LEN), leaf = (int)(row / (LEN * LEN));
+  const long y = (long)S * ky + offs[3 * leaf + 1], z = (long)S * kz + offs[3 * leaf + 2];
+  const double* src = u + (long)N * (y + (long)N * z) + offs[3 * leaf];
+  cx<double> a[R2];
+#pragma unroll
+  for (int m = 0; m < R2; ++m) a[m] = {src[(long)S * (r + R1 * m)], 0.0};
+  fft_ml<double, double, LEN, -1, GP>(a, r, gl, xch);
+  if (valid) {
+    cx<double>* dst = g + row * LEN;
+#pragma unroll
+    for (int q = 0; q < R2; ++q) dst[ml_out_pos<LEN>(r, q)] = a[q];
+  }
+}
+
 template <int LEN>
-void fft3(cx<double>* g, int n_rep, hipStream_t s) {
+void fft3(const double* u, cx<double>* g, const cfft_p3c* h, hipStream_t s) {
+  constexpr int WG = 64, G = WG / MLS<LEN>::R1;
   const long M2 = LEN, n = M2 * M2 * M2;
-  const long n_tr = n_rep * M2 * M2;
-  const dim3 grid(nblk(n_tr, MLGeom<LEN>::G)), block(MLGeom<LEN>::WG);
-  // x (stride 1), y (stride M2), z (stride M2^2); transform i starts at (i % inner) + (i / inner) outer
-  hipLaunchKernelGGL((k_fft_ml<double, double, LEN, -1>), grid, block, 0, s, g, n_tr, 1L, M2, 1L);
-  hipLaunchKernelGGL((k_fft_ml<double, double, LEN, -1>), grid, block, 0, s, g, n_tr, M2, M2 * M2, M2);
-  hipLaunchKernelGGL((k_fft_ml<double, double, LEN, -1>), grid, block, 0, s, g, n_tr, M2 * M2, n, M2 * M2);
-  (void)n;
+  const long n_tr = h->n_rep * M2 * M2;
+  const dim3 grid(nblk(n_tr, G)), block(WG);
+  // x (from the full grid), y (stride M2), z (stride M2^2); transform i
+  // starts at (i % inner) + (i / inner) outer
+  hipLaunchKernelGGL((k_leaf_pack_fftx<LEN, WG>), grid, block, 0, s, u, g, n_tr, h->N, h->S,
+                     static_cast<const int*>(h->offs.p));
+  hipLaunchKernelGGL((k_leaf_fft<double, double, LEN, -1, WG>), grid, block, 0, s, g, n_tr, M2, M2 * M2, M2);
+  hipLaunchKernelGGL((k_leaf_fft<double, double, LEN, -1, WG>), grid, block, 0, s, g, n_tr, M2 * M2, n, M2 * M2);
 }
 
 void run(cfft_p3c* h, const double* u, double* F, hipStream_t s) {
   cx<double>* g = static_cast<cx<double>*>(h->g.p);
-  hipLaunchKernelGGL(k_p3c_pack, dim3(nblk(h->n_in0, 256)), dim3(256), 0, s, u, g, h->N, h->S, h->Ml, h->n_rep,
-                     static_cast<const int*>(h->offs.p));
-  hip_check(hipGetLastError(), "k_p3c_pack");
   switch (h->Ml) {
-    case 2: fft3<2>(g, h->n_rep, s); break;
-    case 3: fft3<3>(g, h->n_rep, s); break;
-    case 4: fft3<4>(g, h->n_rep, s); break;
-    case 6: fft3<6>(g, h->n_rep, s); break;
-    case 8: fft3<8>(g, h->n_rep, s); break;
-    case 12: fft3<12>(g, h->n_rep, s); break;
-    case 16: fft3<16>(g, h->n_rep, s); break;
-    case 24: fft3<24>(g, h->n_rep, s); break;
-    case 32: fft3<32>(g, h->n_rep, s); break;
-    case 48: fft3<48>(g, h->n_rep, s); break;
-    case 64: fft3<64>(g, h->n_rep, s); break;
+    case 2: fft3<2>(u, g, h, s); break;
+    case 3: fft3<3>(u, g, h, s); break;
+    case 4: fft3<4>(u, g, h, s); break;
+    case 6: fft3<6>(u, g, h, s); break;
+    case 8: fft3<8>(u, g, h, s); break;
+    case 12: fft3<12>(u, g, h, s); break;
+    case 16: fft3<16>(u, g, h, s); break;
+    case 24: fft3<24>(u, g, h, s); break;
+    case 32: fft3<32>(u, g, h, s); break;
+    case 48: fft3<48>(u, g, h, s); break;
+    case 64: fft3<64>(u, g, h, s); break;
     default: throw Error{CFFT_ERR_UNSUPPORTED, "leaf FFT length not supported"};
   }
-  hip_check(hipGetLastError(), "k_fft_ml");
+  hip_check(hipGetLastError(), "k_leaf_fft");
   const cx<double>* x = g;
   for (int st = 0; st < h->n_stage; ++st) {
     cx<double>* y = st + 1 == h->n_stage ? reinterpret_cast<cx<double>*>(F)
