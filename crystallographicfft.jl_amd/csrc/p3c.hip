@@ -104,17 +104,53 @@ __global__ void __launch_bounds__(WG) k_leaf_pack_fftx(const double* __restrict_
   }
 }
 
+// The leaf pack fused with the x and y transforms of one z-plane (leaf, kz)
+// of a leaf: group g of R1 lanes takes x-row g straight from the full grid,
+// leaves it transformed in LDS (row stride LEN + 1), then column g, whose
+// y transform is stored to the plane in g.  LEN * R1 threads.
+template <int LEN>
+__global__ void __launch_bounds__(LEN * MLS<LEN>::R1) k_leaf_pack_fftxy(const double* __restrict__ u,
+                                                                      cx<double>* __restrict__ g, int N, int S,
+                                                                      const int* __restrict__ offs) {
+  constexpr int R1 = MLS<LEN>::R1, R2 = MLS<LEN>::R2, rs = LEN + 1;
+  __shared__ cx<double> lds[LEN * rs];
+  const int r = threadIdx.x % R1, gi = threadIdx.x / R1;  // gi < LEN
+  const int kz = blockIdx.x % LEN, leaf = blockIdx.x / LEN;
+  const long y = (long)S * gi + offs[3 * leaf + 1], z = (long)S * kz + offs[3 * leaf + 2];
+  const double* src = u + (long)N * (y + (long)N * z) + offs[3 * leaf];
+  cx<double> v[R2];
+#pragma unroll
+  for (int m = 0; m < R2; ++m) v[m] = {src[(long)S * (r + R1 * m)], 0.0};
+  cx<double>* row = lds + gi * rs;
+  fft_ml_lds<double, double, LEN, -1>(v, r, row, 1);
+#pragma unroll
+  for (int q = 0; q < R2; ++q) row[ml_out_pos<LEN>(r, q)] = v[q];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < R2; ++m) v[m] = lds[(r + R1 * m) * rs + gi];
+  fft_ml_lds<double, double, LEN, -1>(v, r, lds + gi, rs);
+  cx<double>* dst = g + (long)blockIdx.x * LEN * LEN + gi;
+#pragma unroll
+  for (int q = 0; q < R2; ++q) dst[(long)ml_out_pos<LEN>(r, q) * LEN] = v[q];
+}
+
 template <int LEN>
 void fft3(const double* u, cx<double>* g, const cfft_p3c* h, hipStream_t s) {
   constexpr int WG = 64, G = WG / MLS<LEN>::R1;
   const long M2 = LEN, n = M2 * M2 * M2;
   const long n_tr = h->n_rep * M2 * M2;
   const dim3 grid(nblk(n_tr, G)), block(WG);
-  // x (from the full grid), y (stride M2), z (stride M2^2); transform i
-  // starts at (i % inner) + (i / inner) outer
-  hipLaunchKernelGGL((k_leaf_pack_fftx<LEN, WG>), grid, block, 0, s, u, g, n_tr, h->N, h->S,
-                     static_cast<const int*>(h->offs.p));
-  hipLaunchKernelGGL((k_leaf_fft<double, double, LEN, -1, WG>), grid, block, 0, s, g, n_tr, M2, M2 * M2, M2);
+  const int* offs = static_cast<const int*>(h->offs.p);
+  if constexpr (LEN >= 16) {
+    // x and y in one pass over the leaves' z-planes, then z (stride M2^2)
+    hipLaunchKernelGGL((k_leaf_pack_fftxy<LEN>), dim3(h->n_rep * LEN), dim3(LEN * MLS<LEN>::R1), 0, s, u, g, h->N,
+                       h->S, offs);
+  } else {
+    // x (from the full grid), y (stride M2), z (stride M2^2); transform i
+    // starts at (i % inner) + (i / inner) outer
+    hipLaunchKernelGGL((k_leaf_pack_fftx<LEN, WG>), grid, block, 0, s, u, g, n_tr, h->N, h->S, offs);
+    hipLaunchKernelGGL((k_leaf_fft<double, double, LEN, -1, WG>), grid, block, 0, s, g, n_tr, M2, M2 * M2, M2);
+  }
   hipLaunchKernelGGL((k_leaf_fft<double, double, LEN, -1, WG>), grid, block, 0, s, g, n_tr, M2 * M2, n, M2 * M2);
 }
 

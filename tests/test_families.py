@@ -89,11 +89,12 @@ def test_forward_families_match_full_fft(sg, N):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("sg,n,depths", [(229, 32, (1, 2, 3, 4)), (230, 24, (1, 2, 3)), (139, 32, (1, 2, 3, 4)),
-                                         (2, 16, (1,)), (229, 128, (None,))])
+                                         (2, 16, (1,)), (229, 128, (None, 1, 2)), (230, 96, (1,))])
 def test_fractal_depths_on_device(sg, n, depths):
     """The fractal plan at each depth of its tree (None: the depth it picks,
     6 for Im-3m 128^3 with 5984 leaves of 2^3) against the full FFT at the
-    spectral ASU, 1e-12 relative."""
+    spectral ASU, 1e-12 relative; the leaf lengths 64, 48, 32 and 16 run the
+    fused pack + x + y transform, the shorter ones the separate passes."""
     N = (n, n, n)
     ops_s, R, t, spec = _spec(sg, N)
     u = O.make_symmetric(R, t, N, seed=24)
