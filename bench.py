@@ -464,11 +464,15 @@ def families_fm3m(cf, torch, n=128, reps=100):
         return (time.perf_counter() - t0) / reps * 1e6
 
     out = {"grid": list(N3), "n_spec": len(spec.points)}
+    # the fractal plan takes the group's own ops and spectral ASU (its field is
+    # symmetric under the half-grid shifted ops)
+    ops_raw = cf.get_ops(225, 3, N3)
+    spec_raw = cf.calc_spectral_asu(ops_raw, 3, N3)
     for name, planner in (("recursive", FAM.plan_krfft_recursive), ("sparse", FAM.plan_krfft_sparse),
                           ("selective", FAM.plan_krfft_selective), ("g0asu", FAM.plan_krfft_g0asu),
                           ("fractal", FAM.plan_fractal_krfft)):
         t0 = time.perf_counter()
-        plan = planner(spec, ops_s)
+        plan = planner(spec_raw, ops_raw) if name == "fractal" else planner(spec, ops_s)
         plan_s = time.perf_counter() - t0
         eng = plan.p3c
         out[name] = {"us_per_forward": timed(lambda: eng.forward_dev(u.data_ptr(), F.data_ptr(), s)),

@@ -31,7 +31,7 @@ from .families import (FractalCFFTPlan, G0ASUBackwardPlan, G0ASUPlan, GeneralG0A
                        execute_fractal_krfft_v3, execute_g0asu_ikrfft, execute_g0asu_krfft,
                        execute_general_g0asu_krfft, execute_recursive_krfft,
                        execute_selective_krfft, execute_sparse_krfft, execute_staged,
-                       execute_staged_full, has_cubic_p3c_symmetry, lookup_full_spectrum,
+                       execute_staged_full, has_cubic_p3c_symmetry, lookup_full_spectrum, shift_ops_half_grid,
                        plan_fractal_krfft, plan_fractal_krfft_v2, plan_fractal_krfft_v3,
                        plan_krfft_g0asu, plan_krfft_g0asu_backward, plan_krfft_g0asu_general,
                        plan_krfft_recursive, plan_krfft_selective, plan_krfft_sparse,

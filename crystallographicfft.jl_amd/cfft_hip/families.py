@@ -262,9 +262,32 @@ def execute_g0asu_ikrfft(plan: G0ASUBackwardPlan, spec: SpectralIndexing, F_spec
     return u_out
 
 
+def shift_ops_half_grid(ops, N):
+    """src/fractal_krfft.jl:265-287: the ops of the grid sampled half a cell
+    off the origin, t' = t + (I - R) b mod N with b = 1/2 on every axis.  The
+    fractal plans take the group's own ops and expect a field symmetric under
+    these (a mirror then maps even-parity sectors onto odd ones).  Signed
+    permutation matrices give integer t'; a fractional one (some hexagonal
+    settings) is refused."""
+    from .symmetry_ops import SymOp
+    Nv = np.asarray(N, dtype=np.int64)
+    out = []
+    for op in ops:
+        R = np.asarray(op.Rm, dtype=np.int64)
+        t2 = 2 * np.asarray(op.tv, dtype=np.int64) + (np.eye(len(Nv), dtype=np.int64) - R).sum(axis=1)
+        if np.any(t2 % 2):
+            raise L.CFFTError(L.CFFT_ERR_UNSUPPORTED,
+                              "the half-grid shift leaves a fractional translation for this group")
+        out.append(SymOp.make(R, np.mod(t2 // 2, Nv)))
+    return out
+
+
 class FractalCFFTPlan(_ForwardFamilyPlan):
-    """src/fractal_krfft.jl:670-676 (also the v2 and v3 plans): on a cubic
-    grid the decimation tree of `cfft_hip.p3c.fractal_tables` — leaves at
+    """src/fractal_krfft.jl:670-676 (also the v2 and v3 plans), planned from
+    the group's own ops, for a field symmetric under `shift_ops_half_grid(ops)`
+    (the reference's b = 1/2 sampling, src/fractal_krfft.jl:300-304, and its
+    tests' `make_symmetric`).  On a cubic grid the decimation tree of
+    `cfft_hip.p3c.fractal_tables` over those shifted ops — leaves at
     stride 2^depth, one per orbit of the offsets under the group, split by
     parity while that still reduces the leaves, and only the values the
     spectral ASU needs formed at each level (the v2 plan's needed sets,
@@ -272,26 +295,66 @@ class FractalCFFTPlan(_ForwardFamilyPlan):
     `fractal` then summarises the tree.  With no level that reduces, or a
     leaf length the device does not transform, the device pipeline."""
 
-    def __init__(self, spec, ops_shifted, depth=None):
-        super().__init__(spec, ops_shifted)
+    def __init__(self, spec, ops, depth=None):
+        ops_half = shift_ops_half_grid(ops, tuple(int(n) for n in spec.N))
+        super().__init__(spec, ops_half)
         from . import p3c
+        self.ops_half = ops_half
+        # the values are returned at the caller's spectral ASU (that of `ops`)
+        self.n_spec = len(spec.points)
+        self.output_buffer = np.zeros(self.n_spec, dtype=np.complex128)
         self.p3c = None
         self.fractal = None
         n = self.grid_N[0]
-        if self.grid_N != (n, n, n):
-            return
-        if depth is None:
-            depth = p3c.fractal_depth(ops_shifted, n, lens=_P3C_LENS)
-        if depth < 1:
-            return
-        stride, offs, stages, info = p3c.fractal_tables(spec, ops_shifted, depth)
-        self.p3c = p3c.P3cEngine(n, None, stages, stride=stride, offsets=offs)
-        self.fractal = dict(info, stride=stride, nnz=self.p3c.nnz)
+        if self.grid_N == (n, n, n):
+            if depth is None:
+                depth = p3c.fractal_depth(ops_half, n, lens=_P3C_LENS)
+            if depth >= 1:
+                stride, offs, stages, info = p3c.fractal_tables(spec, ops_half, depth)
+                self.p3c = p3c.P3cEngine(n, None, stages, stride=stride, offsets=offs)
+                self.fractal = dict(info, stride=stride, nnz=self.p3c.nnz)
+                return
+        self._map_to_device_asu(spec, ops_half)
+
+    def _map_to_device_asu(self, spec, ops_half):
+        """Without the tree the device pipeline runs on the spectral ASU of
+        the shifted ops; each requested h is read there through an op,
+        F(h) = exp(2 pi i (R^-T h).t / N) F(R^-T h mod N), or is 0 when no op
+        reaches that ASU (h extinct for the shifted ops)."""
+        N = np.asarray(self.grid_N, dtype=np.int64)
+        dev_idx = self.dev.tables()[0][:, :len(N)]  # the points the device forward returns, in its order
+        lin = lambda h: np.ravel_multi_index(tuple(np.asarray(h).T), tuple(int(v) for v in N), order="F")
+        pos = {int(k): i for i, k in enumerate(np.atleast_1d(lin(dev_idx)))}
+        H = np.asarray(spec.idx, dtype=np.int64)
+        idx = np.full(len(H), -1, dtype=np.int64)
+        phase = np.zeros(len(H), dtype=np.complex128)
+        for op in ops_half:
+            Rit = np.rint(np.linalg.inv(op.Rm).T).astype(np.int64)
+            g = np.mod(H @ Rit.T, N)
+            for i in np.nonzero(idx < 0)[0]:
+                j = pos.get(int(lin(g[i])))
+                if j is not None:
+                    idx[i] = j
+                    phase[i] = np.exp(2j * np.pi * np.sum(g[i] * op.tv / N))
+        self._map_idx, self._map_phase = idx, phase
+
+    def _forward(self, u):
+        if self.p3c is not None:
+            return super()._forward(u)
+        u = np.asarray(u, dtype=np.float64)
+        if u.shape != self.grid_N:
+            raise L.CFFTError(L.CFFT_ERR_SIZE, f"field shape {u.shape} does not match N = {self.grid_N}")
+        F = self.dev.forward(self.dev.full_to_sub(u))
+        ok = self._map_idx >= 0
+        self.output_buffer[:] = 0
+        self.output_buffer[ok] = self._map_phase[ok] * F[self._map_idx[ok]]
+        return self.output_buffer
 
 
-def plan_fractal_krfft(spec: SpectralIndexing, ops_shifted, depth=None) -> FractalCFFTPlan:
-    """src/fractal_krfft.jl:678-722; `depth` overrides the tree depth."""
-    return FractalCFFTPlan(spec, ops_shifted, depth)
+def plan_fractal_krfft(spec: SpectralIndexing, ops, depth=None) -> FractalCFFTPlan:
+    """src/fractal_krfft.jl:678-722: `ops` are the group's ops (not shifted;
+    the plan applies the half-grid shift); `depth` overrides the tree depth."""
+    return FractalCFFTPlan(spec, ops, depth)
 
 
 def execute_fractal_krfft(plan: FractalCFFTPlan, u):

@@ -332,6 +332,13 @@ class P3cEngine:
         u = np.ascontiguousarray(np.asarray(u, dtype=np.float64).ravel(order="F"))
         if u.size != self.n ** 3:
             raise L.CFFTError(L.CFFT_ERR_SIZE, "field must have N^3 points")
-        F = np.empty(self.n_out, dtype=np.complex128) if out is None else out
+        if out is None:
+            F = np.empty(self.n_out, dtype=np.complex128)
+        else:
+            F = out
+            if not (isinstance(F, np.ndarray) and F.dtype == np.complex128 and F.shape == (self.n_out,)
+                    and F.flags.c_contiguous):
+                raise L.CFFTError(L.CFFT_ERR_SIZE, f"output must be a contiguous complex128 array of "
+                                                   f"{self.n_out} values")
         L.check(L.lib().cfft_p3c_forward(self._h, L.f64p(u), L.f64p(F.view(np.float64))))
         return F

@@ -19,6 +19,15 @@ def _spec(sg, N):
     return ops_s, R, t, cf.calc_spectral_asu(ops_s, 3, N)
 
 
+def _spec_half(sg, N):
+    """The fractal plans' convention (test/test_fractal_krfft.jl): the
+    group's own ops and spectral ASU, a field symmetric under the half-grid
+    shifted ops."""
+    ops = cf.get_ops(sg, 3, N)
+    R, t = cf.ops_arrays(cf.shift_ops_half_grid(ops, N))
+    return ops, R, t, cf.calc_spectral_asu(ops, 3, N)
+
+
 @pytest.mark.parametrize("sg,expect", [(221, True), (229, True), (230, True), (223, True),
                                        (47, False), (70, False), (139, False)])
 def test_has_cubic_p3c_symmetry(sg, expect):
@@ -82,9 +91,12 @@ def test_forward_families_match_full_fft(sg, N):
         out = execute(plan, spec, u)
         assert out is plan.output_buffer
         assert np.max(np.abs(out - ref)) / scale < 1e-12, planner.__name__
-    fp = cf.plan_fractal_krfft(spec, ops_s)
+    ops, Rh, th, spec_h = _spec_half(sg, N)
+    uh = O.make_symmetric(Rh, th, N, seed=21)
+    refh = np.fft.fftn(uh)[tuple(spec_h.idx[:, d] for d in range(3))]
+    fp = cf.plan_fractal_krfft(spec_h, ops)
     assert fp.p3c is not None  # the decimation tree on the device (cfft_p3c_create_strided)
-    assert np.max(np.abs(cf.execute_fractal_krfft(fp, u) - ref)) / scale < 1e-12
+    assert np.max(np.abs(cf.execute_fractal_krfft(fp, uh) - refh)) / np.max(np.abs(refh)) < 1e-12
 
 
 @pytest.mark.gpu
@@ -96,12 +108,12 @@ def test_fractal_depths_on_device(sg, n, depths):
     spectral ASU, 1e-12 relative; the leaf lengths 64, 48, 32 and 16 run the
     fused pack + x + y transform, the shorter ones the separate passes."""
     N = (n, n, n)
-    ops_s, R, t, spec = _spec(sg, N)
+    ops, R, t, spec = _spec_half(sg, N)
     u = O.make_symmetric(R, t, N, seed=24)
     ref = np.fft.fftn(u)[tuple(spec.idx[:, d] for d in range(3))]
     for depth in depths:
         print(f"fractal sg={sg} N={N} depth={depth}", flush=True)
-        fp = cf.plan_fractal_krfft(spec, ops_s, depth)
+        fp = cf.plan_fractal_krfft(spec, ops, depth)
         assert fp.p3c is not None
         out = cf.execute_fractal_krfft(fp, u)
         assert np.max(np.abs(out - ref)) / np.max(np.abs(ref)) < 1e-12, fp.fractal
@@ -148,6 +160,24 @@ def test_staged_pm3m(n):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("sg,N", [(47, (16, 16, 32)), (139, (16, 16, 32)), (70, (16, 32, 32))])
+def test_fractal_fallback_to_device_pipeline(sg, N):
+    """Without a tree (a non-cubic grid; the leaf transforms are cubic) the
+    fractal plan runs the device pipeline on the shifted ops' spectral ASU
+    and reads the caller's points through the ops: the full FFT at the
+    unshifted ops' spectral ASU, 1e-12."""
+    print(f"fractal fallback sg={sg} N={N}", flush=True)
+    ops, R, t, spec = _spec_half(sg, N)
+    u = O.make_symmetric(R, t, N, seed=25)
+    ref = np.fft.fftn(u)[tuple(spec.idx[:, d] for d in range(3))]
+    fp = cf.plan_fractal_krfft(spec, ops)
+    assert fp.p3c is None
+    out = cf.execute_fractal_krfft(fp, u)
+    assert out.shape == (len(spec.points),)
+    assert np.max(np.abs(out - ref)) / np.max(np.abs(ref)) < 1e-12
+
+
+@pytest.mark.gpu
 def test_engine_device_calls():
     """cfft_p3c_forward_dev on device buffers, called repeatedly: new field
     values in the same buffers, a second output buffer and torch work between
@@ -155,9 +185,11 @@ def test_engine_device_calls():
     fractal tree, Im-3m 32^3)."""
     import torch
     N = (32, 32, 32)
-    ops_s, R, t, spec = _spec(229, N)
     s = torch.cuda.current_stream().cuda_stream
-    for plan in (cf.plan_krfft_sparse(spec, ops_s), cf.plan_fractal_krfft(spec, ops_s)):
+    ops_s, R_s, t_s, spec_s = _spec(229, N)
+    ops, R_h, t_h, spec_h = _spec_half(229, N)
+    for plan, R, t, spec in ((cf.plan_krfft_sparse(spec_s, ops_s), R_s, t_s, spec_s),
+                             (cf.plan_fractal_krfft(spec_h, ops), R_h, t_h, spec_h)):
         u = torch.empty(32 ** 3, dtype=torch.float64, device="cuda")
         Fs = [torch.empty(2 * len(spec.points), dtype=torch.float64, device="cuda") for _ in range(2)]
         for rep in range(4):

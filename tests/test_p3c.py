@@ -148,3 +148,42 @@ def test_fractal_depth_and_orbits():
     # P1: nothing to split
     N = (16, 16, 16)
     assert p3c.fractal_depth(cf.get_ops(1, 3, N), 16) == 0
+
+
+def test_half_grid_shift():
+    """shift_ops_half_grid (src/fractal_krfft.jl:265-287): t' = t + (I - R)/2
+    summed over each row; a mirror x -> -x gains t = 1, a 3-fold permutation
+    nothing, the identity nothing; the shifted ops of Im-3m form a group (the
+    original conjugated by the half-cell translation); the fractal tables over
+    them give the full FFT of a field symmetric under them with the spectral
+    ASU of the unshifted ops (the reference's fractal test convention)."""
+    N = (16, 16, 16)
+    ops = cf.get_ops(221, 3, N)
+    sh = cf.shift_ops_half_grid(ops, N)
+    for op, o2 in zip(ops, sh):
+        R = op.Rm
+        assert np.array_equal(o2.Rm, R)
+        want = np.mod(op.tv + ((np.eye(3, dtype=np.int64) - R).sum(axis=1)) // 2, 16)
+        assert np.array_equal(o2.tv, want)
+    mirror = [o2 for op, o2 in zip(ops, sh) if np.array_equal(op.Rm, np.diag([-1, 1, 1])) and not op.tv.any()]
+    assert mirror and np.array_equal(mirror[0].tv, [1, 0, 0])
+    for sg, n in ((229, 16), (47, 16), (230, 24)):
+        N = (n, n, n)
+        ops = cf.get_ops(sg, 3, N)
+        sh = cf.shift_ops_half_grid(ops, N)
+        keys = {(tuple(o.Rm.ravel()), tuple(np.mod(o.tv, n))) for o in sh}
+        for a in sh[:8]:
+            for b in sh[:8]:
+                R = a.Rm @ b.Rm
+                t = np.mod(a.Rm @ b.tv + a.tv, n)
+                assert (tuple(R.ravel()), tuple(t)) in keys
+        R, t = cf.ops_arrays(sh)
+        u = O.make_symmetric(R, t, N, seed=n)
+        spec = cf.calc_spectral_asu(ops, 3, N)
+        ref = np.fft.fftn(u)[tuple(spec.idx[:, d] for d in range(3))]
+        D = p3c.fractal_depth(sh, n)
+        S, offs, stages, info = p3c.fractal_tables(spec, sh, D)
+        x = np.concatenate([np.fft.fftn(u[o[0]::S, o[1]::S, o[2]::S]).ravel(order="F") for o in offs])
+        for st in stages:
+            x = _spmv(st, x)
+        assert np.max(np.abs(x - ref)) / np.max(np.abs(ref)) < 1e-12, sg
