@@ -202,3 +202,51 @@ def test_engine_device_calls():
             got = F.cpu().numpy().view(np.complex128)
             ref = np.fft.fftn(uh)[tuple(spec.idx[:, d] for d in range(3))]
             assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 1e-12, rep
+
+
+@pytest.mark.gpu
+def test_engine_refuses_bad_tables():
+    """cfft_p3c_create_strided refuses a stride that does not divide N, a leaf
+    length the device does not transform, offsets outside [0, stride), more
+    than eight stages and a column past the leaf outputs; the host forward
+    refuses an output buffer of the wrong size."""
+    import ctypes as C
+    from cfft_hip import p3c as P
+    lib = cf._lib.lib()
+
+    def create(N, S, offs, stages):
+        rp = (C.c_void_p * len(stages))()
+        cs = (C.c_void_p * len(stages))()
+        vs = (C.c_void_p * len(stages))()
+        keep = []
+        for i, (rowptr, cols, vals) in enumerate(stages):
+            rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+            cols = np.ascontiguousarray(cols, dtype=np.int32)
+            vals = np.ascontiguousarray(np.asarray(vals, dtype=np.complex128).view(np.float64))
+            keep += [rowptr, cols, vals]
+            rp[i], cs[i], vs[i] = rowptr.ctypes.data, cols.ctypes.data, vals.ctypes.data
+        off = np.ascontiguousarray(np.asarray(offs, dtype=np.int32).ravel())
+        nr = np.array([len(s[0]) - 1 for s in stages], dtype=np.int64)
+        h = C.c_void_p()
+        rc = lib.cfft_p3c_create_strided(N, S, len(off) // 3, cf._lib.i32p(off), len(stages), cf._lib.i64p(nr),
+                                         rp, cs, vs, C.byref(h))
+        if rc == 0:
+            lib.cfft_p3c_destroy(h)
+        return rc
+
+    one = (np.array([0, 1]), np.array([0]), np.array([1.0 + 0j]))
+    assert create(16, 2, [[0, 0, 0]], [one]) == 0
+    assert create(16, 3, [[0, 0, 0]], [one]) == cf._lib.CFFT_ERR_NOT_DIVISIBLE
+    assert create(20, 2, [[0, 0, 0]], [one]) == cf._lib.CFFT_ERR_UNSUPPORTED  # leaf length 10
+    assert create(16, 2, [[0, 2, 0]], [one]) == cf._lib.CFFT_ERR_INVALID
+    assert create(16, 2, [[0, 0, 0]], [one] * 9) == cf._lib.CFFT_ERR_INVALID
+    bad = (np.array([0, 1]), np.array([8 ** 3]), np.array([1.0 + 0j]))  # one leaf of 8^3 outputs
+    assert create(16, 2, [[0, 0, 0]], [bad]) == cf._lib.CFFT_ERR_BOUNDS
+    N = (16, 16, 16)
+    ops_s, R, t, spec = _spec(229, N)
+    plan = cf.plan_krfft_sparse(spec, ops_s)
+    u = O.make_symmetric(R, t, N, seed=26)
+    with pytest.raises(cf.CFFTError) as e:
+        plan.p3c.forward(u, np.zeros(plan.p3c.n_out + 1, dtype=np.complex128))
+    assert e.value.code == cf._lib.CFFT_ERR_SIZE
+    assert P.P3cEngine is type(plan.p3c)
