@@ -139,6 +139,7 @@ struct EnsClass {
   unsigned cap[kListsPerClass] = {0, 0, 0};  // entries each list can hold (its launches' grid)
   hipStream_t st = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipEvent_t done = nullptr;  // end of the class's last sweep (its reads of the tables)
 };
 
 struct EnsSweep {
@@ -160,6 +161,7 @@ struct EnsSweep {
     if (staged) (void)hipEventSynchronize(staged);
     for (EnsClass& c : cls) {
       if (c.exec) (void)hipGraphExecDestroy(c.exec);
+      if (c.done) (void)hipEventDestroy(c.done);
       if (c.st) (void)hipStreamDestroy(c.st);
     }
     if (cap) (void)hipStreamDestroy(cap);
@@ -234,7 +236,10 @@ EnsSweep* ens_sweep_create(const std::vector<EnsPart>& parts, int ns) {
             "hipHostMalloc");
   hip_check(hipEventCreateWithFlags(&e->staged, hipEventDisableTiming), "hipEventCreate");
   hip_check(hipStreamCreateWithFlags(&e->up, hipStreamNonBlocking), "hipStreamCreate");
-  for (EnsClass& C : e->cls) hip_check(hipStreamCreateWithFlags(&C.st, hipStreamNonBlocking), "hipStreamCreate");
+  for (EnsClass& C : e->cls) {
+    hip_check(hipStreamCreateWithFlags(&C.st, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&C.done, hipEventDisableTiming), "hipEventCreate");
+  }
   return e.release();
 }
 
@@ -278,6 +283,9 @@ void ens_sweep_run(EnsSweep* e, const std::vector<EnsPart>& parts, const std::ve
     }
   }
   const size_t nc = kListsPerClass * e->cls.size();
+  // the device tables are rewritten only after every class's previous sweep
+  // has read them
+  for (EnsClass& C : e->cls) hip_check(hipStreamWaitEvent(e->up, C.done, 0), "hipStreamWaitEvent");
   hip_check(hipMemcpyAsync(e->slots.p, e->slots_h, sizeof(EnsSlot) * e->n, hipMemcpyHostToDevice, e->up), "upload");
   hip_check(hipMemcpyAsync(e->counts.p, e->counts_h, sizeof(int) * nc, hipMemcpyHostToDevice, e->up), "upload");
   for (size_t c = 0; c < e->cls.size(); ++c) {
@@ -330,6 +338,7 @@ void ens_sweep_run(EnsSweep* e, const std::vector<EnsPart>& parts, const std::ve
     // the class's slice-0 work is already on its stream; the tables first
     hip_check(hipStreamWaitEvent(C.st, e->staged, 0), "hipStreamWaitEvent");
     hip_check(hipGraphLaunch(C.exec, C.st), "hipGraphLaunch");
+    hip_check(hipEventRecord(C.done, C.st), "hipEventRecord");
   }
 }
 
