@@ -67,6 +67,7 @@ OPT_TILE_ROWS = 4
 OPT_FP32_TWIDDLES = 5
 OPT_FFT_LANES = 6
 OPT_PAIR_THREADS = 7
+OPT_AXIS_SPLIT = 8
 
 STAGES = ["p1_fold_fftx", "fft_x", "p2_fft_yz", "fft_z", "p3_recon", "p4_irecon_ifft_zy", "ifft_x",
           "ifft_y", "ifft_z", "p5_ifftx_unfold", "qmul", "sweep", "p51_unfold_fold", "p2q_plane_pair"]

@@ -65,6 +65,7 @@ struct cfft_plan {
   int arith32 = 0;     // mixed precision on path QS: FP32 twiddle and fold-factor arithmetic (option)
   int lanes = 0;       // path QS: lane split of the plane-pair and tile transforms (0: the default)
   int pair_threads = 0;  // path QS: threads per plane-pair workgroup (0: 256)
+  int axis_split = 0;    // path QS: the per-axis chain (x mixing in the tile kernel, single planes)
   hipStream_t cap_stream = nullptr;  // capture stream of the chains
   struct Cached {
     std::string key;
@@ -881,6 +882,65 @@ void launch_p51(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipS
   launch_check<T>("k_p51");
 }
 
+// The axis-split chain (CFFT_OPT_AXIS_SPLIT): H = 16 or 32, FP64 factor and
+// twiddle products in both precisions.
+#define CFFT_AX_SWITCH(len, MACRO) \
+  switch (len) {                   \
+    MACRO(16)                      \
+    MACRO(32)                      \
+    default: throw Error{CFFT_ERR_UNSUPPORTED, "the axis-split chain needs H = 16 or 32"}; \
+  }
+
+template <typename T>
+void launch_p1x(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipStream_t s) {
+  StageTimer st(p, s, CFFT_STAGE_FOLD);
+  const unsigned g = (unsigned)n_tiles(p, pp);
+#define CFFT_P1X(L) \
+  case L: hipLaunchKernelGGL((k_p1x<T, double, L>), dim3(g), dim3(kNT), 0, s, pp, B, step); break;
+  CFFT_AX_SWITCH(p->H[0], CFFT_P1X)
+#undef CFFT_P1X
+  launch_check<T>("k_p1x");
+}
+
+template <typename T>
+void launch_p2x(cfft_plan* p, const PipeP& pp, const PipeBufs& B, hipStream_t s) {
+  StageTimer st(p, s, CFFT_STAGE_PLANE_Q);
+  const unsigned g = (unsigned)n_planes(p);
+#define CFFT_P2X(L)                                                                                  \
+  case L:                                                                                            \
+    hipLaunchKernelGGL((k_p2x<T, double, L>), dim3(g), dim3(L * LineSplit<L>::R1), 0, s, pp, B); \
+    break;
+  CFFT_AX_SWITCH(p->H[1], CFFT_P2X)
+#undef CFFT_P2X
+  launch_check<T>("k_p2x");
+}
+
+template <typename T>
+void launch_p51x(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipStream_t s) {
+  StageTimer st(p, s, CFFT_STAGE_UNFOLD_FOLD);
+  const unsigned g = (unsigned)n_tiles(p, pp);
+#define CFFT_P51X(L)                                                                                       \
+  case L:                                                                                                  \
+    if (B.f0_out) hipLaunchKernelGGL((k_p51x<T, double, L, true>), dim3(g), dim3(kNT), 0, s, pp, B, step); \
+    else hipLaunchKernelGGL((k_p51x<T, double, L, false>), dim3(g), dim3(kNT), 0, s, pp, B, step);        \
+    break;
+  CFFT_AX_SWITCH(p->H[0], CFFT_P51X)
+#undef CFFT_P51X
+  launch_check<T>("k_p51x");
+}
+
+template <typename T>
+void launch_p5x(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipStream_t s) {
+  StageTimer st(p, s, CFFT_STAGE_UNFOLD);
+  const unsigned g = (unsigned)n_tiles(p, pp);
+#define CFFT_P5X(L) \
+  case L: hipLaunchKernelGGL((k_p5x<T, double, L>), dim3(g), dim3(kNT), 0, s, pp, B, step); break;
+  CFFT_AX_SWITCH(p->H[0], CFFT_P5X)
+#undef CFFT_P5X
+  launch_check<T>("k_p5x");
+}
+#undef CFFT_AX_SWITCH
+
 // Work buffers of one diffusion call: channel spectra, the Q-multiplied
 // channels (paths M2Q, M7Q) and the spectral ASU values (paths M2, M7).
 struct Work {
@@ -925,6 +985,12 @@ void run_diffuse(cfft_plan* p, const T* in, T* out_f0, const PipeP& pp, int step
   const Work w = wk ? *wk : own_work(p);
   if (is_qs(p)) {
     PipeBufs B = pipe_bufs(p, in, out_f0, nullptr, false, w.ch);
+    if (p->axis_split) {
+      launch_p1x<T>(p, pp, B, step, s);
+      launch_p2x<T>(p, pp, B, s);
+      launch_p5x<T>(p, pp, B, step, s);
+      return;
+    }
     launch_p1<T>(p, pp, B, step, s);
     launch_p2q<T>(p, pp, B, s);
     launch_p5<T>(p, pp, B, step, s);
@@ -1114,6 +1180,22 @@ PipeBufs inner_bufs(PipeBufs B, long stride) {
 template <typename T>
 void run_chain(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int nsteps, hipStream_t s) {
   cx<T>* F = static_cast<cx<T>*>(p->d_F.p);
+  if (is_qs(p) && p->axis_split) {
+    // per step: the plane kernel (A_y A_z), then the unfold of the step fused
+    // with the fold and A_x of the next
+    launch_p1x<T>(p, pp, pipe_bufs(p, q, nullptr, nullptr, false), step0, s);
+    for (int i = 0; i < nsteps; ++i) {
+      PipeBufs B = pipe_bufs(p, q + (i + 1) * stride, q + (i + 1) * stride, F, false);
+      launch_p2x<T>(p, pp, B, s);
+      if (i + 1 < nsteps && p->fuse) {
+        launch_p51x<T>(p, pp, inner_bufs(B, stride), step0 + i, s);
+      } else {
+        launch_p5x<T>(p, pp, B, step0 + i, s);
+        if (i + 1 < nsteps) launch_p1x<T>(p, pp, B, step0 + i + 1, s);
+      }
+    }
+    return;
+  }
   if (is_qs(p)) {
     // per step: the plane-pair kernel, then P5 of the step fused with P1 of the next
     launch_p1<T>(p, pp, pipe_bufs(p, q, nullptr, nullptr, false), step0, s);
@@ -1168,8 +1250,8 @@ constexpr size_t kMaxGraphs = 16;
 template <typename T>
 void run_steps_graph(cfft_plan* p, T* q, long stride, const PipeP& pp, int step0, int nsteps, hipStream_t s) {
   std::string key(reinterpret_cast<const char*>(&pp), sizeof(PipeP));
-  const long hdr[8] = {reinterpret_cast<long>(q), stride, step0, nsteps, p->fuse, p->arith32, p->lanes,
-                       p->pair_threads};
+  const long hdr[9] = {reinterpret_cast<long>(q), stride, step0, nsteps, p->fuse, p->arith32, p->lanes,
+                       p->pair_threads, p->axis_split};
   key.append(reinterpret_cast<const char*>(hdr), sizeof(hdr));
   for (size_t i = 0; i < p->graphs.size(); ++i)
     if (p->graphs[i].key == key) {
@@ -1238,13 +1320,17 @@ template <typename T>
 double time_stage(cfft_plan* p, T* q, const ESel& es, int stage, int reps, hipStream_t s) {
   PipeP pp = pipe_params(p, es);
   PipeBufs B = pipe_bufs(p, nullptr, q, p->d_F.p, false);
+  const bool ax = is_qs(p) && p->axis_split;
   auto launch = [&] {
-    if (stage == CFFT_STAGE_UNFOLD_FOLD)
-      launch_p51<T>(p, pp, inner_bufs(B, 0), 0, s);  // as between two steps of a sweep
-    else if (stage == CFFT_STAGE_PLANE_Q)
-      launch_p2q<T>(p, pp, B, s);
-    else
+    if (stage == CFFT_STAGE_UNFOLD_FOLD) {
+      if (ax) launch_p51x<T>(p, pp, inner_bufs(B, 0), 0, s);
+      else launch_p51<T>(p, pp, inner_bufs(B, 0), 0, s);  // as between two steps of a sweep
+    } else if (stage == CFFT_STAGE_PLANE_Q) {
+      if (ax) launch_p2x<T>(p, pp, B, s);
+      else launch_p2q<T>(p, pp, B, s);
+    } else {
       launch_p2<T>(p, pp, B, s);
+    }
   };
   launch();
   hipEvent_t a = nullptr, b = nullptr;
@@ -1305,6 +1391,7 @@ EnsPlanInfo ens_plan_info(cfft_plan* p) {
   i.batch = p->batch;
   i.sep = p->qs_sep ? 1 : 0;
   i.lanes = p->lanes;
+  i.axis_split = p->axis_split;
   i.pair_items = (long)p->qs_items * p->batch;
   i.tile_items = n_tiles(p, pipe_params(p, ESel{}));
   i.M3 = p->M3;
@@ -1827,12 +1914,18 @@ int cfft_plan_set_option(cfft_plan* p, int32_t key, int64_t value) {
                 CFFT_ERR_INVALID, "tile rows must be 0 or 1, 2, 4, 8 dividing H1");
         p->tile_rows = (int)value;
         break;
-      case CFFT_OPT_FP32_TWIDDLES: p->arith32 = value != 0; break;
+      case CFFT_OPT_FP32_TWIDDLES:
+        require(value == 0 || !p->axis_split, CFFT_ERR_UNSUPPORTED,
+                "the axis-split chain forms its twiddle products in FP64");
+        p->arith32 = value != 0;
+        break;
       case CFFT_OPT_FFT_LANES: {
         // 0 or the default split: the default kernels; 2 on H = 16, 32 (path QS)
         const int def = (p->H[0] % 16 == 0) ? 4 : ((p->H[0] % 4 == 0) ? 2 : 1);
         require(value == 0 || value == def || (value == 2 && is_qs(p) && (p->H[0] == 16 || p->H[0] == 32)),
                 CFFT_ERR_UNSUPPORTED, "lane split not available for this plan");
+        require(value == 0 || value == def || !p->axis_split, CFFT_ERR_UNSUPPORTED,
+                "the axis-split chain has its own line transforms");
         p->lanes = (value == 0 || value == def) ? 0 : (int)value;
         break;
       }
@@ -1843,6 +1936,14 @@ int cfft_plan_set_option(cfft_plan* p, int32_t key, int64_t value) {
         require(value != 512 || !fp32_arith(p), CFFT_ERR_UNSUPPORTED,
                 "the FP32-twiddle plane-pair kernel runs 256 threads");
         p->pair_threads = value == 512 ? 512 : 0;
+        break;
+      case CFFT_OPT_AXIS_SPLIT:
+        require(value == 0 || (value == 1 && is_qs(p) && p->qs_sep && cubic(p) &&
+                               (p->H[0] == 16 || p->H[0] == 32) && !p->arith32 && !p->lanes),
+                CFFT_ERR_UNSUPPORTED,
+                "the axis-split chain needs path QS, a separable multiply, H = 16 or 32, FP64 twiddle "
+                "products and the default lane split");
+        p->axis_split = value != 0;
         break;
       default: throw Error{CFFT_ERR_INVALID, "unknown option " + std::to_string(key)};
     }

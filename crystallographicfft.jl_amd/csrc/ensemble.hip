@@ -183,8 +183,8 @@ EnsSweep* ens_sweep_create(const std::vector<EnsPart>& parts, int ns) {
   for (int k = 0; k < e->n; ++k) {
     const EnsPlanInfo i = ens_plan_info(parts[k].plan);
     require(i.path == CFFT_PATH_QS, CFFT_ERR_UNSUPPORTED, "an ensemble runs plans on path QS");
-    require(!i.arith32 && !i.lanes, CFFT_ERR_UNSUPPORTED,
-            "an ensemble runs the FP64-twiddle kernels with the default lane split");
+    require(!i.arith32 && !i.lanes && !i.axis_split, CFFT_ERR_UNSUPPORTED,
+            "an ensemble runs the plane-pair kernels with FP64 twiddles and the default lane split");
     require(i.cubic && ens_len_ok(i.L), CFFT_ERR_UNSUPPORTED,
             "an ensemble needs cubic channel grids of length 8, 12, 16, 24 or 32");
     require(i.device == e->device, CFFT_ERR_INVALID, "the plans of an ensemble share the current device");

@@ -61,6 +61,7 @@ struct EnsPlanInfo {
   int path, prec, arith32, device, L, cubic, batch;
   int sep;                      // path QS: the separable Q multiply
   int lanes;                    // path QS: a lane split other than the default (0: none)
+  int axis_split;               // path QS: the per-axis chain (not run by ensembles)
   long pair_items, tile_items;  // plane-pair and tile work items of one launch
   long M3;
 };

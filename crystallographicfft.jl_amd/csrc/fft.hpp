@@ -270,4 +270,67 @@ __device__ __forceinline__ int ml_out_pos(int r, int p) {
   return (r + R1 * (p / R1)) + R2 * (p % R1);
 }
 
+// A 32-point transform over eight lanes of four values (lane r < 8, on entry
+// v[m] = x[r + 8 m]): in-lane 4-point DFT, twiddles w32^(r k), exchange
+// through the transform's own LDS slots; then lane r takes the outputs
+// X[k + 4 k2] of k = r / 2 with k2 = h + 2 j, h = r % 2:
+//   X[k + 4 (h + 2 j)] = sum_{s<4} w8^(s h) w4^(s j) (y_s + (-1)^h y_{s+4})
+// with y_s the twiddled value of lane s for k (the parity split of the
+// 8-point DFT across the lanes).  On exit v[j] = X[l32_out_pos(r, j)].
+// The R1 = 8 lanes are consecutive lanes of one wavefront.
+__device__ __forceinline__ int l32_out_pos(int r, int j) { return (r >> 1) + 4 * ((r & 1) + 2 * j); }
+
+template <typename T, typename TW, int DIR>
+__device__ __forceinline__ void fft32_l8_lds(cx<T> (&v)[4], int r, cx<T>* slot, int es) {
+  stockham<T, TW, 4, DIR, 1>(v);
+  lane_twiddles<T, TW, 32, DIR, 8, 4>(v, r);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int k = 0; k < 4; ++k) slot[(k * 8 + r) * es] = v[k];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int k = r >> 1;
+  const bool h = (r & 1) != 0;
+  cx<T> y[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) y[s] = slot[(k * 8 + s) * es];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    v[s] = h ? csub(y[s], y[s + 4]) : cadd(y[s], y[s + 4]);
+    if (s > 0) {
+      const cx<TW> w8 = tw_c<8, TW>(s);
+      const cx<TW> w = {h ? w8.x : TW(1), h ? w8.y : TW(0)};
+      v[s] = twmul<T, TW, DIR>(v[s], w);
+    }
+  }
+  dft_small<4, DIR, T, TW>(v);
+  // the group's reads of the exchange precede the caller's stores
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Line transforms of the axis-split chain: length 32 over eight lanes of four
+// values, other lengths the default multi-lane split.
+template <int LEN>
+struct LineSplit {
+  static constexpr int R1 = LEN == 32 ? 8 : MLS<LEN>::R1;
+  static constexpr int R2 = LEN / R1;
+};
+
+template <int LEN>
+__device__ __forceinline__ int line_out_pos(int r, int j) {
+  if constexpr (LEN == 32) return l32_out_pos(r, j);
+  else return ml_out_pos<LEN>(r, j);
+}
+
+template <typename T, typename TW, int LEN, int DIR>
+__device__ __forceinline__ void fft_line_lds(cx<T> (&v)[LineSplit<LEN>::R2], int r, cx<T>* slot, int es) {
+  if constexpr (LEN == 32) fft32_l8_lds<T, TW, DIR>(v, r, slot, es);
+  else fft_ml_lds<T, TW, LEN, DIR>(v, r, slot, es);
+}
+
 }  // namespace cfft

@@ -944,6 +944,309 @@ __device__ void ph_unfold_fold(const PipeP& p, const PipeBufs& B, long item, int
   }
 }
 
+// ---- axis-split chain (path QS, separable multiply) ------------------------
+// With an orthogonal cell and translations separable in the sign pattern the
+// Q multiply is a product of three 2 x 2 mixings, one per axis: along axis d
+// the value at channel frequency k (M-grid q = 2k + o_d) becomes
+//   F_d[0][q] g(k) + F_d[1 << d][q] g(kbar),  kbar the index of -q mod M,
+// with the axis factors F of the plane-pair kernel (1/8 folded into the x
+// factors).  With the line transforms the step is then
+//   E U (A_z A_y A_x) F E,   A_d = IFFT_d Q_d FFT_d  (F fold, U unfold),
+// the A_d acting on different axes and commuting.  The tile kernel applies A_x
+// to its x rows after the fold; the plane kernel applies A_y A_z to single
+// x-planes (twice the work items of the plane pairs the octet multiply
+// needs).  Between kernels the channels hold real-space values on every axis.
+
+// Index of -q mod M on an axis of L channel indices with parity offset o.
+template <int L>
+__device__ __forceinline__ int axis_partner(int k, int o) {
+  return o ? L - 1 - k : (k == 0 ? 0 : L - k);
+}
+
+// a u + b w formed in FP64 and rounded once to the data type.
+template <typename T>
+__device__ __forceinline__ cx<T> axis_mix(cx<double> a, cx<T> u, cx<double> b, cx<T> w) {
+  const cx<double> ud = {(double)u.x, (double)u.y}, wd = {(double)w.x, (double)w.y};
+  const cx<double> v = cadd(cmul(a, ud), cmul(b, wd));
+  return {(T)v.x, (T)v.y};
+}
+
+// Parity offset of channel c (selects: the offsets are not indexed at run time).
+__device__ __forceinline__ int chan_offset(const PipeP& p, int c) {
+  int o = p.offs[0];
+#pragma unroll
+  for (int cc = 1; cc < 8; ++cc) o = c == cc ? p.offs[cc] : o;
+  return o;
+}
+
+// Tile (b, z, y0 .. y0 + YB) of every channel into LDS A[c][yy][x], row stride RS.
+template <typename T, int L0, int RS>
+__device__ __forceinline__ void tile_to_lds_rs(const PipeP& p, const cx<T>* __restrict__ ch, int b, int z, int y0,
+                                               cx<T>* lds) {
+  constexpr int MAXIT = (64 * L0 + kNT - 1) / kNT;
+  const int YB = p.YB, n = p.n_ch * L0 * YB;
+  const long HH = (long)p.H[2] * p.H[1];
+  cx<T> r[MAXIT];
+#pragma unroll
+  for (int it = 0; it < MAXIT; ++it) {
+    const int i0 = threadIdx.x + it * kNT;
+    const int i = i0 < n ? i0 : n - 1;  // unconditional load, clamped index
+    const int yy = i & (YB - 1), x = (i >> p.ybs) % L0, c = (i >> p.ybs) / L0;
+    r[it] = ch[(((long)b * p.n_ch + c) * L0 + x) * HH + (long)z * p.H[1] + y0 + yy];
+  }
+#pragma unroll
+  for (int it = 0; it < MAXIT; ++it) {
+    const int i = threadIdx.x + it * kNT;
+    if (i < n) {
+      const int yy = i & (YB - 1), x = (i >> p.ybs) % L0, c = (i >> p.ybs) / L0;
+      lds[(c * YB + yy) * RS + x] = r[it];
+    }
+  }
+}
+
+// A_x on the tile rows (c, yy) held in LDS (row stride RS): FFT x, the x
+// mixing, IFFT x, each row stored from its transform lanes to ch.  Groups
+// past the last row take no part (no workgroup barrier inside).
+template <typename T, typename TW, int L0, int RS>
+__device__ __forceinline__ void tile_rows_ax_to_global(const PipeP& p, const PipeBufs& B, cx<T>* lds,
+                                                       cx<T>* __restrict__ ch, int b, int z, int y0) {
+  constexpr int R1 = LineSplit<L0>::R1, R2 = LineSplit<L0>::R2, G = kNT / R1;
+  const int n_rows = p.n_ch * p.YB;
+  const long HH = (long)p.H[2] * p.H[1];
+  const cx<double>* F = static_cast<const cx<double>*>(B.qsep);
+  const int Ms = p.M[0];
+  const int r = threadIdx.x % R1, g = threadIdx.x / R1;
+  for (int t0 = 0; t0 < n_rows; t0 += G) {
+    const int t = t0 + g;
+    if (t >= n_rows) continue;
+    const int c = t >> p.ybs, yy = t & (p.YB - 1);
+    const int o0 = chan_offset(p, c) & 1;
+    cx<double> f0[R2], f1[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) {
+      const int q = 2 * (r + R1 * m) + o0;
+      f0[m] = F[q];       // s = 0
+      f1[m] = F[Ms + q];  // s = 1 (x flipped)
+    }
+    cx<T>* slot = lds + t * RS;
+    cx<T> v[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) v[m] = slot[r + R1 * m];
+    fft_line_lds<T, TW, L0, -1>(v, r, slot, 1);
+#pragma unroll
+    for (int j = 0; j < R2; ++j) slot[line_out_pos<L0>(r, j)] = v[j];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int m = 0; m < R2; ++m) {
+      const int k = r + R1 * m;
+      v[m] = axis_mix(f0[m], slot[k], f1[m], slot[axis_partner<L0>(k, o0)]);
+    }
+    fft_line_lds<T, TW, L0, +1>(v, r, slot, 1);
+    cx<T>* dst = ch + ((long)b * p.n_ch + c) * L0 * HH + (long)z * p.H[1] + y0 + yy;
+#pragma unroll
+    for (int j = 0; j < R2; ++j) dst[(long)line_out_pos<L0>(r, j) * HH] = v[j];
+  }
+}
+
+// First step: fold + E of slice 0 (as P1), then A_x; channels stored in real space.
+template <typename T, typename TW, int L0>
+__device__ void ph_fold_ax(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds) {
+  const int H0 = p.H[0], H1 = p.H[1], H2 = p.H[2], YB = p.YB;
+  int yb, z, b;
+  tile_decode((unsigned)item, (unsigned)(H1 / YB), (unsigned)H2, yb, z, b);
+  const int y0 = yb * YB;
+  constexpr int rs = kRow<T, L0>;
+  const long M3 = (long)p.M[0] * p.M[1] * p.M[2];
+  const T* fb = static_cast<const T*>(B.f0_in) + b * M3;
+  const double* Eb = pipe_E(p, b, step, M3);
+  const cx<double>* twM = static_cast<const cx<double>*>(B.twM);
+  constexpr int NPT = TilePts<L0>::NPT;
+  T v[NPT][8];
+  FoldTw<TW> ft[NPT];
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int pt0 = threadIdx.x + k * kNT;
+    const int pt = pt0 < H0 * YB ? pt0 : H0 * YB - 1;  // unconditional loads
+    const int x = pt % H0, y = y0 + pt / H0;
+    ft[k] = load_fold_tw<TW>(twM, x, y, z, p);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const long idx = corner_index(p, x, y, z, e);
+      v[k][e] = Eb ? (T)((double)fb[idx] * Eb[idx]) : fb[idx];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int pt = threadIdx.x + k * kNT;
+    if (pt >= H0 * YB) continue;
+    const int x = pt % H0, yy = pt / H0;
+    wht8(v[k]);
+    scatter_parities<T, TW>(p, lds, yy * rs + x, YB * rs, ft[k], v[k]);
+  }
+  __syncthreads();
+  tile_rows_ax_to_global<T, TW, L0, rs>(p, B, lds, static_cast<cx<T>*>(B.ch), b, z, y0);
+}
+
+// Between two steps: unfold + E of step `step` (q(step + 1) written to
+// f0_out when STORE), E of step + 1, fold, then A_x.
+template <typename T, typename TW, int L0, bool STORE>
+__device__ void ph_unfold_fold_ax(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds) {
+  const int H0 = p.H[0], H1 = p.H[1], H2 = p.H[2], YB = p.YB;
+  int yb, z, b;
+  tile_decode((unsigned)item, (unsigned)(H1 / YB), (unsigned)H2, yb, z, b);
+  const int y0 = yb * YB;
+  constexpr int rs = kRow<T, L0>;
+  cx<T>* ch = static_cast<cx<T>*>(B.ch);
+  const long M3 = (long)p.M[0] * p.M[1] * p.M[2];
+  T* fb = STORE ? static_cast<T*>(B.f0_out) + b * M3 : nullptr;
+  const double* Eo = pipe_E(p, b, step, M3);
+  const double* En = pipe_E(p, b, step + 1, M3);
+  const cx<double>* twM = static_cast<const cx<double>*>(B.twM);
+  constexpr int NPT = TilePts<L0>::NPT;
+  double eo[NPT][8], en[NPT][8];
+  FoldTw<TW> ft[NPT];
+  const bool same_E = En == Eo;
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int pt0 = threadIdx.x + k * kNT;
+    const int pt = pt0 < H0 * YB ? pt0 : H0 * YB - 1;  // unconditional loads
+    const int x = pt % H0, y = y0 + pt / H0;
+    ft[k] = load_fold_tw<TW>(twM, x, y, z, p);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const long idx = corner_index(p, x, y, z, e);
+      eo[k][e] = Eo ? Eo[idx] : 1.0;
+      en[k][e] = same_E ? eo[k][e] : (En ? En[idx] : 1.0);
+    }
+  }
+  tile_to_lds_rs<T, L0, rs>(p, ch, b, z, y0, lds);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int pt = threadIdx.x + k * kNT;
+    if (pt >= H0 * YB) continue;
+    const int x = pt % H0, yy = pt / H0, y = y0 + yy;
+    T v[8];
+    gather_parities<T, TW>(p, lds, yy * rs + x, YB * rs, ft[k], v);
+    wht8(v);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      T r = (T)((TW)v[e] * p.scale_unfold);
+      if (Eo) r = (T)((double)r * eo[k][e]);
+      if constexpr (STORE) __builtin_nontemporal_store(r, fb + corner_index(p, x, y, z, e));
+      v[e] = En ? (T)((double)r * en[k][e]) : r;
+    }
+    wht8(v);
+    // the entries of this thread's point only
+    scatter_parities<T, TW>(p, lds, yy * rs + x, YB * rs, ft[k], v);
+  }
+  __syncthreads();
+  tile_rows_ax_to_global<T, TW, L0, rs>(p, B, lds, ch, b, z, y0);
+}
+
+// Last step: unfold + E of real-space channels into f0_out.
+template <typename T, typename TW, int L0>
+__device__ void ph_unfold_ax(const PipeP& p, const PipeBufs& B, long item, int step, cx<T>* lds) {
+  const int H0 = p.H[0], H1 = p.H[1], H2 = p.H[2], YB = p.YB;
+  int yb, z, b;
+  tile_decode((unsigned)item, (unsigned)(H1 / YB), (unsigned)H2, yb, z, b);
+  const int y0 = yb * YB;
+  constexpr int rs = kRow<T, L0>;
+  const long M3 = (long)p.M[0] * p.M[1] * p.M[2];
+  T* fb = static_cast<T*>(B.f0_out) + b * M3;
+  const double* Eb = pipe_E(p, b, step, M3);
+  const cx<double>* twM = static_cast<const cx<double>*>(B.twM);
+  constexpr int NPT = TilePts<L0>::NPT;
+  double ev[NPT][8];
+  FoldTw<TW> ft[NPT];
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int pt0 = threadIdx.x + k * kNT;
+    const int pt = pt0 < H0 * YB ? pt0 : H0 * YB - 1;  // unconditional loads
+    const int x = pt % H0, y = y0 + pt / H0;
+    ft[k] = load_fold_tw<TW>(twM, x, y, z, p);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) ev[k][e] = Eb ? Eb[corner_index(p, x, y, z, e)] : 1.0;
+  }
+  tile_to_lds_rs<T, L0, rs>(p, static_cast<const cx<T>*>(B.ch), b, z, y0, lds);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int pt = threadIdx.x + k * kNT;
+    if (pt >= H0 * YB) continue;
+    const int x = pt % H0, yy = pt / H0, y = y0 + yy;
+    T v[8];
+    gather_parities<T, TW>(p, lds, yy * rs + x, YB * rs, ft[k], v);
+    wht8(v);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const T r = (T)((TW)v[e] * p.scale_unfold);
+      fb[corner_index(p, x, y, z, e)] = Eb ? (T)((double)r * ev[k][e]) : r;
+    }
+  }
+}
+
+// A_y A_z on the x-plane `item` = (b, c, x) of the channels (L x L entries,
+// y fastest): rows (z) FFT y into A; columns (y) the y mixing on read, FFT z,
+// the z mixing, IFFT z into B (each column group in its own column); rows
+// IFFT y from B, stored.  L * R1 threads: one line per lane group in each pass.
+template <typename T, typename TW, int L>
+__device__ void ph_plane_ax(const PipeP& p, const PipeBufs& B, long item, cx<T>* A, cx<T>* Bv) {
+  constexpr int R1 = LineSplit<L>::R1, R2 = LineSplit<L>::R2, rs = kRow<T, L>;
+  const int c = (int)((unsigned)(item / L) % (unsigned)p.n_ch);
+  const int off = chan_offset(p, c), o1 = (off >> 1) & 1, o2 = (off >> 2) & 1;
+  cx<T>* plane = static_cast<cx<T>*>(B.ch) + item * (long)(L * L);
+  const cx<double>* F = static_cast<const cx<double>*>(B.qsep);
+  const int Ms = p.M[0];
+  const int r = threadIdx.x % R1, g = threadIdx.x / R1;  // g < L
+  const int gb = axis_partner<L>(g, o1);
+  const cx<double> fy0 = F[(8 + 0) * Ms + 2 * g + o1], fy1 = F[(8 + 2) * Ms + 2 * g + o1];
+  cx<double> fz0[R2], fz1[R2];
+#pragma unroll
+  for (int m = 0; m < R2; ++m) {
+    const int q = 2 * (r + R1 * m) + o2;
+    fz0[m] = F[(16 + 0) * Ms + q];
+    fz1[m] = F[(16 + 4) * Ms + q];
+  }
+  cx<T> v[R2];
+  // FFT y: row z = g, straight from global memory
+#pragma unroll
+  for (int m = 0; m < R2; ++m) v[m] = plane[g * L + r + R1 * m];
+  fft_line_lds<T, TW, L, -1>(v, r, A + g * rs, 1);
+#pragma unroll
+  for (int j = 0; j < R2; ++j) A[g * rs + line_out_pos<L>(r, j)] = v[j];
+  __syncthreads();
+  // column y = g: the y mixing on read, FFT z, the z mixing, IFFT z
+#pragma unroll
+  for (int m = 0; m < R2; ++m) {
+    const int z = r + R1 * m;
+    v[m] = axis_mix(fy0, A[z * rs + g], fy1, A[z * rs + gb]);
+  }
+  fft_line_lds<T, TW, L, -1>(v, r, Bv + g, rs);
+#pragma unroll
+  for (int j = 0; j < R2; ++j) Bv[line_out_pos<L>(r, j) * rs + g] = v[j];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int m = 0; m < R2; ++m) {
+    const int k = r + R1 * m;
+    v[m] = axis_mix(fz0[m], Bv[k * rs + g], fz1[m], Bv[axis_partner<L>(k, o2) * rs + g]);
+  }
+  fft_line_lds<T, TW, L, +1>(v, r, Bv + g, rs);
+#pragma unroll
+  for (int j = 0; j < R2; ++j) Bv[line_out_pos<L>(r, j) * rs + g] = v[j];
+  __syncthreads();
+  // IFFT y: row z = g, stored from its transform lanes
+#pragma unroll
+  for (int m = 0; m < R2; ++m) v[m] = Bv[g * rs + r + R1 * m];
+  fft_line_lds<T, TW, L, +1>(v, r, Bv + g * rs, 1);
+#pragma unroll
+  for (int j = 0; j < R2; ++j) plane[g * L + line_out_pos<L>(r, j)] = v[j];
+}
+
 // ---- kernels: one per phase -------------------------------------------------
 template <typename T, typename TW, int L0>
 __global__ void __launch_bounds__(kNT) k_p1(PipeP p, PipeBufs B, int step) {
@@ -1069,6 +1372,32 @@ __global__ void __launch_bounds__(kNT) k_p51(PipeP p, PipeBufs B, int step) {
   __shared__ cx<T> lds[64 * kRow<T, L0>];
   const long item = p51_item(blockIdx.x, gridDim.x, p.H[1] / p.YB, p.YB * (int)sizeof(cx<T>) < 128);
   ph_unfold_fold<T, TW, L0, STORE, LANES>(p, B, item, step, lds);
+}
+
+// Axis-split chain kernels (path QS with CFFT_OPT_AXIS_SPLIT).
+template <typename T, typename TW, int L0>
+__global__ void __launch_bounds__(kNT) k_p1x(PipeP p, PipeBufs B, int step) {
+  __shared__ cx<T> lds[64 * kRow<T, L0>];
+  ph_fold_ax<T, TW, L0>(p, B, blockIdx.x, step, lds);
+}
+
+template <typename T, typename TW, int L>
+__global__ void __launch_bounds__(L * LineSplit<L>::R1) k_p2x(PipeP p, PipeBufs B) {
+  __shared__ cx<T> A[L * kRow<T, L>];
+  __shared__ cx<T> Bv[L * kRow<T, L>];
+  ph_plane_ax<T, TW, L>(p, B, blockIdx.x, A, Bv);
+}
+
+template <typename T, typename TW, int L0, bool STORE>
+__global__ void __launch_bounds__(kNT) k_p51x(PipeP p, PipeBufs B, int step) {
+  __shared__ cx<T> lds[64 * kRow<T, L0>];
+  ph_unfold_fold_ax<T, TW, L0, STORE>(p, B, blockIdx.x, step, lds);
+}
+
+template <typename T, typename TW, int L0>
+__global__ void __launch_bounds__(kNT) k_p5x(PipeP p, PipeBufs B, int step) {
+  __shared__ cx<T> lds[64 * kRow<T, L0>];
+  ph_unfold_ax<T, TW, L0>(p, B, blockIdx.x, step, lds);
 }
 
 // ---- persistent sweep: nsteps Strang steps in one cooperative launch ---------
