@@ -143,25 +143,25 @@ def profile_stages(sw, steps, torch):
 
 
 STRATEGY_PATH = {"m7_chain": 0, "m7_persistent": 0, "qr_chain": 5, "qs_chain": 6, "qs_chain_eager": 6,
-                 "qs_chain_tuned": 6}
+                 "qs_chain_axis": 6, "qs_chain_tuned": 6}
 # the FETCH_SIZE / WRITE_SIZE record of each pipeline path's chain kernels
 # (eager launches and graph replays run the same kernels)
 TRAFFIC_RECORD = {0: "m7_chain", 5: "qr_chain", 6: "qs_chain"}
 
 
-def profiled_traffic(strategy, stage_name, prec_name):
+def profiled_traffic(strategy, stage_name, prec_name, axis=False):
     """HBM bytes per launch of a chain kernel at this workload (2 fields per
     launch), from the FETCH_SIZE / WRITE_SIZE profiler runs recorded under
     profiles/ for the pipeline path of that launch strategy (each counter
-    scaled by what it reads for a plain 64 MiB copy); None when no such
-    record exists."""
+    scaled by what it reads for a plain 64 MiB copy), the per-axis chain's
+    kernels under their own keys; None when no such record exists."""
     name = TRAFFIC_RECORD.get(STRATEGY_PATH.get(strategy))
     if name is None:
         return None
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_im3m128_{name}.json")
     try:
         with open(path) as fh:
-            rec = json.load(fh)["kernels"].get(f"{stage_name}/{prec_name}")
+            rec = json.load(fh)["kernels"].get(f"{stage_name}/{prec_name}" + ("/axis" if axis else ""))
     except (OSError, ValueError, KeyError):
         return None
     return None if rec is None else float(rec["bytes"])
@@ -271,7 +271,7 @@ def scft_im3m(cf, ops_s, wA, wB, tol, max_iter):
     """Plain cold start of the Im-3m 128^3 melt (f 0.3, chiN 20, Ns 100) on the
     device, FP64: first-iteration Q, F, densities; wall time to `tol`."""
     lat = CELL * np.eye(3)
-    solver = cf.scft.SCFTSolver(N, ops_s, lat, F_A, CHIN, NS, cf.FP64, m_hist=10)
+    solver = cf.scft.SCFTSolver(N, ops_s, lat, F_A, CHIN, NS, cf.FP64, m_hist=10, axis_split=True)
     solver.set_fields(wA, wB)
     first = solver.iterate()
     phi0 = solver.densities()
@@ -671,14 +671,17 @@ def main():
         # the plane-pair chain's schedule (tile rows, graph replay, pair
         # workgroup size, lane split) tuned on this workload (cfft_hip.tune)
         from cfft_hip.tune import tune_schedule
-        tuned, tuned_times = tune_schedule(N, ops_s, lat, NS, prec, 2, cf.PATH_QS, lanes=True)
+        tuned, tuned_times = tune_schedule(N, ops_s, lat, NS, prec, 2, cf.PATH_QS, lanes=True, axis=True)
         for label, path, persist, graph in (("m7_chain", cf.PATH_AUTO, 0, 1), ("m7_persistent", cf.PATH_AUTO, 1, 1),
                                             ("qr_chain", cf.PATH_QR, 0, 1), ("qs_chain", cf.PATH_QS, 0, 1),
-                                            ("qs_chain_eager", cf.PATH_QS, 0, 0), ("qs_chain_tuned", cf.PATH_QS, 0, 1)):
+                                            ("qs_chain_eager", cf.PATH_QS, 0, 0), ("qs_chain_axis", cf.PATH_QS, 0, 1),
+                                            ("qs_chain_tuned", cf.PATH_QS, 0, 1)):
             sw = cf.scft.PropagatorSweep(N, ops_s, lat, F_A, NS, prec, path=path)
             sw.set_fields(wA, wB)
             sw.plan.set_option(cf._lib.OPT_PERSISTENT, persist)
             sw.plan.set_option(cf._lib.OPT_GRAPH, graph)
+            if label == "qs_chain_axis":  # the per-axis chain of path QS
+                sw.plan.set_option(cf._lib.OPT_AXIS_SPLIT, 1)
             if label == "qs_chain_tuned":
                 for k, v in tuned.items():
                     sw.plan.set_option(k, v)
@@ -724,7 +727,8 @@ def main():
                          "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
         res[name] = dict(el=el, med=med, Qq=Qq, Qd=Qd, q=qfields, stages=stages, dom=dom, avg_ms=avg_ms,
                          achieved=achieved, plan=sw.plan, sbytes=nb, chain=chain,
-                         step_bytes=step_bytes(sw.plan, e))
+                         step_bytes=step_bytes(sw.plan, e),
+                         axis=bool(sw.plan.options.get(cf._lib.OPT_AXIS_SPLIT)))
         del sw
 
     # ---- many fields per launch (bandwidth regime of the same kernels) ----
@@ -802,7 +806,7 @@ def main():
                                  "(2 Ns diffusion steps)"},
         "roofline": {"bound": "hbm", "kernel": r64["dom"], "achieved": r64["achieved"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(variants["fp64"]["used"], r64["dom"], "fp64"),
+                     "traffic": profiled_traffic(variants["fp64"]["used"], r64["dom"], "fp64", r64["axis"]),
                      "bytes_per_launch": r64["sbytes"],
                      "avg_launch_ms": r64["avg_ms"], "chain_kernels": r64["chain"]},
         "mixed": {"value": mixed_value, "dtype": "f32 data/FFT/propagator, f64 tables+reductions",

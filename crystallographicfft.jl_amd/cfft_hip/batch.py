@@ -210,9 +210,10 @@ ENSEMBLE_LENS = (8, 12, 16, 24, 32)  # channel transform lengths of the ensemble
 
 def ensemble_ok(plan):
     """Whether a plan can join an ensemble: path QS, a cubic channel grid of a
-    supported length (the ensemble runs the FP64-twiddle kernels)."""
+    supported length, the plane-pair kernels (the ensemble runs the
+    FP64-twiddle plane-pair kernels, not the per-axis chain)."""
     return (plan.path == L.PATH_QS and plan.H[0] == plan.H[1] == plan.H[2]
-            and plan.H[0] in ENSEMBLE_LENS)
+            and plan.H[0] in ENSEMBLE_LENS and not plan.options.get(L.OPT_AXIS_SPLIT))
 
 
 # Mixing of the batch's contexts: Anderson steps from residual 0.1 down

@@ -252,6 +252,7 @@ class DevicePlan:
         self.centering = CenteringType(int(info[8]))
         self.d = int(info[9])
         self.path = int(info[10])
+        self.options = {}  # plan options set through set_option
         self.precision = int(info[11])
         self.batch = int(info[12])
         self.H = tuple(int(v) for v in info[13:16])
@@ -384,6 +385,7 @@ class DevicePlan:
 
     def set_option(self, key, value):
         L.check(L.lib().cfft_plan_set_option(self._h, int(key), int(value)))
+        self.options[int(key)] = int(value)
 
 
 # ---------------------------------------------------------------------------

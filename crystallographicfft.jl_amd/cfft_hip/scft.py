@@ -165,12 +165,14 @@ class SCFTSolver:
     KEYS = ("Q", "Qd", "F", "res", "phi_total", "phiA_mean", "anderson", "hist")
 
     def __init__(self, N, ops_shifted, lattice, f, chiN, ns, precision=L.FP64, path=None,
-                 m_hist=10):
+                 m_hist=10, axis_split=False):
         self.ns = int(ns)
         self.f = float(f)
         self.chiN = float(chiN)
         path = default_path(N, ops_shifted, lattice, precision) if path is None else path
         self.plan = DevicePlan(N, ops_shifted, 1.0 / self.ns, lattice, path, precision, batch=2)
+        if axis_split:  # the per-axis chain of path QS (CFFT_OPT_AXIS_SPLIT)
+            self.plan.set_option(L.OPT_AXIS_SPLIT, 1)
         self.M = self.plan.M
         self.M3 = int(np.prod(self.M))
         h = C.c_void_p()

@@ -9,7 +9,8 @@ schedule computes each field with the same arithmetic), only the
 decomposition of the phase kernels and how the step sequence is launched.
 The lane split R1 x R2 of a length-H transform (in-lane R2-point passes,
 then a cross-lane radix-R1 pass) reorders the butterflies, so results move
-by rounding only; it is searched only when `lanes` is set.  A timing is one
+by rounding only; it is searched only when `lanes` is set.  So does the
+per-axis chain of path QS (CFFT_OPT_AXIS_SPLIT), searched when `axis` is set.  A timing is one
 contour sweep of `steps` Strang steps over `batch` fields with all
 interaction factors 1 (a plain multi-step sweep, the kernels of the stored
 sweeps apart from the q store), after one untimed sweep that captures the
@@ -35,12 +36,16 @@ def schedule_candidates(H1):
     return [{L.OPT_TILE_ROWS: r, L.OPT_GRAPH: g} for r in rows for g in (1, 0)]
 
 
-def extra_dimensions(plan, lanes):
-    """Further schedule dimensions of a plan on path QS: the plane-pair
-    workgroup size (512 threads for H = 16, 24, 32) and, with `lanes`, the
-    lane split R1 = 2 of the transforms (H = 16, 32)."""
+def extra_dimensions(plan, lanes, axis=False):
+    """Further schedule dimensions of a plan on path QS: with `axis`, the
+    per-axis chain (H = 16, 32; plans without the separable multiply refuse
+    it and keep the plane pairs); the plane-pair workgroup size (512 threads
+    for H = 16, 24, 32) and, with `lanes`, the lane split R1 = 2 of the
+    transforms (H = 16, 32)."""
     out = []
     if plan.path == L.PATH_QS:
+        if axis and plan.H[0] in (16, 32):
+            out.append((L.OPT_AXIS_SPLIT, 1))
         if plan.H[1] in (16, 24, 32):
             out.append((L.OPT_PAIR_THREADS, 512))
         if lanes and plan.H[0] in (16, 32):
@@ -50,6 +55,7 @@ def extra_dimensions(plan, lanes):
 
 def _label(opt):
     return (f"rows{opt[L.OPT_TILE_ROWS]}_graph{opt[L.OPT_GRAPH]}"
+            + ("_axis" if opt.get(L.OPT_AXIS_SPLIT) else "")
             + (f"_pt{opt[L.OPT_PAIR_THREADS]}" if opt.get(L.OPT_PAIR_THREADS) else "")
             + (f"_lanes{opt[L.OPT_FFT_LANES]}" if opt.get(L.OPT_FFT_LANES) else ""))
 
@@ -61,12 +67,16 @@ def _key(N, ops_shifted, lattice, ns, precision, batch, path):
             int(precision), int(batch), int(path))
 
 
-def tune_schedule(N, ops_shifted, lattice, ns, precision, batch, path, steps=None, reps=3, lanes=False):
+def tune_schedule(N, ops_shifted, lattice, ns, precision, batch, path, steps=None, reps=3, lanes=False,
+                  axis=False):
     """(best options, {label: microseconds per step}) for a plan of `batch`
     fields with ds = 1/ns; options are {OPT_TILE_ROWS: r, OPT_GRAPH: g,
-    OPT_PAIR_THREADS: t, OPT_FFT_LANES: l} (the last two 0 = default)."""
+    OPT_PAIR_THREADS: t, OPT_FFT_LANES: l} (the last two 0 = default), plus
+    OPT_AXIS_SPLIT with `axis`.  A candidate the plan refuses (the per-axis
+    chain without the separable multiply, or with the R1 = 2 lane split) is
+    skipped."""
     import torch
-    key = _key(N, ops_shifted, lattice, ns, precision, batch, path) + (bool(lanes),)
+    key = _key(N, ops_shifted, lattice, ns, precision, batch, path) + (bool(lanes), bool(axis))
     if key in _CACHE:
         return _CACHE[key]
     steps = int(ns if steps is None else steps)
@@ -79,6 +89,8 @@ def tune_schedule(N, ops_shifted, lattice, ns, precision, batch, path, steps=Non
     times = {}
 
     def timed(opt):
+        # the per-axis chain first: it excludes the R1 = 2 lane split
+        plan.set_option(L.OPT_AXIS_SPLIT, opt.get(L.OPT_AXIS_SPLIT, 0))
         for k, v in opt.items():
             plan.set_option(k, v)
         plan.step_dev(q.data_ptr(), E.data_ptr(), steps, s)  # captures the graph when enabled
@@ -97,13 +109,20 @@ def tune_schedule(N, ops_shifted, lattice, ns, precision, batch, path, steps=Non
     for opt in schedule_candidates(plan.H[1]):
         opt = dict(opt)
         opt.update({L.OPT_PAIR_THREADS: 0, L.OPT_FFT_LANES: 0})
+        if axis:
+            opt[L.OPT_AXIS_SPLIT] = 0
         t = timed(opt)
         if best_t is None or t < best_t:
             best, best_t = opt, t
-    for k, v in extra_dimensions(plan, lanes):
+    for k, v in extra_dimensions(plan, lanes, axis):
         opt = dict(best)
         opt[k] = v
-        t = timed(opt)
+        try:
+            t = timed(opt)
+        except L.CFFTError as e:
+            if e.code != L.CFFT_ERR_UNSUPPORTED:
+                raise
+            continue
         if t < best_t:
             best, best_t = opt, t
     del plan

@@ -1391,7 +1391,9 @@ __global__ void __launch_bounds__(L * LineSplit<L>::R1) k_p2x(PipeP p, PipeBufs 
 template <typename T, typename TW, int L0, bool STORE>
 __global__ void __launch_bounds__(kNT) k_p51x(PipeP p, PipeBufs B, int step) {
   __shared__ cx<T> lds[64 * kRow<T, L0>];
-  ph_unfold_fold_ax<T, TW, L0, STORE>(p, B, blockIdx.x, step, lds);
+  // neighbouring y-blocks of FP32 rows share 128-byte lines: same XCD
+  const long item = p51_item(blockIdx.x, gridDim.x, p.H[1] / p.YB, p.YB * (int)sizeof(cx<T>) < 128);
+  ph_unfold_fold_ax<T, TW, L0, STORE>(p, B, item, step, lds);
 }
 
 template <typename T, typename TW, int L0>

@@ -133,3 +133,20 @@ def test_axis_split_refused():
     with pytest.raises(cf.CFFTError) as e:
         SCFTEnsemble([ctx])
     assert e.value.code == E
+
+
+def test_tuner_searches_axis_split():
+    """tune_schedule(axis=True) times the per-axis chain from the best tile
+    and graph schedule where the plan takes it (Im-3m 64^3) and skips it
+    where the plan refuses it (Ia-3d 64^3, no separable multiply)."""
+    from cfft_hip.tune import tune_schedule
+    lat = 4.0 * np.eye(3)
+    for sg, takes in ((229, True), (230, False)):
+        N = (64, 64, 64)
+        _, ops_s = cf.find_optimal_shift(cf.get_ops(sg, 3, N), N)
+        best, times = tune_schedule(N, ops_s, lat, 20, cf.FP64, 2, cf.PATH_QS, reps=2, axis=True)
+        print(f"  sg={sg}", best, {k: round(v, 2) for k, v in times.items()}, flush=True)
+        assert cf._lib.OPT_AXIS_SPLIT in best
+        assert any("_axis" in k for k in times) == takes
+        if not takes:
+            assert best[cf._lib.OPT_AXIS_SPLIT] == 0
