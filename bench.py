@@ -332,13 +332,16 @@ def scft_ia3d(cf, tol64, tol32, max_iter, cell=3.8):
     }
 
 
-def batched_fields(cf, torch, ops_s, wA, prec, e, batch, steps, path):
+def batched_fields(cf, torch, ops_s, wA, prec, e, batch, steps, path, options=None):
     """Field-steps/s and the dominant kernel's bandwidth when one plan carries
     `batch` Im-3m 128^3 fields (the same workload's q, replicated): the same
-    kernels once there is enough work per launch to leave the latency floor."""
+    kernels (and plan options) once there is enough work per launch to leave
+    the latency floor."""
     import ctypes as C
     lat = CELL * np.eye(3)
     plan = cf.DevicePlan(N, ops_s, 1.0 / NS, lat, path, prec, batch=batch)
+    for k, v in (options or {}).items():
+        plan.set_option(k, v)
     M3 = int(np.prod(plan.M))
     dt = torch.float64 if prec == cf.FP64 else torch.float32
     w = torch.from_numpy(np.asarray(wA, dtype=np.float64).ravel(order="F")).to("cuda")
@@ -628,6 +631,9 @@ def main():
     ap.add_argument("--ensemble-iters", type=int, default=0,
                     help="profiling aid: only build the 64-instance batch as one ensemble and run this many "
                          "iterations in FP64, then in mixed precision, from one host thread")
+    ap.add_argument("--strategy", default=None, choices=sorted(STRATEGY_PATH),
+                    help="profiling aid: use this launch strategy for the timed region instead of the fastest "
+                         "(under a kernel tracer the strategy trials and the tuner are distorted)")
     ap.add_argument("--only-families", action="store_true",
                     help="profiling aid: only time the sparse-stage KRFFT families on Fm-3m 128^3")
     args = ap.parse_args()
@@ -687,7 +693,7 @@ def main():
                     sw.plan.set_option(k, v)
             trial[label] = time_sweeps(sw, 3, 1, torch)
             sweeps[label] = sw
-        best = min(trial, key=trial.get)
+        best = args.strategy if args.strategy else min(trial, key=trial.get)
         sw = sweeps[best]
         for label in list(sweeps):
             if label != best:
@@ -728,7 +734,10 @@ def main():
         res[name] = dict(el=el, med=med, Qq=Qq, Qd=Qd, q=qfields, stages=stages, dom=dom, avg_ms=avg_ms,
                          achieved=achieved, plan=sw.plan, sbytes=nb, chain=chain,
                          step_bytes=step_bytes(sw.plan, e),
-                         axis=bool(sw.plan.options.get(cf._lib.OPT_AXIS_SPLIT)))
+                         axis=bool(sw.plan.options.get(cf._lib.OPT_AXIS_SPLIT)),
+                         options={k: v for k, v in sw.plan.options.items()
+                                  if k in (cf._lib.OPT_AXIS_SPLIT, cf._lib.OPT_FFT_LANES,
+                                           cf._lib.OPT_PAIR_THREADS)})
         del sw
 
     # ---- many fields per launch (bandwidth regime of the same kernels) ----
@@ -737,7 +746,7 @@ def main():
         wide = {}
         for name, prec, e in (("fp64", cf.FP64, 8), ("mixed", cf.MIXED, 4)):
             wide[name] = batched_fields(cf, torch, ops_s, wA, prec, e, args.wide, 50,
-                                        STRATEGY_PATH[variants[name]["used"]])
+                                        STRATEGY_PATH[variants[name]["used"]], res[name]["options"])
 
     # ---- the Pmmm 64^3 configuration (one field) ----
     pmmm = config_pmmm_64(cf, torch)
