@@ -519,7 +519,8 @@ def config_pmmm_64(cf, torch):
     """Pmmm (#47) 64^3, cell 4.0, ds 0.01, one field (the reference's
     CPU-runnable configuration): device-resident forward and backward KRFFT
     (us per call, 200 calls) and 100-step sweeps (field-steps/s, best of 5) on
-    each path that applies, FP64; w = 5 cos cos cos symmetrised (no noise)."""
+    each path that applies (path QS also as the per-axis chain), FP64;
+    w = 5 cos cos cos symmetrised (no noise)."""
     from cfft_hip.scft import symmetrize_subgrid
     n3 = (64, 64, 64)
     ops = cf.get_ops(47, 3, n3)
@@ -531,8 +532,10 @@ def config_pmmm_64(cf, torch):
     w = symmetrize_subgrid(5.0 * np.cos(X) * np.cos(Y) * np.cos(Z), ops_s, Lv)
     s = torch.cuda.current_stream().cuda_stream
     out = {}
-    for name, path in (("m2", cf.PATH_M2), ("qs_chain", cf.PATH_QS)):
+    for name, path, axis in (("m2", cf.PATH_M2, 0), ("qs_chain", cf.PATH_QS, 0), ("qs_chain_axis", cf.PATH_QS, 1)):
         plan = cf.DevicePlan(n3, ops_s, 0.01, lat, path)
+        if axis:  # the per-axis chain (H = 16 here)
+            plan.set_option(cf._lib.OPT_AXIS_SPLIT, 1)
         M3 = int(np.prod(plan.M))
         wd = torch.from_numpy(np.asarray(w, dtype=np.float64).ravel(order="F")).cuda()
         E = torch.exp(-wd * 0.005)

@@ -52,10 +52,12 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
 
-def _sweep(case, prec, path):
+def _sweep(case, prec, path, axis=0):
     import torch
     from cfft_hip.scft import PropagatorSweep
     sw = PropagatorSweep(case["N"], case["ops_s"], 4.0 * np.eye(3), 0.3, 100, prec, path=path)
+    if axis:
+        sw.plan.set_option(cf._lib.OPT_AXIS_SPLIT, 1)
     sw.set_fields(case["wA"], case["wB"])
     sw.reset()
     sw.run()
@@ -63,32 +65,38 @@ def _sweep(case, prec, path):
     return sw.fields()
 
 
-def test_im3m128_plane_pair_sweep_fp64_vs_c_oracle(im3m128):
-    """100-step q and q+ sweep on path QS (the timed chain, H = 32) vs the C
-    restatement of the step: 1e-12 relative."""
-    print("Im-3m 128^3 QS sweep FP64", flush=True)
-    q, qd = _sweep(im3m128, cf.FP64, cf.PATH_QS)
+@pytest.mark.parametrize("axis", [0, 1])
+def test_im3m128_plane_pair_sweep_fp64_vs_c_oracle(im3m128, axis):
+    """100-step q and q+ sweep on path QS (the timed chain, H = 32; plane
+    pairs, and the per-axis chain the tuned bench runs) vs the C restatement
+    of the step: 1e-12 relative."""
+    print(f"Im-3m 128^3 QS sweep FP64 axis={axis}", flush=True)
+    q, qd = _sweep(im3m128, cf.FP64, cf.PATH_QS, axis)
     ref = im3m128["ref"]
     assert _rel(q, ref["q"]) < 1e-12
     assert _rel(qd, ref["qd"]) < 1e-12
 
 
-def test_im3m128_plane_pair_sweep_mixed_vs_fp64(im3m128):
-    print("Im-3m 128^3 QS sweep mixed", flush=True)
-    q, qd = _sweep(im3m128, cf.MIXED, cf.PATH_QS)
+@pytest.mark.parametrize("axis", [0, 1])
+def test_im3m128_plane_pair_sweep_mixed_vs_fp64(im3m128, axis):
+    print(f"Im-3m 128^3 QS sweep mixed axis={axis}", flush=True)
+    q, qd = _sweep(im3m128, cf.MIXED, cf.PATH_QS, axis)
     ref = im3m128["ref"]
     assert _rel(q, ref["q"]) < 2e-5
     assert _rel(qd, ref["qd"]) < 2e-5
 
 
-@pytest.mark.parametrize("prec,path", [(cf.FP64, cf.PATH_QS), (cf.MIXED, cf.PATH_QS), (cf.MIXED, cf.PATH_AUTO)])
-def test_im3m128_first_scft_iteration(im3m128, prec, path):
+@pytest.mark.parametrize("prec,path,axis", [(cf.FP64, cf.PATH_QS, False), (cf.FP64, cf.PATH_QS, True),
+                                            (cf.MIXED, cf.PATH_QS, False), (cf.MIXED, cf.PATH_AUTO, False)])
+def test_im3m128_first_scft_iteration(im3m128, prec, path, axis):
     """First SCFT iteration at the timed workload: Q, F, phi_A, phi_B against
-    the C oracle, 1e-12 (FP64) / 2e-5 relative (mixed)."""
+    the C oracle, 1e-12 (FP64) / 2e-5 relative (mixed); path QS also as the
+    per-axis chain (the bench's SCFT solve)."""
     from cfft_hip.scft import SCFTSolver
-    print(f"Im-3m 128^3 first SCFT iteration prec={prec} path={path}", flush=True)
+    print(f"Im-3m 128^3 first SCFT iteration prec={prec} path={path} axis={axis}", flush=True)
     tol = 1e-12 if prec == cf.FP64 else 2e-5
-    s = SCFTSolver(im3m128["N"], im3m128["ops_s"], 4.0 * np.eye(3), 0.3, 20.0, 100, prec, path=path)
+    s = SCFTSolver(im3m128["N"], im3m128["ops_s"], 4.0 * np.eye(3), 0.3, 20.0, 100, prec, path=path,
+                   axis_split=axis)
     s.set_fields(im3m128["wA"], im3m128["wB"])
     g = s.iterate()
     ref = im3m128["ref"]

@@ -327,22 +327,25 @@ def test_full_size_im3m_128_properties():
     assert rel(p.diffuse(f0), p2.diffuse(f0)) < TOL64
 
 
-@pytest.mark.parametrize("path", [cf.PATH_M2, cf.PATH_QS])
-def test_config_pmmm_64(path):
+@pytest.mark.parametrize("path,axis", [(cf.PATH_M2, 0), (cf.PATH_QS, 0), (cf.PATH_QS, 1)])
+def test_config_pmmm_64(path, axis):
     """Pmmm (#47) 64^3, cell 4.0, ds 0.01 (the reference's CPU-runnable
     configuration) at its full size: u ~ U(-1, 1) symmetrised; forward KRFFT
     at the 32768 spectral-ASU points against the full-grid FFT and the backward
     round trip (path M2; the plane-pair chain has no backward); then 100
     Strang steps from q = 1 with w = 5 cos(2 pi x/a) cos(2 pi y/a)
     cos(2 pi z/a) + N(0, 0.1) symmetrised, against the same steps on the full
-    grid (NumPy FFT, K, IFFT between the two half-step factors)."""
+    grid (NumPy FFT, K, IFFT between the two half-step factors); path QS also
+    as the per-axis chain (CFFT_OPT_AXIS_SPLIT)."""
     N = (64, 64, 64)
-    print(f"Pmmm 64^3 path={path}", flush=True)
+    print(f"Pmmm 64^3 path={path} axis={axis}", flush=True)
     ops, ops_s, shift, R, t = group_setup(47, N)
     Lv = O.auto_L(t)
     u = O.make_symmetric(R, t, N, seed=47, dist="uniform")
     f0 = O.extract_subgrid(u, Lv)
     plan = cf.DevicePlan(N, ops_s, DS, LAT4, path)
+    if axis:
+        plan.set_option(cf._lib.OPT_AXIS_SPLIT, 1)
     spec = O.calc_spectral_asu(R, t, N)
     assert spec.idx.shape[0] == 32768
     F = plan.forward(f0)
