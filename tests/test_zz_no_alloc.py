@@ -62,9 +62,13 @@ def test_noalloc_matches_diffuse_dev(first, second):
         assert not np.array_equal(a, x.cpu().numpy())  # the operator ran
 
 
-def test_noalloc_batched_qs():
-    print("no-alloc diffuse: Im-3m 128^3 QS, 2 fields", flush=True)
+@pytest.mark.parametrize("axis", [0, 1])
+def test_noalloc_batched_qs(axis):
+    """Path QS, 2 fields at Im-3m 128^3, with the plane-pair chain and with
+    the per-axis chain (CFFT_OPT_AXIS_SPLIT)."""
+    print(f"no-alloc diffuse: Im-3m 128^3 QS, 2 fields, axis={axis}", flush=True)
     plan, x = _case(229, 128, cf.PATH_QS, cf.FP64, 21, batch=2)
+    plan.set_option(cf._lib.OPT_AXIS_SPLIT, axis)
     a, b = _run_both(plan, x)
     assert np.array_equal(a, b)
 
