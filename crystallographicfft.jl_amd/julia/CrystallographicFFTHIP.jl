@@ -48,6 +48,9 @@ const libcfft = get(ENV, "CFFT_HIP_LIB", joinpath(@__DIR__, "..", "libcfft_hip.s
 
 const PATH_AUTO, PATH_M2, PATH_M7, PATH_M2Q, PATH_M7Q, PATH_QR, PATH_QS = Int32.(0:6)
 const FP64, MIXED = Int32(0), Int32(1)
+# plan options (cfft_plan_set_option); no counterpart in the reference
+const OPT_PERSISTENT, OPT_FUSE, OPT_GRAPH, OPT_TILE_ROWS, OPT_FP32_TWIDDLES, OPT_FFT_LANES,
+      OPT_PAIR_THREADS, OPT_AXIS_SPLIT = Int32.(1:8)
 
 struct CFFTError <: Exception
     code::Int32
@@ -109,6 +112,12 @@ function DevicePlan(N::NTuple{3,Int}, ops_shifted, ds::Float64, lattice::Abstrac
     end
     return p
 end
+
+# Launch schedule of a plan (e.g. set_option!(dev, OPT_AXIS_SPLIT, 1): the per-axis chain
+# of path QS); results do not depend on it beyond rounding.
+set_option!(dev::DevicePlan, key::Integer, value::Integer) =
+    check(ccall((:cfft_plan_set_option, libcfft), Cint, (Ptr{Cvoid}, Int32, Int64),
+                dev.handle, Int32(key), Int64(value)))
 
 _ops_for(N, sg_num, dim) = find_optimal_shift(get_ops(sg_num, dim, N), N)[2]
 
