@@ -114,17 +114,29 @@ def tune_schedule(N, ops_shifted, lattice, ns, precision, batch, path, steps=Non
         t = timed(opt)
         if best_t is None or t < best_t:
             best, best_t = opt, t
-    for k, v in extra_dimensions(plan, lanes, axis):
-        opt = dict(best)
-        opt[k] = v
+
+    def trial(opt):
+        nonlocal best, best_t
         try:
             t = timed(opt)
         except L.CFFTError as e:
             if e.code != L.CFFT_ERR_UNSUPPORTED:
                 raise
-            continue
+            return False
         if t < best_t:
             best, best_t = opt, t
+        return True
+
+    grid_best = dict(best)
+    for k, v in extra_dimensions(plan, lanes, axis):
+        opt = dict(best)
+        opt[k] = v
+        if trial(opt) and k == L.OPT_AXIS_SPLIT:
+            # the per-axis chain's kernels cut the tiles differently: every
+            # tile height with the best graph setting
+            for r in sorted({c[L.OPT_TILE_ROWS] for c in schedule_candidates(plan.H[1])}):
+                if r != grid_best[L.OPT_TILE_ROWS]:
+                    trial({**grid_best, L.OPT_TILE_ROWS: r, L.OPT_AXIS_SPLIT: 1})
     del plan
     _CACHE[key] = (best, times)
     return best, times
