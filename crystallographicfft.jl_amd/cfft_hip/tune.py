@@ -38,13 +38,13 @@ def schedule_candidates(H1):
 
 def extra_dimensions(plan, lanes, axis=False):
     """Further schedule dimensions of a plan on path QS: with `axis`, the
-    per-axis chain (H = 16, 32; plans without the separable multiply refuse
-    it and keep the plane pairs); the plane-pair workgroup size (512 threads
+    per-axis chain (H = 8, 12, 16, 24, 32; plans without the separable
+    multiply refuse it and keep the plane pairs); the plane-pair workgroup size (512 threads
     for H = 16, 24, 32) and, with `lanes`, the lane split R1 = 2 of the
     transforms (H = 16, 32)."""
     out = []
     if plan.path == L.PATH_QS:
-        if axis and plan.H[0] in (16, 32):
+        if axis and plan.H[0] in (8, 12, 16, 24, 32):
             out.append((L.OPT_AXIS_SPLIT, 1))
         if plan.H[1] in (16, 24, 32):
             out.append((L.OPT_PAIR_THREADS, 512))

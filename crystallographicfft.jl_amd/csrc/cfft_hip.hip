@@ -882,13 +882,16 @@ void launch_p51(cfft_plan* p, const PipeP& pp, const PipeBufs& B, int step, hipS
   launch_check<T>("k_p51");
 }
 
-// The axis-split chain (CFFT_OPT_AXIS_SPLIT): H = 16 or 32, FP64 factor and
-// twiddle products in both precisions.
+// The axis-split chain (CFFT_OPT_AXIS_SPLIT): H = 8, 12, 16, 24 or 32, FP64
+// factor and twiddle products in both precisions.
 #define CFFT_AX_SWITCH(len, MACRO) \
   switch (len) {                   \
+    MACRO(8)                       \
+    MACRO(12)                      \
     MACRO(16)                      \
+    MACRO(24)                      \
     MACRO(32)                      \
-    default: throw Error{CFFT_ERR_UNSUPPORTED, "the axis-split chain needs H = 16 or 32"}; \
+    default: throw Error{CFFT_ERR_UNSUPPORTED, "the axis-split chain needs H = 8, 12, 16, 24 or 32"}; \
   }
 
 template <typename T>
@@ -1939,10 +1942,12 @@ int cfft_plan_set_option(cfft_plan* p, int32_t key, int64_t value) {
         break;
       case CFFT_OPT_AXIS_SPLIT:
         require(value == 0 || (value == 1 && is_qs(p) && p->qs_sep && cubic(p) &&
-                               (p->H[0] == 16 || p->H[0] == 32) && !p->arith32 && !p->lanes),
+                               (p->H[0] == 8 || p->H[0] == 12 || p->H[0] == 16 || p->H[0] == 24 ||
+                                p->H[0] == 32) &&
+                               !p->arith32 && !p->lanes),
                 CFFT_ERR_UNSUPPORTED,
-                "the axis-split chain needs path QS, a separable multiply, H = 16 or 32, FP64 twiddle "
-                "products and the default lane split");
+                "the axis-split chain needs path QS, a separable multiply, H = 8, 12, 16, 24 or 32, FP64 "
+                "twiddle products and the default lane split");
         p->axis_split = value != 0;
         break;
       default: throw Error{CFFT_ERR_INVALID, "unknown option " + std::to_string(key)};

@@ -295,9 +295,10 @@ int cfft_sweep_hist_dev(cfft_plan* plan, void* hist, const void* const* E1, cons
  * factors into one 2 x 2 mixing per axis; 1 applies the x mixing with the x
  * transforms in the unfold/fold tile kernel and the y and z mixings in a
  * kernel over single x-planes (twice the work items of the plane pairs),
- * with 32-point lines over eight lanes.  H = 16 or 32, FP64 twiddle products,
- * default lane split; results differ from the default chain by rounding
- * only.  Ensembles refuse such plans.  Otherwise CFFT_ERR_UNSUPPORTED. */
+ * with 32-point lines over eight lanes (other lengths: the default lane
+ * split).  H = 8, 12, 16, 24 or 32, FP64 twiddle products, default lane
+ * split; results differ from the default chain by rounding only.  Ensembles
+ * refuse such plans.  Otherwise CFFT_ERR_UNSUPPORTED. */
 #define CFFT_OPT_AXIS_SPLIT 8
 /* No counterpart in the reference. */
 int cfft_plan_set_option(cfft_plan* plan, int32_t key, int64_t value);

@@ -38,12 +38,14 @@ def _sweep(sg, N, prec, axis, ns=20, graph=1):
     return [np.asarray(x, dtype=np.float64).copy() for x in sw.fields()]
 
 
-@pytest.mark.parametrize("sg,n", [(229, 64), (229, 128), (47, 64), (47, 128), (223, 64)])
+@pytest.mark.parametrize("sg,n", [(229, 64), (229, 128), (47, 64), (47, 128), (223, 64), (229, 32), (229, 48),
+                                  (229, 96), (47, 48)])
 @pytest.mark.parametrize("prec,tol", [(cf.FP64, 1e-12), (cf.MIXED, 2e-6)])
 def test_axis_split_sweep_matches_plane_pairs(sg, n, prec, tol):
     """20-step q and q+ sweep, per-axis chain against the plane-pair chain:
-    channel length H = 16 at 64^3, 32 at 128^3; four channels (Im-3m, Pm-3n)
-    or eight (Pmmm: 64 tile rows, two passes of the row transforms)."""
+    channel length H = N / 4 (8 at 32^3 to 32 at 128^3; lengths other than
+    32 on the default lane split); four channels (Im-3m, Pm-3n) or eight
+    (Pmmm: 64 tile rows, two passes of the row transforms)."""
     print(f"axis split sg={sg} N={n} prec={prec}", flush=True)
     N = (n, n, n)
     a = _sweep(sg, N, prec, 0)
@@ -99,9 +101,9 @@ def test_axis_split_scft_iteration(prec, tol):
 
 
 def test_axis_split_refused():
-    """Off path QS, at H = 24, without the separable multiply (Ia-3d), with
-    the R1 = 2 lane split or FP32 twiddle products set, and for an ensemble
-    of such a plan: CFFT_ERR_UNSUPPORTED."""
+    """Off path QS, without the separable multiply (Ia-3d), with the R1 = 2
+    lane split or FP32 twiddle products set, and for an ensemble of such a
+    plan: CFFT_ERR_UNSUPPORTED."""
     from cfft_hip.scft import SCFTBatch, SCFTEnsemble
     lat = 4.0 * np.eye(3)
     E = cf._lib.CFFT_ERR_UNSUPPORTED
@@ -111,7 +113,7 @@ def test_axis_split_refused():
         _, ops_s = cf.find_optimal_shift(cf.get_ops(sg, 3, N), N)
         return cf.DevicePlan(N, ops_s, 0.01, lat, path, prec)
 
-    for p in (plan(64, cf.PATH_M7), plan(96), plan(64, sg=230)):
+    for p in (plan(64, cf.PATH_M7), plan(64, sg=230)):
         with pytest.raises(cf.CFFTError) as e:
             p.set_option(cf._lib.OPT_AXIS_SPLIT, 1)
         assert e.value.code == E
