@@ -39,13 +39,14 @@ def _sweep(sg, N, prec, axis, ns=20, graph=1):
 
 
 @pytest.mark.parametrize("sg,n", [(229, 64), (229, 128), (47, 64), (47, 128), (223, 64), (229, 32), (229, 48),
-                                  (229, 96), (47, 48)])
+                                  (229, 96), (47, 48), (47, 32), (47, 96), (223, 96)])
 @pytest.mark.parametrize("prec,tol", [(cf.FP64, 1e-12), (cf.MIXED, 2e-6)])
 def test_axis_split_sweep_matches_plane_pairs(sg, n, prec, tol):
     """20-step q and q+ sweep, per-axis chain against the plane-pair chain:
     channel length H = N / 4 (8 at 32^3 to 32 at 128^3; lengths other than
     32 on the default lane split); four channels (Im-3m, Pm-3n) or eight
-    (Pmmm: 64 tile rows, two passes of the row transforms)."""
+    (Pmmm: 64 tile rows, two passes of the row transforms; at 32^3 eight
+    channels of 8-row tiles fill the tile's 64 LDS rows)."""
     print(f"axis split sg={sg} N={n} prec={prec}", flush=True)
     N = (n, n, n)
     a = _sweep(sg, N, prec, 0)
